@@ -1,0 +1,275 @@
+/*
+ * mdy.h -- C ABI of libmdy_hip.so, the MI355X (gfx950) implementation of
+ * MEDYAN's mechanical-equilibration path: force-field energies/forces,
+ * cylinder neighbor lists and the Polak-Ribiere conjugate-gradient minimizer.
+ *
+ * Conventions
+ *   - Every entry point returns MDY_OK (0) or a negative MDY_ERR_* code;
+ *     mdy_last_error() gives a message. No C++ exception crosses the ABI.
+ *   - Host pointers are owned by the caller. A context owns its device memory.
+ *   - Coordinates and forces are AoS doubles, bead b at [3b, 3b+1, 3b+2],
+ *     exactly the layout of MEDYAN's serializeDof
+ *     (src/Structure/DofSerializer.hpp:64-187).
+ *   - Interaction tables use MEDYAN's vectorized layout: a beadSet entry is a
+ *     coordinate OFFSET (3 * bead index), as written by each force field's
+ *     vectorize() (e.g. src/Mechanics/ForceField/Filament/FilamentStretching.cpp:35-36).
+ *   - One context = one HIP device + one stream. Not reentrant: calls on one
+ *     context must come from one thread at a time (MEDYAN's caller is a single
+ *     thread, src/Controller/MController.h:109-211).
+ */
+#ifndef MDY_H
+#define MDY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDY_ABI_VERSION 1
+
+enum {
+    MDY_OK = 0,
+    MDY_ERR_ARG = -1,        /* invalid argument or size */
+    MDY_ERR_HIP = -2,        /* HIP runtime error */
+    MDY_ERR_STATE = -3,      /* call out of order (e.g. no network set) */
+    MDY_ERR_NONFINITE = -4,  /* max force is not finite (CGMethod::maxF throws) */
+    MDY_ERR_CAPACITY = -5,   /* caller buffer too small */
+    MDY_ERR_NOGPU = -6,      /* no HIP device */
+    MDY_ERR_LAUNCH = -7,     /* cooperative launch not possible */
+    MDY_ERR_FORCE = -8,      /* excluded-volume force stayed non-finite after the
+                                quadrature fallback (reference exits) */
+    MDY_ERR_RECOVERY = -9,   /* recovery move had infinite step (reference throws) */
+    MDY_ERR_TIMEOUT = -10    /* device-side grid barrier watchdog fired */
+};
+
+/* Force-field slots, in the order MController::initializeFF registers them
+ * (src/Controller/MController.cpp:65-188). Energies are summed in this order. */
+enum {
+    MDY_FF_FS = 0, /* FilamentStretching<Harmonic>           */
+    MDY_FF_FB,     /* FilamentBending<Cosine>                */
+    MDY_FF_LS,     /* LinkerStretching<Harmonic>             */
+    MDY_FF_MS,     /* MotorGhostStretching<Harmonic>         */
+    MDY_FF_BS,     /* BranchingStretching<Harmonic>          */
+    MDY_FF_BB,     /* BranchingBending<Cosine>               */
+    MDY_FF_EV,     /* CylinderExclVolume<CylinderExclVolRepulsion> */
+    MDY_FF_BD,     /* BoundaryCylinderRepulsion<Exp>         */
+    MDY_NFF
+};
+
+/* Cylinder table: what HybridCylinderCylinderNL / BoundaryCylinderNL read from
+ * the cylinder database (src/Structure/Cylinder.cpp:30-49). */
+typedef struct mdy_cylinders {
+    int32_t n;
+    const int32_t *bead1;        /* coord offset of getFirstBead()                  */
+    const int32_t *bead2;        /* coord offset of getSecondBead()                 */
+    const int64_t *id;           /* getId(): half list keeps pairs with id(c) > id(n) */
+    const int32_t *filament;     /* parent filament id                              */
+    const int32_t *position;     /* position on filament                            */
+    const int16_t *type;         /* filament type                                   */
+    const double  *eq_length;    /* MCylinder::getEqLength                          */
+    const double  *exvol_k;      /* MCylinder::getExVolConst                        */
+    const int32_t *branch_child; /* index of getBranchingCylinder() or -1           */
+    const uint8_t *minus_end;    /* isMinusEnd()                                    */
+} mdy_cylinders;
+
+/* Vectorized bonded/cross-link tables (each FF's vectorize()). */
+typedef struct mdy_ff_tables {
+    /* FilamentStretching::vectorize  (FilamentStretching.cpp:26-46) */
+    int32_t fs_n; const int32_t *fs_beadset; const double *fs_k; const double *fs_l0;
+    /* FilamentBending::vectorize     (FilamentBending.cpp:30-60) */
+    int32_t fb_n; const int32_t *fb_beadset; const double *fb_k; const double *fb_theta0;
+    /* LinkerStretching::vectorize    (LinkerStretching.cpp:30-55) */
+    int32_t ls_n; const int32_t *ls_beadset; const double *ls_k; const double *ls_l0;
+    const double *ls_pos1; const double *ls_pos2;
+    /* MotorGhostStretching::vectorize (MotorGhostStretching.cpp:30-60) */
+    int32_t ms_n; const int32_t *ms_beadset; const double *ms_k; const double *ms_l0;
+    const double *ms_pos1; const double *ms_pos2;
+    /* BranchingStretching::vectorize (BranchingStretching.cpp:29-50) */
+    int32_t bs_n; const int32_t *bs_beadset; const double *bs_k; const double *bs_l0;
+    const double *bs_pos;
+    /* BranchingBending::vectorize    (BranchingBending.cpp:29-48) */
+    int32_t bb_n; const int32_t *bb_beadset; const double *bb_k; const double *bb_theta0;
+} mdy_ff_tables;
+
+/* Plane boundary elements (PlaneBoundaryElement, src/Structure/BoundaryElementImpl.h:30-119):
+ * plane[4i..4i+3] = (a, b, c, d) of a*x + b*y + c*z + d = 0. */
+typedef struct mdy_boundary {
+    int32_t n_planes;
+    const double *plane;
+    const double *k;       /* getRepulsionConst   */
+    const double *screen;  /* getScreeningLength  */
+    float cutoff;          /* BoundaryCylinderNL rMax (stored as float in the reference) */
+} mdy_boundary;
+
+/* Everything a minimization needs besides coordinates. */
+typedef struct mdy_system {
+    int32_t n_beads;           /* independent DOF = 3 * n_beads */
+    mdy_cylinders cyl;
+    mdy_ff_tables ff;
+    mdy_boundary bnd;
+    float exvol_cutoff;        /* VolumeCutoff, rMax of the hybrid NL (float in the reference) */
+    int32_t exvol_enabled;
+    int32_t boundary_enabled;
+} mdy_system;
+
+/* ConjugateGradientParams (src/Mechanics/Minimizer/CGMethod.hpp:44-61). */
+enum { MDY_CG_STEEPEST = 0, MDY_CG_FLETCHER_RIEVES = 1, MDY_CG_POLAK_RIBIERE = 2 };
+typedef struct mdy_cg_params {
+    int32_t method;               /* MDY_CG_*                                          */
+    double  max_dist;             /* MAXDISTANCE                                       */
+    double  lambda_max;           /* LAMBDAMAX                                         */
+    double  grad_tol;             /* GRADIENTTOLERANCE (component-wise infinity norm)  */
+    double  energy_rel_tol;       /* energy-change-relative-tolerance (default 1e-9)   */
+    int32_t energy_call_limit;    /* 0 = no limit                                      */
+    int32_t force_call_limit;     /* effective limit is max(this, 3 * n_beads_limit)   */
+    int32_t n_beads_limit;        /* Bead::numBeads() used by the call limit           */
+    int32_t try_to_recover;       /* tryToRecoverInLineSearchError                     */
+} mdy_cg_params;
+
+/* MinimizationResult (src/Mechanics/Minimizer/MinimizationTypes.hpp:10-64). */
+#define MDY_ERROR_LINESEARCH 1u
+typedef struct mdy_min_result {
+    double   energies_before[MDY_NFF + 1]; /* per FF, then total */
+    double   energies_after[MDY_NFF + 1];
+    uint32_t error_flags;
+    int32_t  num_energy_call;
+    int32_t  energy_call_limit;
+    int32_t  num_force_call;
+    int32_t  force_call_limit;
+    int32_t  num_iter;
+    double   grad_inf_norm;
+    double   grad_tol;
+    double   energy_rel_change;
+    double   energy_rel_tol;
+    /* additions of this implementation */
+    int32_t  num_energy_eval;       /* energy evaluations actually performed (E(x) is cached) */
+    int32_t  num_safe_mode;         /* times safe mode was entered                         */
+    int32_t  num_recovery;          /* recovery moves                                      */
+    int32_t  status;                /* MDY_OK or the error that stopped the minimization   */
+} mdy_min_result;
+
+/* ------------------------------------------------------------------------- */
+/* Context                                                                    */
+/* ------------------------------------------------------------------------- */
+enum { MDY_PREC_FP64 = 0, MDY_PREC_MIXED = 1 };
+enum { MDY_COPLANAR_REFERENCE = 0, MDY_COPLANAR_EXACT = 1 };
+
+typedef struct mdy_opts {
+    int32_t precision;       /* MDY_PREC_*                                                 */
+    int32_t coplanar_mode;   /* MDY_COPLANAR_REFERENCE: perturb c2 like the reference force */
+    int32_t block_size;      /* threads per block of the persistent CG kernel (0 = auto)   */
+    int32_t blocks_per_cu;   /* resident blocks per CU for the persistent kernel (0 = max) */
+} mdy_opts;
+
+typedef struct mdy_ctx mdy_ctx;
+
+int         mdy_abi_version(void);
+int         mdy_device_count(void);
+void        mdy_default_opts(mdy_opts *opts);
+int         mdy_create(mdy_ctx **out, int device, const mdy_opts *opts);
+void        mdy_destroy(mdy_ctx *ctx);
+const char *mdy_last_error(const mdy_ctx *ctx);
+int         mdy_set_opts(mdy_ctx *ctx, const mdy_opts *opts);
+
+/* Upload a system (tables, cylinders, boundary). Replaces ForceFieldManager::
+ * vectorizeAllForceFields for the bonded / cross-link force fields
+ * (src/Mechanics/ForceField/ForceFieldManager.cpp:35-93). */
+int mdy_set_network(mdy_ctx *ctx, const mdy_system *sys);
+
+/* Neighbor-list rebuild + excluded-volume / boundary vectorization from the
+ * given coordinates: HybridCylinderCylinderNL::reset (HybridNeighborListImpl.cpp:527-562),
+ * BoundaryCylinderNL::reset (NeighborListImpl.cpp:801-809), then
+ * CylinderExclVolume::vectorize (CylinderExclVolume.cpp:34-179) and
+ * BoundaryCylinderRepulsion::vectorize (BoundaryCylinderRepulsion.cpp:33-88). */
+int mdy_build_neighbors(mdy_ctx *ctx, const double *coord);
+/* Same from device-resident coordinates. */
+int mdy_build_neighbors_dev(mdy_ctx *ctx, const double *d_coord);
+/* Cylinder-cylinder pairs (index of c, index of neighbor n, id(c) > id(n)),
+ * sorted by (c, n); mother-daughter pairs already dropped. */
+int mdy_get_pairs(mdy_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n);
+/* Boundary interaction table: per-plane counts and bead offsets (reference order). */
+int mdy_get_boundary_table(mdy_ctx *ctx, int32_t *nneighbors, int32_t *beadset, int64_t cap, int64_t *n);
+
+/* ForceFieldManager::computeEnergy (ForceFieldManager.cpp:96-207): *e_total is
+ * +INFINITY if a force field returned the -1 sentinel. e_per_ff (nullable)
+ * receives MDY_NFF individual energies (computeIndividualEnergies). */
+int mdy_energy(mdy_ctx *ctx, const double *coord, double *e_total, double *e_per_ff);
+/* ForceFieldManager::computeForces (ForceFieldManager.cpp:242-290) but with
+ * ForceField::computeForces semantics: forces are ADDED to force_accum. */
+int mdy_forces(mdy_ctx *ctx, const double *coord, double *force_accum);
+
+/* Device-pointer forms on a caller stream (hipStream_t passed as void*; NULL =
+ * the context stream). d_out receives MDY_NFF energies then the total. */
+int mdy_energy_dev(mdy_ctx *ctx, const double *d_coord, double *d_out, void *stream);
+int mdy_forces_dev(mdy_ctx *ctx, const double *d_coord, double *d_force, int accumulate, void *stream);
+/* Hessian-vector product H v = d/dt grad E(x + t v) at t = 0 of the
+ * exact-gradient force field (no coplanar perturbation); writes H v. */
+int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *d_hv, void *stream);
+
+/* CGMethod::minimize<method> with backtracking line search
+ * (src/Mechanics/Minimizer/CGMethod.hpp:826-1743), run on the device without
+ * returning to the host. coord_inout: 3*n_beads doubles (host). force_out
+ * (nullable) receives the final forces (ptrOutForce). */
+int mdy_minimize(mdy_ctx *ctx, const mdy_cg_params *params, double *coord_inout,
+                 double *force_out, mdy_min_result *result);
+/* Same, coordinates already on the device (d_coord updated in place). */
+int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *params, double *d_coord,
+                     double *d_force_out, mdy_min_result *result);
+
+/* Per-iteration trace of the last minimization (lambda, energy after the line
+ * search, max force, beta, safe-mode flag), for parity checks. */
+int mdy_get_trace(mdy_ctx *ctx, double *lambda, double *energy, double *maxf,
+                  double *beta, int32_t *safe, int64_t cap, int64_t *n);
+int mdy_set_trace(mdy_ctx *ctx, int64_t cap);
+
+/* Timing of the last minimization (device events), milliseconds. */
+int mdy_last_timing(mdy_ctx *ctx, double *ms_total);
+/* Phase profile of the last minimization: the device timestamp (100 MHz
+ * wall clock) at the end of each grid-wide phase and the phase kind
+ * (1 energy, 2 energy fallback, 3 forces, 4 force fallback, 5 gather,
+ * 6 reduction, 7 restore). Recording is off unless a capacity is set. */
+int mdy_set_profile(mdy_ctx *ctx, int64_t cap);
+int mdy_get_profile(mdy_ctx *ctx, int32_t *kind, int64_t *ticks, int64_t cap, int64_t *n);
+
+/* ------------------------------------------------------------------------- */
+/* Seeded synthetic networks (the generator behind the benchmark workloads)   */
+/* ------------------------------------------------------------------------- */
+typedef struct mdy_gen_params {
+    uint64_t seed;
+    int32_t  n_filaments;       /* total filaments, daughters included          */
+    int32_t  cyl_per_filament;
+    double   box;               /* cubic box edge (nm); planes inset 25 nm      */
+    int32_t  n_linkers;
+    int32_t  n_motors;
+    int32_t  n_branches;        /* daughters are the last n_branches filaments  */
+    double   cyl_size;          /* 108 nm                                       */
+    double   jitter;            /* Gaussian bead jitter sigma (nm)              */
+    double   min_sep;           /* rejection: min segment distance at placement */
+    double   pair_cut;          /* linker/motor binding-point distance cap      */
+    double   rest_sigma;        /* rest length = d0 + N(0, rest_sigma)          */
+    double   k_stretch, k_bend, theta_bend;
+    double   k_exvol;  float exvol_cutoff;
+    double   k_bd, screen_bd; float bd_cutoff;
+    double   k_linker, k_motor;
+    double   k_br_stretch, l_br_stretch, k_br_bend, theta_br_bend;
+} mdy_gen_params;
+
+typedef struct mdy_network {
+    mdy_system sys;
+    double    *coord;        /* 3*n_beads */
+    double     box;
+    int32_t    n_filaments;
+    int32_t   *fil_first_bead;
+    int32_t   *fil_ncyl;
+    void      *storage;      /* owned */
+} mdy_network;
+
+void mdy_gen_defaults(mdy_gen_params *p);
+int  mdy_generate(const mdy_gen_params *p, mdy_network **out);
+void mdy_network_free(mdy_network *net);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDY_H */

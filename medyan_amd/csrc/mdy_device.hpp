@@ -1,0 +1,334 @@
+// mdy_device.hpp -- per-interaction energy and force math for gfx950.
+//
+// Each function evaluates ONE interaction of one force field from bead
+// positions already in registers; the kernels decide where positions come
+// from (x, or x + lambda * d during a line search) and where forces go.
+// Expressions keep the operation order of the reference lines they cite so
+// that FP64 results reproduce the CPU path to rounding (built with
+// -ffp-contract=off); transcendental functions are the ROCm device library's.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace mdy {
+
+struct d3 { double x, y, z; };
+
+__device__ __forceinline__ d3 ld3(const double *__restrict__ p) { return d3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ d3 sub(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// std::max / std::min argument order (NaN handling identical to the host code)
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+
+// (v2-v1).(v4-v3), MathFunctions.h:469-476
+__device__ __forceinline__ double sprod(d3 v1, d3 v2, d3 v3, d3 v4) {
+    return ((v2.x - v1.x) * (v4.x - v3.x) + (v2.y - v1.y) * (v4.y - v3.y) + (v2.z - v1.z) * (v4.z - v3.z));
+}
+__device__ __forceinline__ double tpdist(d3 v1, d3 v2) {
+    return sqrt((v2.x - v1.x) * (v2.x - v1.x) + (v2.y - v1.y) * (v2.y - v1.y) + (v2.z - v1.z) * (v2.z - v1.z));
+}
+__device__ __forceinline__ d3 midpt(d3 v1, d3 v2, double alpha) {
+    double beta = 1 - alpha;
+    return d3{(v1.x * beta + alpha * v2.x), (v1.y * beta + alpha * v2.y), (v1.z * beta + alpha * v2.z)};
+}
+__device__ __forceinline__ d3 cross3(d3 v1, d3 v2) {
+    return d3{v1.y * v2.z - v1.z * v2.y, v1.z * v2.x - v1.x * v2.z, v1.x * v2.y - v1.y * v2.x};
+}
+__device__ __forceinline__ d3 normalize3(d3 v) {
+    double x2 = v.x * v.x, y2 = v.y * v.y, z2 = v.z * v.z;
+    double n = sqrt(x2 + y2 + z2);
+    return d3{v.x / n, v.y / n, v.z / n};
+}
+__device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ bool are_equal0(double a) { return fabs(a - 0.0) < 1e-6; }
+
+// ---------------------------------------------------------------------------
+// a3 FilamentStretchingHarmonic (FilamentStretchingHarmonic.cpp:185-275)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double fs_energy(d3 c1, d3 c2, double k, double l0) {
+    double dist = tpdist(c1, c2) - l0;
+    return 0.5 * k * dist * dist;
+}
+// forces on c1, c2
+__device__ __forceinline__ void fs_force(d3 c1, d3 c2, double k, double l0, d3 &f1, d3 &f2) {
+    double dist = tpdist(c1, c2);
+    double invL = 1 / dist;
+    double f0 = k * (dist - l0) * invL;
+    double Fx = -f0 * (c2.x - c1.x), Fy = -f0 * (c2.y - c1.y), Fz = -f0 * (c2.z - c1.z);
+    f2 = d3{Fx, Fy, Fz};
+    f1 = d3{-Fx, -Fy, -Fz};
+}
+
+// ---------------------------------------------------------------------------
+// a4 / a8 cosine bending (FilamentBendingCosine.cpp:199-398, BranchingBendingCosine.cpp:196-411)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cosbend_energy(double l1l2, double L1, double L2, double k, double th0) {
+    double L1L2 = L1 * L2;
+    double x = l1l2 / L1L2;
+    if (x < -1.0) x = -1.0;
+    else if (x > 1.0) x = 1.0;
+    if (are_equal0(th0)) return k * (1 - x);
+    double cosA = x;
+    double sinA = smax(sqrt(1 - cosA * cosA), 0.0);
+    double cosAmB = cosA * cos(th0) + sinA * sin(th0);
+    return k * (1 - cosAmB);
+}
+__device__ __forceinline__ void cosbend_coef(double L1sq, double L2sq, double l1l2, double kb, double th0,
+                                             double &A, double &B, double &C, double &ke) {
+    A = 1 / sqrt(L1sq * L2sq);
+    double x = l1l2 * A;
+    B = x / L1sq;
+    C = x / L2sq;
+    if (are_equal0(th0)) { ke = kb; return; }
+    if (fabs(fabs(x) - 1.0) < 0.001) x = 0.999 * x;
+    if (x < -1.0) x = -1.0;
+    else if (x > 1.0) x = 1.0;
+    double cosp = x;
+    double sinp = sqrt(smax((1 - cosp * cosp), 0.0));
+    double sinpmq = sinp * cos(th0) - cosp * sin(th0);
+    ke = kb * sinpmq / sinp;
+}
+// filament: beads c1, c2, c3; r1 = c2-c1, r2 = c3-c2
+__device__ __forceinline__ double fb_energy(d3 c1, d3 c2, d3 c3, double k, double th0) {
+    double L1 = sqrt(sprod(c1, c2, c1, c2));
+    double L2 = sqrt(sprod(c2, c3, c2, c3));
+    double l1l2 = sprod(c1, c2, c2, c3);
+    return cosbend_energy(l1l2, L1, L2, k, th0);
+}
+__device__ __forceinline__ void fb_force(d3 c1, d3 c2, d3 c3, double k, double th0, d3 &f1, d3 &f2, d3 &f3) {
+    double A, B, C, ke;
+    cosbend_coef(sprod(c1, c2, c1, c2), sprod(c2, c3, c2, c3), sprod(c1, c2, c2, c3), k, th0, A, B, C, ke);
+    d3 r1 = sub(c2, c1), r2 = sub(c3, c2);
+    d3 F1 = d3{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
+    d3 F2 = d3{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
+    f1 = d3{-F1.x, -F1.y, -F1.z};
+    f2 = d3{F1.x - F2.x, F1.y - F2.y, F1.z - F2.z};
+    f3 = F2;
+}
+// branch: mother (c1, c2), daughter (c3, c4)
+__device__ __forceinline__ double bb_energy(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0) {
+    double L1 = sqrt(sprod(c1, c2, c1, c2));
+    double L2 = sqrt(sprod(c3, c4, c3, c4));
+    double l1l2 = sprod(c1, c2, c3, c4);
+    return cosbend_energy(l1l2, L1, L2, k, th0);
+}
+__device__ __forceinline__ void bb_force(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0, d3 &f1, d3 &f2, d3 &f3,
+                                         d3 &f4) {
+    double A, B, C, ke;
+    cosbend_coef(sprod(c1, c2, c1, c2), sprod(c3, c4, c3, c4), sprod(c1, c2, c3, c4), k, th0, A, B, C, ke);
+    d3 r1 = sub(c2, c1), r2 = sub(c4, c3);
+    d3 F1 = d3{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
+    d3 F2 = d3{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
+    f1 = d3{-F1.x, -F1.y, -F1.z};
+    f2 = F1;
+    f3 = d3{-F2.x, -F2.y, -F2.z};
+    f4 = F2;
+}
+
+// ---------------------------------------------------------------------------
+// a5 / a6 linker and motor springs (LinkerStretchingHarmonic.cpp:191-336)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ls_energy(d3 c1, d3 c2, d3 c3, d3 c4, double k, double l0, double p1, double p2) {
+    d3 v1 = midpt(c1, c2, p1), v2 = midpt(c3, c4, p2);
+    double dist = tpdist(v1, v2) - l0;
+    return 0.5 * k * dist * dist;
+}
+__device__ __forceinline__ void ls_force(d3 c1, d3 c2, d3 c3, d3 c4, double k, double l0, double p1, double p2, d3 &f1,
+                                         d3 &f2, d3 &f3, d3 &f4) {
+    d3 v1 = midpt(c1, c2, p1), v2 = midpt(c3, c4, p2);
+    double dist = tpdist(v1, v2);
+    double invL = 1 / dist;
+    double f0 = k * (dist - l0) * invL;
+    d3 Fl = d3{-f0 * (v2.x - v1.x), -f0 * (v2.y - v1.y), -f0 * (v2.z - v1.z)};
+    f1 = d3{-Fl.x * (1 - p1), -Fl.y * (1 - p1), -Fl.z * (1 - p1)};
+    f2 = d3{-Fl.x * (p1), -Fl.y * (p1), -Fl.z * (p1)};
+    f3 = d3{Fl.x * (1 - p2), Fl.y * (1 - p2), Fl.z * (1 - p2)};
+    f4 = d3{Fl.x * (p2), Fl.y * (p2), Fl.z * (p2)};
+}
+
+// ---------------------------------------------------------------------------
+// a7 BranchingStretchingHarmonic (BranchingStretchingHarmonic.cpp:195-318)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bs_energy(d3 c1, d3 c2, d3 c3, double k, double l0, double p) {
+    d3 v1 = midpt(c1, c2, p);
+    double dist = tpdist(v1, c3) - l0;
+    return 0.5 * k * dist * dist;
+}
+__device__ __forceinline__ void bs_force(d3 c1, d3 c2, d3 c3, double k, double l0, double p, d3 &f1, d3 &f2, d3 &f3) {
+    d3 v1 = midpt(c1, c2, p);
+    double dist = tpdist(v1, c3);
+    double invL = 1 / dist;
+    double f0 = k * (dist - l0) * invL;
+    d3 r = sub(c3, v1);
+    f1 = d3{-f0 * (r.x) * (p - 1), -f0 * (r.y) * (p - 1), -f0 * (r.z) * (p - 1)};
+    f2 = d3{f0 * (r.x) * p, f0 * (r.y) * p, f0 * (r.z) * p};
+    f3 = d3{-f0 * (r.x), -f0 * (r.y), -f0 * (r.z)};
+}
+
+// ---------------------------------------------------------------------------
+// a10 BoundaryCylinderRepulsionExp (BoundaryCylinderRepulsionExp.cpp:27-110)
+// plane = (a, b, c, d); distance per BoundaryElementImpl.h:71-74
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double plane_dist(const double *__restrict__ pl, d3 p) {
+    double a = pl[0], b = pl[1], c = pl[2], d = pl[3];
+    return (a * p.x + b * p.y + c * p.z + d) / sqrt(a * a + b * b + c * c);
+}
+__device__ __forceinline__ double bd_energy(const double *__restrict__ pl, d3 p, double k, double sl) {
+    double r = plane_dist(pl, p);
+    double R = -r / sl;
+    return k * exp(R);
+}
+__device__ __forceinline__ d3 bd_force(const double *__restrict__ pl, d3 p, double k, double sl) {
+    double r = plane_dist(pl, p);
+    double R = -r / sl;
+    double f0 = k * exp(R) / sl;
+    return d3{f0 * pl[0], f0 * pl[1], f0 * pl[2]};
+}
+
+// ---------------------------------------------------------------------------
+// a9 CylinderExclVolRepulsion (CylinderExclVolRepulsion.cpp:329-882)
+// ---------------------------------------------------------------------------
+// areInPlane, MathFunctions.h:1101-1133
+__device__ __forceinline__ bool are_in_plane(d3 p1, d3 p2, d3 p3, d3 p4) {
+    d3 v1 = sub(p2, p1), v2 = sub(p3, p1), v3 = sub(p4, p1);
+    d3 cp = normalize3(cross3(v1, v2));
+    v3 = normalize3(v3);
+    return fabs(dot3(v3, cp) - 0.0) < 1e-1;
+}
+// movePointOutOfPlane for bead i in {1,2}, MathFunctions.h:1221-1270
+__device__ __forceinline__ d3 moved_off_plane(d3 p1, d3 p2, d3 p3, d3 src, double d) {
+    d3 n = normalize3(cross3(sub(p2, p1), sub(p3, p1)));
+    return d3{(src.x + n.x * d), (src.y + n.y * d), (src.z + n.z * d)};
+}
+
+struct EvInv {
+    double a, b, c, d, e, F;
+    double AA, BB, CC, DD, EE, FF, GG, HH, JJ;
+    double ATG1, ATG2, ATG3, ATG4;
+};
+
+__device__ __forceinline__ void ev_invariants(d3 c1, d3 c2, d3 c3, d3 c4, EvInv &v) {
+    v.a = sprod(c1, c2, c1, c2);
+    v.b = sprod(c3, c4, c3, c4);
+    v.c = sprod(c3, c1, c3, c1);
+    v.d = sprod(c1, c2, c3, c4);
+    v.e = sprod(c1, c2, c3, c1);
+    v.F = sprod(c3, c4, c3, c1);
+    double g = sprod(c1, c2, c1, c4);
+    double h = sprod(c3, c4, c3, c2);
+    double I = sprod(c3, c4, c1, c4);
+    double sqD, sqE;
+    {
+        double x = c1.x - c4.x, y = c1.y - c4.y, z = c1.z - c4.z;
+        sqD = (x * x + y * y + z * z);
+        x = c3.x - c2.x; y = c3.y - c2.y; z = c3.z - c2.z;
+        sqE = (x * x + y * y + z * z);
+    }
+    const double a = v.a, b = v.b, c = v.c, d = v.d, e = v.e, F = v.F;
+    double ac = sqrt(a * c), bc = sqrt(b * c), aD = sqrt(a * sqD), bE = sqrt(b * sqE);
+    v.AA = sqrt(smax(0.0, (ac + e) * (ac - e)));
+    v.BB = sqrt(smax(0.0, (bc + F) * (bc - F)));
+    v.CC = d * e - a * F;
+    v.DD = b * e - d * F;
+    v.EE = sqrt(smax(0.0, (aD + g) * (aD - g)));
+    v.FF = sqrt(smax(0.0, (bE + h) * (bE - h)));
+    v.GG = d * g - a * I;
+    v.HH = v.CC + v.GG - v.DD;
+    d3 cp = cross3(sub(c1, c2), sub(c3, c4));
+    double JJ = dot3(cp, sub(c3, c1));
+    v.JJ = -JJ * JJ;
+    v.ATG1 = atan((a + e) / v.AA) - atan(e / v.AA);
+    v.ATG2 = atan((a + e - d) / v.EE) - atan((e - d) / v.EE);
+    v.ATG3 = atan((F) / v.BB) - atan((F - b) / v.BB);
+    v.ATG4 = atan((d + F) / v.FF) - atan((d + F - b) / v.FF);
+}
+
+__device__ __forceinline__ double ev_energy(d3 c1, d3 c2, d3 c3, d3 c4, double kr) {
+    EvInv v;
+    ev_invariants(c1, c2, c3, c4, v);
+    return 0.5 * kr / v.JJ * (v.CC / v.AA * v.ATG1 + v.GG / v.EE * v.ATG2 + v.DD / v.BB * v.ATG3 + v.HH / v.FF * v.ATG4);
+}
+__device__ __forceinline__ bool ev_energy_bad(double U) { return fabs(U) == INFINITY || U != U || U < -1.0; }
+
+// Analytic pair force; f[0..3] = forces on c1..c4. Direction vectors use the
+// c2 passed in (the moved one in the coplanar case), as the reference does.
+__device__ __forceinline__ void ev_force(d3 c1, d3 c2, d3 c3, d3 c4, double kr, d3 f[4]) {
+    EvInv v;
+    ev_invariants(c1, c2, c3, c4, v);
+    const double a = v.a, b = v.b, c = v.c, d = v.d, e = v.e, F = v.F;
+    const double AA = v.AA, BB = v.BB, CC = v.CC, DD = v.DD, EE = v.EE, FF = v.FF, GG = v.GG, HH = v.HH;
+    const double ATG1 = v.ATG1, ATG2 = v.ATG2, ATG3 = v.ATG3, ATG4 = v.ATG4;
+    const double invJJ = 1 / v.JJ;
+    const double sumBlock = CC / AA * ATG1 + GG / EE * ATG2 + DD / BB * ATG3 + HH / FF * ATG4;
+    const double enFactor = kr * invJJ / 2;
+    const double A1 = AA * AA / (AA * AA + (a + e) * (a + e));
+    const double A2 = AA * AA / (AA * AA + e * e);
+    const double E1 = EE * EE / (EE * EE + (a + e - d) * (a + e - d));
+    const double E2 = EE * EE / (EE * EE + (e - d) * (e - d));
+    const double B1 = BB * BB / (BB * BB + F * F);
+    const double B2 = BB * BB / (BB * BB + (F - b) * (F - b));
+    const double F1 = FF * FF / (FF * FF + (d + F) * (d + F));
+    const double F2 = FF * FF / (FF * FF + (d + F - b) * (d + F - b));
+    const double pA_C = ATG1 / AA;
+    const double pA_A = -(ATG1 * CC) / (AA * AA) + (-A1 * (a + e) + A2 * e) * CC / (AA * AA * AA);
+    const double pA_e = ((A1 - A2) * CC) / (AA * AA);
+    const double pA_a = (A1 * CC) / (AA * AA);
+    const double pE_G = ATG2 / EE;
+    const double pE_E = -(ATG2 * GG) / (EE * EE) + (-E1 * (a + e - d) + E2 * (e - d)) * GG / (EE * EE * EE);
+    const double pE_emd = ((E1 - E2) * GG) / (EE * EE);
+    const double pE_a = (E1 * GG) / (EE * EE);
+    const double pB_D = ATG3 / BB;
+    const double pB_B = -(ATG3 * DD) / (BB * BB) + (-B1 * F + B2 * (F - b)) * DD / (BB * BB * BB);
+    const double pB_fmb = ((B1 - B2) * DD) / (BB * BB);
+    const double pB_b = (B1 * DD) / (BB * BB);
+    const double pF_H = ATG4 / FF;
+    const double pF_F = -(ATG4 * HH) / (FF * FF) + (-F1 * (d + F) + F2 * (d + F - b)) * HH / (FF * FF * FF);
+    const double pF_dfb = ((F1 - F2) * HH) / (FF * FF);
+    const double pF_b = (F1 * HH) / (FF * FF);
+    const double k11 = pA_C * (-2 * F) + pA_A * (c / AA) + pA_a * 2 + pE_G * (2 * (F - b)) +
+                       pE_E * ((b + c - 2 * F) / EE) + pE_a * 2 + pF_H * (-2 * b) + pF_F * (b / FF);
+    const double k12 = pA_C * e + pE_G * (2 * d - e) + pE_E * ((e - d) / EE) + pE_emd * (-1) + pB_D * (-F) +
+                       pF_H * (2 * d + F) + pF_F * (-(d + F) / FF) + pF_dfb;
+    const double k13 = pA_C * d + pA_A * (-e / AA) + pA_e + pE_G * (-d) + pE_E * (-(e - d) / EE) + pE_emd +
+                       pB_D * b + pF_H * (-b) + pF_F * (b / FF);
+    const double k22 = pE_G * (-2 * a) + pE_E * (a / EE) + pB_D * (2 * e) + pB_B * (c / BB) + pB_fmb * (-2) +
+                       pB_b * 2 + pF_H * (-2 * (a + e)) + pF_F * ((a + c + 2 * e) / FF) + pF_dfb * (-2) + pF_b * 2;
+    const double k23 = pA_C * (-a) + pE_G * a + pE_E * (-a / EE) + pB_D * (-d) + pB_B * (-F / BB) + pB_fmb +
+                       pF_H * d + pF_F * (-(d + F) / FF) + pF_dfb;
+    const double k33 = pA_A * (a / AA) + pE_E * (a / EE) + pB_B * (b / BB) + pF_F * (b / FF);
+    const double k21 = k12, k31 = k13, k32 = k23;
+    const double J11 = 2 * (-b * c + F * F), J12 = 2 * (c * d - e * F), J13 = 2 * (b * e - d * F);
+    const double J21 = 2 * (c * d - e * F), J22 = 2 * (-a * c + e * e), J23 = 2 * (-d * e + a * F);
+    const double J31 = 2 * (b * e - d * F), J32 = 2 * (-d * e + a * F), J33 = 2 * (d * d - a * b);
+    const double sb = -invJJ * sumBlock;
+    double dv[4][3];
+    dv[0][0] = enFactor * (sb * (J31 - J11) + (k31 - k11));
+    dv[0][1] = enFactor * (sb * (J32 - J12) + (k32 - k12));
+    dv[0][2] = enFactor * (sb * (J33 - J13) + (k33 - k13));
+    dv[1][0] = enFactor * (sb * (J11) + (k11));
+    dv[1][1] = enFactor * (sb * (J12) + (k12));
+    dv[1][2] = enFactor * (sb * (J13) + (k13));
+    dv[2][0] = enFactor * (sb * (-J31 - J21) + (-k31 - k21));
+    dv[2][1] = enFactor * (sb * (-J32 - J22) + (-k32 - k22));
+    dv[2][2] = enFactor * (sb * (-J33 - J23) + (-k33 - k23));
+    dv[3][0] = enFactor * (sb * (J21) + (k21));
+    dv[3][1] = enFactor * (sb * (J22) + (k22));
+    dv[3][2] = enFactor * (sb * (J23) + (k23));
+    const d3 u1 = sub(c2, c1), u2 = sub(c4, c3), u3 = sub(c1, c3);
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++) {
+        f[bi].x = -(dv[bi][0] * u1.x + dv[bi][1] * u2.x + dv[bi][2] * u3.x);
+        f[bi].y = -(dv[bi][0] * u1.y + dv[bi][1] * u2.y + dv[bi][2] * u3.y);
+        f[bi].z = -(dv[bi][0] * u1.z + dv[bi][1] * u2.z + dv[bi][2] * u3.z);
+    }
+}
+__device__ __forceinline__ bool finite_d3(d3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+// Simpson integrands (CylinderExclVolRepulsion.h:53-72)
+__device__ __forceinline__ double ev_r2(double a, double b, double c, double d, double e, double F, double s, double t) {
+    return c + 2 * s * e + s * s * a - 2 * t * F - 2 * s * t * d + t * t * b;
+}
+
+}  // namespace mdy

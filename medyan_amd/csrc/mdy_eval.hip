@@ -1,0 +1,183 @@
+// mdy_eval.hip -- one-shot energy / force evaluation kernels behind
+// mdy_energy(_dev) and mdy_forces(_dev) (ForceFieldManager::computeEnergy and
+// ::computeForces), including the excluded-volume quadrature fallbacks.
+#include "mdy_eval.hpp"
+#include "mdy_launch.hpp"
+
+namespace mdy {
+
+template <class P>
+__global__ void __launch_bounds__(kBlock) k_energy(Dev D, P pos, double *red) {
+    __shared__ double lds[17 * kNFF];
+    double acc[kNFF];
+#pragma unroll
+    for (int k = 0; k < kNFF; k++) acc[k] = 0.0;
+    const int total = D.n_items(), gs = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs) energy_item(D, i, pos, acc);
+    block_reduce<kNFF>(acc, lds);
+    if (threadIdx.x < kNFF) red[(size_t)blockIdx.x * kRed + threadIdx.x] = acc[threadIdx.x];
+}
+
+// One block per queued pair: 2-D Simpson energy (energyN, 400 x 400 intervals).
+template <class P>
+__global__ void __launch_bounds__(kBlock) k_energy_simpson(Dev D, P pos, double *qe_u) {
+    __shared__ double lds[17];
+    const int nq = min(*D.qe_n, kQueueCap);
+    const int N = 400;
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        const int i = D.qe[q];
+        int4 b = D.ev_bs[i];
+        SimpsonGeo g = simpson_geo(pos(b.x), pos(b.y), pos(b.z), pos(b.w));
+        double acc[1] = {0.0};
+        for (int node = threadIdx.x; node < (N + 1) * (N + 1); node += blockDim.x)
+            acc[0] += simpson_e_node(g, node / (N + 1), node % (N + 1), N);
+        block_reduce<1>(acc, lds);
+        if (threadIdx.x == 0) {
+            const double ds = 1.0 / ((double)N);
+            double U = D.ev_kr[i] * (ds * ds / 9.0) * (simpson_e_corners(g) + acc[0]);
+            qe_u[q] = U;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_energy_final(Dev D, const double *red, int nblk, const double *qe_u,
+                                                         double *out) {
+    __shared__ double lds[17 * kNFF];
+    double e[kNFF + 2];
+    energy_epilogue(D, red, nblk, *D.qe_n, qe_u, lds, e);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kNFF + 2; k++) out[k] = e[k];
+}
+
+template <class P>
+__global__ void __launch_bounds__(kBlock) k_force(Dev D, P pos, double *contrib) {
+    const int total = D.n_items(), gs = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs) force_item(D, i, pos, contrib);
+}
+
+// Quadrature force (forceN, 10^4 x 10^4 intervals) of one queued pair, split
+// over the whole grid; partial sums per block.
+__global__ void __launch_bounds__(kBlock) k_force_simpson_part(Dev D, const double *x, int q, int moved,
+                                                               double *part) {
+    __shared__ double lds[17 * 6];
+    const int N = 10000;
+    const int i = D.qf[q];
+    int4 b = D.ev_bs[i];
+    PosX pos{x};
+    d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+    if (moved) {  // both beads of the first cylinder moved off the plane (:1017-1022)
+        d3 n2 = moved_off_plane(c1, c2, c3, c2, 0.01), n1 = moved_off_plane(c1, c2, c3, c1, 0.01);
+        c1 = n1; c2 = n2;
+    }
+    SimpsonGeo g = simpson_geo(c1, c2, c3, c4);
+    double o[6] = {0, 0, 0, 0, 0, 0};
+    const long long nodes = (long long)(N + 1) * (N + 1);
+    for (long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x; nd < nodes;
+         nd += (long long)gridDim.x * blockDim.x)
+        simpson_f_node(g, (int)(nd / (N + 1)), (int)(nd % (N + 1)), N, o);
+    block_reduce<6>(o, lds);
+    if (threadIdx.x < 6) part[(size_t)blockIdx.x * kRed + threadIdx.x] = o[threadIdx.x];
+}
+
+// flag[0] = 1 if the result was non-finite (host retries with moved beads or fails)
+__global__ void __launch_bounds__(kBlock) k_force_simpson_final(Dev D, const double *x, int q, int moved,
+                                                                const double *part, int nblk, double *contrib,
+                                                                int *flag) {
+    __shared__ double lds[17 * 6];
+    const int N = 10000;
+    double T[6];
+    grid_partials_reduce<6>(part, nblk, T, lds);
+    if (threadIdx.x != 0) return;
+    const int i = D.qf[q];
+    int4 b = D.ev_bs[i];
+    PosX pos{x};
+    d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+    if (moved) {
+        d3 n2 = moved_off_plane(c1, c2, c3, c2, 0.01), n1 = moved_off_plane(c1, c2, c3, c1, 0.01);
+        c1 = n1; c2 = n2;
+    }
+    SimpsonGeo g = simpson_geo(c1, c2, c3, c4);
+    double corners[6] = {0, 0, 0, 0, 0, 0};
+    fint6(g, 0.0, 0.0, 1.0, corners); fint6(g, 0.0, 1.0, 1.0, corners);
+    fint6(g, 1.0, 0.0, 1.0, corners); fint6(g, 1.0, 1.0, 1.0, corners);
+    for (int k = 0; k < 6; k++) T[k] = corners[k] + T[k];
+    d3 f[4];
+    simpson_f_assemble(g, T, D.ev_kr[i], N, f);
+    bool ok = finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]);
+    // the retry with moved beads happens only when the pair is in plane
+    flag[1] = are_in_plane(pos(b.x), pos(b.y), pos(b.z), pos(b.w)) ? 1 : 0;
+    flag[0] = ok ? 0 : 1;
+    if (ok) {
+        int s = D.slot_ev + 4 * i;
+        put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather(Dev D, const double *contrib, double *force, int accumulate) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < D.nb; b += gridDim.x * blockDim.x) {
+        d3 f = gather_bead(D, contrib, b);
+        if (accumulate) {
+            force[3 * b] += f.x; force[3 * b + 1] += f.y; force[3 * b + 2] += f.z;
+        } else {
+            force[3 * b] = f.x; force[3 * b + 1] = f.y; force[3 * b + 2] = f.z;
+        }
+    }
+}
+
+__global__ void k_zero_int(int *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static int grid_for(const Ctx &c, long long n) {
+    long long g = (n + kBlock - 1) / kBlock;
+    long long cap = (long long)c.num_cu * 8;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int launch_energy(Ctx &c, const double *d_x, double *d_out, hipStream_t st) {
+    Dev D = c.dev();
+    const int nblk = grid_for(c, D.n_items());
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c.qn.p, 4);
+    hipLaunchKernelGGL(k_energy<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.red.p);
+    hipLaunchKernelGGL(k_energy_simpson<PosX>, dim3(64), dim3(kBlock), 0, st, D, PosX{d_x}, c.qe_u.p);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, D, c.red.p, nblk, c.qe_u.p, d_out);
+    return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
+}
+
+int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hipStream_t st) {
+    Dev D = c.dev();
+    const int nblk = grid_for(c, D.n_items());
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c.qn.p, 4);
+    hipLaunchKernelGGL(k_force<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
+    if (hipGetLastError() != hipSuccess) return MDY_ERR_HIP;
+    // quadrature fallbacks: rare; the count is read back to drive them
+    int h_q[4];
+    if (hipMemcpyAsync(h_q, c.qn.p, sizeof(h_q), hipMemcpyDeviceToHost, st) != hipSuccess) return MDY_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return MDY_ERR_HIP;
+    int nq = h_q[1] < kQueueCap ? h_q[1] : kQueueCap;
+    if (h_q[1] > kQueueCap) return MDY_ERR_CAPACITY;
+    const int sblk = c.num_cu * 4;
+    for (int q = 0; q < nq; q++) {
+        for (int moved = 0; moved < 2; moved++) {
+            hipLaunchKernelGGL(k_force_simpson_part, dim3(sblk), dim3(kBlock), 0, st, D, d_x, q, moved,
+                               c.simpson_part.p);
+            hipLaunchKernelGGL(k_force_simpson_final, dim3(1), dim3(kBlock), 0, st, D, d_x, q, moved,
+                               c.simpson_part.p, sblk, c.contrib.p, c.qn.p + 2);
+            int fl[2];
+            if (hipMemcpyAsync(fl, c.qn.p + 2, sizeof(fl), hipMemcpyDeviceToHost, st) != hipSuccess) return MDY_ERR_HIP;
+            if (hipStreamSynchronize(st) != hipSuccess) return MDY_ERR_HIP;
+            if (!fl[0]) break;
+            if (moved || !fl[1]) return MDY_ERR_FORCE;  // reference exits here
+        }
+    }
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_force, accumulate);
+    return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
+}
+
+}  // namespace mdy

@@ -1,0 +1,47 @@
+// mdy_launch.hpp -- host-side launchers shared between translation units.
+#pragma once
+#include "mdy_internal.hpp"
+
+namespace mdy {
+
+// mdy_eval.hip
+int launch_energy(Ctx &c, const double *d_x, double *d_out, hipStream_t st);
+int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hipStream_t st);
+// mdy_nl.hip
+int build_neighbors(Ctx &c, const double *d_x);
+int exclusive_scan(Ctx &c, const int *in, int *out, int n, hipStream_t st);  // out has n+1 entries
+// mdy_cg.hip
+int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res);
+// mdy_hvp.hip
+int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st);
+
+template <class T>
+int ensure(DBuf<T> &b, size_t n) {
+    if (n == 0) n = 1;
+    if (b.n >= n) return MDY_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+    size_t cap = n + n / 4 + 16;
+    if (hipMalloc((void **)&b.p, cap * sizeof(T)) != hipSuccess) return MDY_ERR_HIP;
+    b.n = cap;
+    return MDY_OK;
+}
+template <class T>
+void release(DBuf<T> &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+}
+
+#define MDY_TRY(expr)                    \
+    do {                                 \
+        int _rc = (expr);                \
+        if (_rc != MDY_OK) return _rc;   \
+    } while (0)
+#define MDY_HIP(expr)                                        \
+    do {                                                     \
+        if ((expr) != hipSuccess) return MDY_ERR_HIP;        \
+    } while (0)
+
+}  // namespace mdy

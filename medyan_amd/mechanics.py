@@ -1,0 +1,283 @@
+"""Python host binding of libmdy_hip.so.
+
+Mirrors MEDYAN's mechanics interface for the minimization path:
+
+* ``ForceFieldManager``  -- vectorizeAllForceFields / computeEnergy /
+  computeIndividualEnergies / computeForces
+  (src/Mechanics/ForceField/ForceFieldManager.h:48-216)
+* ``CGMethod.minimize``  -- CGMethod::minimize<polakRibiere> with backtracking
+  (src/Mechanics/Minimizer/CGMethod.hpp:826-1759), run on the device
+* ``ConjugateGradientParams`` / ``MinimizationResult``
+
+Error behaviour follows the reference: an energy that a force field reports
+as invalid makes computeEnergy return +inf; a non-finite max force raises
+RuntimeError("maxF error") like CGMethod::maxF; a line-search failure without
+recovery sets MinimizationResult.errorFlags.
+
+The library is required: there is no CPU fallback. Loading fails loudly if
+libmdy_hip.so is missing, and every GPU entry point fails if no HIP device is
+present.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .network import Network
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdy_hip.so")
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libmdy_hip.so (built in-tree by medyan_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m medyan_amd.build` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(path)
+    f64p = C.POINTER(C.c_double)
+    i32p = C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    sig = {
+        "mdy_abi_version": (C.c_int, []),
+        "mdy_device_count": (C.c_int, []),
+        "mdy_default_opts": (None, [C.POINTER(_abi.Opts)]),
+        "mdy_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(_abi.Opts)]),
+        "mdy_destroy": (None, [vp]),
+        "mdy_last_error": (C.c_char_p, [vp]),
+        "mdy_set_opts": (C.c_int, [vp, C.POINTER(_abi.Opts)]),
+        "mdy_set_network": (C.c_int, [vp, C.POINTER(_abi.System)]),
+        "mdy_build_neighbors": (C.c_int, [vp, f64p]),
+        "mdy_build_neighbors_dev": (C.c_int, [vp, vp]),
+        "mdy_get_pairs": (C.c_int, [vp, i32p, C.c_int64, C.POINTER(C.c_int64)]),
+        "mdy_get_boundary_table": (C.c_int, [vp, i32p, i32p, C.c_int64, C.POINTER(C.c_int64)]),
+        "mdy_energy": (C.c_int, [vp, f64p, f64p, f64p]),
+        "mdy_forces": (C.c_int, [vp, f64p, f64p]),
+        "mdy_energy_dev": (C.c_int, [vp, vp, vp, vp]),
+        "mdy_forces_dev": (C.c_int, [vp, vp, vp, C.c_int, vp]),
+        "mdy_hvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mdy_minimize": (C.c_int, [vp, C.POINTER(_abi.CGParams), f64p, f64p, C.POINTER(_abi.MinResult)]),
+        "mdy_minimize_dev": (C.c_int, [vp, C.POINTER(_abi.CGParams), vp, vp, C.POINTER(_abi.MinResult)]),
+        "mdy_get_trace": (C.c_int, [vp, f64p, f64p, f64p, f64p, i32p, C.c_int64, C.POINTER(C.c_int64)]),
+        "mdy_set_trace": (C.c_int, [vp, C.c_int64]),
+        "mdy_last_timing": (C.c_int, [vp, f64p]),
+        "mdy_set_profile": (C.c_int, [vp, C.c_int64]),
+        "mdy_get_profile": (C.c_int, [vp, i32p, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]),
+        "mdy_gen_defaults": (None, [C.POINTER(_abi.GenParams)]),
+        "mdy_generate": (C.c_int, [C.POINTER(_abi.GenParams), C.POINTER(C.POINTER(_abi.Network))]),
+        "mdy_network_free": (None, [C.POINTER(_abi.Network)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.mdy_abi_version() != _abi.MDY_ABI_VERSION:
+        raise RuntimeError("libmdy_hip.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def _f64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class MdyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_abi.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+@dataclass
+class ConjugateGradientParams:
+    """ConjugateGradientParams (CGMethod.hpp:44-61) with the benchmark defaults
+    (GRADIENTTOLERANCE 1, MAXDISTANCE 1, LAMBDAMAX 0.01)."""
+    descentSearchMethod: int = _abi.MDY_CG_POLAK_RIBIERE
+    maxDist: float = 1.0
+    lambdaMax: float = 0.01
+    gradTol: float = 1.0
+    energyRelTol: float = 1e-9
+    energyCallLimit: int = 0
+    forceCallLimit: int = 10000
+    tryToRecoverInLineSearchError: bool = True
+
+    def to_c(self, n_beads) -> _abi.CGParams:
+        p = _abi.CGParams()
+        p.method = int(self.descentSearchMethod)
+        p.max_dist = self.maxDist
+        p.lambda_max = self.lambdaMax
+        p.grad_tol = self.gradTol
+        p.energy_rel_tol = self.energyRelTol
+        p.energy_call_limit = int(self.energyCallLimit)
+        p.force_call_limit = int(self.forceCallLimit)
+        p.n_beads_limit = int(n_beads)
+        p.try_to_recover = 1 if self.tryToRecoverInLineSearchError else 0
+        return p
+
+
+class Context:
+    """One mdy_ctx: a HIP device, its stream and the device-resident system."""
+
+    def __init__(self, device=0, precision=_abi.MDY_PREC_FP64, coplanar_mode=_abi.MDY_COPLANAR_REFERENCE,
+                 blocks_per_cu=0):
+        self.lib = load_library()
+        o = _abi.Opts()
+        self.lib.mdy_default_opts(C.byref(o))
+        o.precision = precision
+        o.coplanar_mode = coplanar_mode
+        o.blocks_per_cu = blocks_per_cu
+        self.opts = o
+        h = C.c_void_p()
+        rc = self.lib.mdy_create(C.byref(h), device, C.byref(o))
+        if rc != 0:
+            raise MdyError(rc, "mdy_create failed (no HIP device?)")
+        self.h = h
+        self.net = None
+        self._sys = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mdy_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MdyError(rc, self.lib.mdy_last_error(self.h).decode())
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            setattr(self.opts, k, v)
+        self._check(self.lib.mdy_set_opts(self.h, C.byref(self.opts)))
+
+    # -- system ----------------------------------------------------------------
+    def set_network(self, net: Network):
+        self.net = net
+        self._sys = net.system()
+        self._check(self.lib.mdy_set_network(self.h, C.byref(self._sys)))
+
+    def build_neighbors(self, coord=None):
+        x = np.ascontiguousarray(self.net.coord if coord is None else coord, dtype=np.float64)
+        self._check(self.lib.mdy_build_neighbors(self.h, _f64(x)))
+
+    def pairs(self):
+        n = C.c_int64()
+        self._check(self.lib.mdy_get_pairs(self.h, None, 0, C.byref(n)))
+        out = np.zeros(2 * max(n.value, 1), dtype=np.int32)
+        self._check(self.lib.mdy_get_pairs(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n)))
+        return out[: 2 * n.value].reshape(-1, 2)
+
+    def boundary_table(self):
+        n = C.c_int64()
+        npl = len(self.net.plane_k)
+        nn = np.zeros(max(npl, 1), dtype=np.int32)
+        self._check(self.lib.mdy_get_boundary_table(self.h, nn.ctypes.data_as(C.POINTER(C.c_int32)), None, 0,
+                                                    C.byref(n)))
+        bs = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._check(self.lib.mdy_get_boundary_table(self.h, nn.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                    bs.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n)))
+        return nn[:npl], bs[: n.value]
+
+    # -- ForceFieldManager -------------------------------------------------------
+    def energy(self, coord):
+        x = np.ascontiguousarray(coord, dtype=np.float64)
+        tot = C.c_double()
+        per = np.zeros(_abi.MDY_NFF)
+        self._check(self.lib.mdy_energy(self.h, _f64(x), C.byref(tot), _f64(per)))
+        return tot.value, per
+
+    def forces(self, coord, accum=None):
+        x = np.ascontiguousarray(coord, dtype=np.float64)
+        f = np.zeros_like(x) if accum is None else np.ascontiguousarray(accum, dtype=np.float64).copy()
+        self._check(self.lib.mdy_forces(self.h, _f64(x), _f64(f)))
+        return f
+
+    # -- CGMethod ------------------------------------------------------------------
+    def minimize(self, params, coord=None, trace_cap=0):
+        """params: ConjugateGradientParams or _abi.CGParams. Returns (coord,
+        force, MinimizationResult-like dict). Raises MdyError on failures the
+        reference turns into exceptions."""
+        if isinstance(params, ConjugateGradientParams):
+            params = params.to_c(self.net.n_beads)
+        x = np.ascontiguousarray(self.net.coord if coord is None else coord, dtype=np.float64).copy()
+        f = np.zeros_like(x)
+        self._check(self.lib.mdy_set_trace(self.h, int(trace_cap)))
+        res = _abi.MinResult()
+        rc = self.lib.mdy_minimize(self.h, C.byref(params), _f64(x), _f64(f), C.byref(res))
+        if rc not in (0,):
+            raise MdyError(rc, self.lib.mdy_last_error(self.h).decode())
+        return x, f, res
+
+    def trace(self):
+        n = C.c_int64()
+        self._check(self.lib.mdy_get_trace(self.h, None, None, None, None, None, 0, C.byref(n)))
+        m = int(n.value)
+        arrs = [np.zeros(max(m, 1)) for _ in range(4)] + [np.zeros(max(m, 1), dtype=np.int32)]
+        self._check(self.lib.mdy_get_trace(self.h, *[_f64(a) for a in arrs[:4]],
+                                           arrs[4].ctypes.data_as(C.POINTER(C.c_int32)), m, C.byref(n)))
+        m = min(m, int(n.value))
+        return dict(lambda_=arrs[0][:m], energy=arrs[1][:m], maxf=arrs[2][:m], beta=arrs[3][:m], safe=arrs[4][:m])
+
+    def set_profile(self, cap):
+        self._check(self.lib.mdy_set_profile(self.h, int(cap)))
+
+    def profile(self):
+        """(kind, ticks) of the phases of the last minimization; ticks are
+        100 MHz device wall-clock stamps taken at each phase's barrier exit."""
+        n = C.c_int64()
+        self._check(self.lib.mdy_get_profile(self.h, None, None, 0, C.byref(n)))
+        m = int(n.value)
+        kind = np.zeros(max(m, 1), dtype=np.int32)
+        ticks = np.zeros(max(m, 1), dtype=np.int64)
+        self._check(self.lib.mdy_get_profile(self.h, kind.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             ticks.ctypes.data_as(C.POINTER(C.c_int64)), m, C.byref(n)))
+        m = min(m, int(n.value))
+        return kind[:m], ticks[:m]
+
+    def last_ms(self):
+        t = C.c_double()
+        self._check(self.lib.mdy_last_timing(self.h, C.byref(t)))
+        return t.value
+
+
+class ForceFieldManager:
+    """ForceFieldManager over the device: vectorizeAllForceFields uploads the
+    tables and rebuilds the neighbor lists from the given coordinates."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def vectorizeAllForceFields(self, net: Network, coord=None):
+        self.ctx.set_network(net)
+        self.ctx.build_neighbors(coord)
+
+    def computeEnergy(self, coord):
+        return self.ctx.energy(coord)[0]
+
+    def computeIndividualEnergies(self, coord):
+        tot, per = self.ctx.energy(coord)
+        return dict(zip(_abi.FF_NAMES, per.tolist())), float(np.sum(per))
+
+    def computeForces(self, coord, force, numVar=None):
+        force[:] = 0.0
+        force[:] = self.ctx.forces(coord, force)
+        return force
+
+
+class CGMethod:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def minimize(self, params: ConjugateGradientParams, coord, trace_cap=0):
+        return self.ctx.minimize(params, coord, trace_cap=trace_cap)

@@ -1,0 +1,1038 @@
+/*
+ * mdy_oracle.c -- CPU restatement of MEDYAN's mechanical-equilibration path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation. Loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by
+ * the product library.
+ *
+ * Written from the reference's mathematics and behaviour. File:line citations
+ * are relative to the MEDYAN source tree (src/). Operation order inside each
+ * expression follows the cited lines so that the IEEE results are reproducible
+ * when built with -ffp-contract=off (the parity build).
+ *
+ * Parity status per function is listed in mdy_oracle.h; the Simpson
+ * quadrature fallbacks of the excluded-volume force field are "parity
+ * unpinned" (only a run of the reference could pin their exact outputs).
+ */
+#include "mdy_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* scalar helpers: src/MathFunctions.h, src/utility.h                        */
+/* ------------------------------------------------------------------------- */
+/* std::max / std::min semantics (first argument kept unless the comparison
+ * holds), which decide what happens to NaN. */
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+/* areEqual, utility.h:75-77 (ZERO_PREC = 1e-6 for double, utility.h:57-66) */
+static inline int are_equal(double a, double b) { return fabs(a - b) < 1e-6; }
+/* areEqualLT, utility.h:80-83 */
+static inline int are_equal_lt(double a, double b) { return fabs(a - b) < 1e-1; }
+
+/* scalarProduct(v1,v2,v3,v4) = (v2-v1).(v4-v3), MathFunctions.h:469-476 */
+static inline double sprod(const double *v1, const double *v2, const double *v3, const double *v4) {
+    return ((v2[0] - v1[0]) * (v4[0] - v3[0]) +
+            (v2[1] - v1[1]) * (v4[1] - v3[1]) +
+            (v2[2] - v1[2]) * (v4[2] - v3[2]));
+}
+/* twoPointDistance, MathFunctions.h:294-297 */
+static inline double tpdist(const double *v1, const double *v2) {
+    return sqrt((v2[0] - v1[0]) * (v2[0] - v1[0]) + (v2[1] - v1[1]) * (v2[1] - v1[1]) +
+                (v2[2] - v1[2]) * (v2[2] - v1[2]));
+}
+/* sqmagnitude(c1,c2), MathFunctions.h:210-215 */
+static inline double sqmag2(const double *c1, const double *c2) {
+    double x = c1[0] - c2[0], y = c1[1] - c2[1], z = c1[2] - c2[2];
+    return (x * x + y * y + z * z);
+}
+/* midPointCoordinate(v, v1, v2, alpha), MathFunctions.h:963-970 */
+static inline void midpt(double *v, const double *v1, const double *v2, double alpha) {
+    double beta = 1 - alpha;
+    v[0] = (v1[0] * beta + alpha * v2[0]);
+    v[1] = (v1[1] * beta + alpha * v2[1]);
+    v[2] = (v1[2] * beta + alpha * v2[2]);
+}
+/* crossProduct, MathFunctions.h:910-916 */
+static inline void cross3(double *cp, const double *v1, const double *v2) {
+    cp[0] = v1[1] * v2[2] - v1[2] * v2[1];
+    cp[1] = v1[2] * v2[0] - v1[0] * v2[2];
+    cp[2] = v1[0] * v2[1] - v1[1] * v2[0];
+}
+/* normalizeVector, MathFunctions.h:178-190 */
+static inline void normalize3(double *v) {
+    double x2 = v[0] * v[0], y2 = v[1] * v[1], z2 = v[2] * v[2];
+    double norm = sqrt(x2 + y2 + z2);
+    v[0] = v[0] / norm; v[1] = v[1] / norm; v[2] = v[2] / norm;
+}
+static inline double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* areInPlane, MathFunctions.h:1101-1133 */
+int orc_are_in_plane(const double *p1, const double *p2, const double *p3, const double *p4) {
+    double v1[3], v2[3], v3[3], cp[3];
+    for (int d = 0; d < 3; d++) { v1[d] = p2[d] - p1[d]; v2[d] = p3[d] - p1[d]; v3[d] = p4[d] - p1[d]; }
+    cross3(cp, v1, v2);
+    normalize3(cp);
+    normalize3(v3);
+    return are_equal_lt(dot3(v3, cp), 0.0);
+}
+/* movePointOutOfPlane (in-place array version), MathFunctions.h:1221-1270 */
+static void move_out_of_plane(const double *p1, const double *p2, const double *p3, const double *p4,
+                              double *newp, int which, double d) {
+    double nrm[3], v1[3], v2[3];
+    for (int k = 0; k < 3; k++) { v1[k] = p2[k] - p1[k]; v2[k] = p3[k] - p1[k]; }
+    cross3(nrm, v1, v2);
+    normalize3(nrm);
+    const double *src = which == 1 ? p1 : which == 2 ? p2 : which == 3 ? p3 : p4;
+    for (int k = 0; k < 3; k++) newp[k] = (src[k] + nrm[k] * d);
+}
+static inline int finite3(const double *v) { return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]); }
+
+/* ------------------------------------------------------------------------- */
+/* a3: FilamentStretchingHarmonic  (Filament/FilamentStretchingHarmonic.cpp:185-207, 209-275) */
+/* ------------------------------------------------------------------------- */
+double orc_fs_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0) {
+    double U = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[2 * i], *c2 = x + bs[2 * i + 1];
+        double dist = tpdist(c1, c2) - l0[i];
+        double Ui = 0.5 * k[i] * dist * dist;
+        U += Ui;
+    }
+    return U;
+}
+void orc_fs_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0) {
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[2 * i], *c2 = x + bs[2 * i + 1];
+        double dist = tpdist(c1, c2);
+        double invL = 1 / dist;
+        double f0 = k[i] * (dist - l0[i]) * invL;
+        double *f1 = f + bs[2 * i], *f2 = f + bs[2 * i + 1];
+        for (int d = 0; d < 3; d++) {
+            double r = c2[d] - c1[d];
+            double F = -f0 * (r);
+            f2[d] += F;
+            f1[d] += -F;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a4 / a8: cosine bending (FilamentBendingCosine.cpp:199-243, 246-398;      */
+/*          BranchingBendingCosine.cpp:196-241, 243-411)                      */
+/* ------------------------------------------------------------------------- */
+/* U = k(1-x) for theta0 == 0, else k(1 - cos(theta - theta0)) with x = cos(theta) */
+static double cosine_bend_energy(double l1l2, double L1, double L2, double k, double th0) {
+    double L1L2 = L1 * L2;
+    double xx = l1l2 / L1L2;
+    if (xx < -1.0) xx = -1.0;
+    else if (xx > 1.0) xx = 1.0;
+    if (are_equal(th0, 0.0)) return k * (1 - xx);
+    double cosA = xx;
+    double sinA = smax(sqrt(1 - cosA * cosA), 0.0);
+    double cosAmB = cosA * cos(th0) + sinA * sin(th0);
+    return k * (1 - cosAmB);
+}
+/* effective force constant and A, B, C of the cosine bending force */
+static void cosine_bend_coef(double L1sq, double L2sq, double l1l2, double kb, double th0,
+                             double *A, double *B, double *C, double *keff) {
+    double a = 1 / sqrt(L1sq * L2sq);
+    double xx = l1l2 * a;
+    *A = a;
+    *B = xx / L1sq;
+    *C = xx / L2sq;
+    if (are_equal(th0, 0.0)) { *keff = kb; return; }
+    if (fabs(fabs(xx) - 1.0) < 0.001) xx = 0.999 * xx;
+    if (xx < -1.0) xx = -1.0;
+    else if (xx > 1.0) xx = 1.0;
+    double cosp = xx;
+    double sinp = sqrt(smax((1 - cosp * cosp), 0.0));
+    double sinpmq = sinp * cos(th0) - cosp * sin(th0);
+    *keff = kb * sinpmq / sinp;
+}
+double orc_fb_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *th0) {
+    double U = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[3 * i], *c2 = x + bs[3 * i + 1], *c3 = x + bs[3 * i + 2];
+        double L1 = sqrt(sprod(c1, c2, c1, c2));
+        double L2 = sqrt(sprod(c2, c3, c2, c3));
+        double l1l2 = sprod(c1, c2, c2, c3);
+        U += cosine_bend_energy(l1l2, L1, L2, k[i], th0[i]);
+    }
+    return U;
+}
+void orc_fb_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *th0) {
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[3 * i], *c2 = x + bs[3 * i + 1], *c3 = x + bs[3 * i + 2];
+        double *f1 = f + bs[3 * i], *f2 = f + bs[3 * i + 1], *f3 = f + bs[3 * i + 2];
+        double L1sq = sprod(c1, c2, c1, c2), L2sq = sprod(c2, c3, c2, c3), l1l2 = sprod(c1, c2, c2, c3);
+        double A, B, C, ke;
+        cosine_bend_coef(L1sq, L2sq, l1l2, k[i], th0[i], &A, &B, &C, &ke);
+        for (int d = 0; d < 3; d++) {
+            double r1 = c2[d] - c1[d], r2 = c3[d] - c2[d];
+            double Fr1 = ke * (r2 * A - r1 * B);
+            double Fr2 = ke * (r1 * A - r2 * C);
+            f1[d] += -Fr1;
+            f2[d] += Fr1 - Fr2;
+            f3[d] += Fr2;
+        }
+    }
+}
+double orc_bb_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *th0) {
+    double U = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
+        double L1 = sqrt(sprod(c1, c2, c1, c2));
+        double L2 = sqrt(sprod(c3, c4, c3, c4));
+        double l1l2 = sprod(c1, c2, c3, c4);
+        U += cosine_bend_energy(l1l2, L1, L2, k[i], th0[i]);
+    }
+    return U;
+}
+void orc_bb_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *th0,
+                   double *stretch3) {
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
+        double *f1 = f + bs[4 * i], *f2 = f + bs[4 * i + 1], *f3 = f + bs[4 * i + 2], *f4 = f + bs[4 * i + 3];
+        double L1sq = sprod(c1, c2, c1, c2), L2sq = sprod(c3, c4, c3, c4), l1l2 = sprod(c1, c2, c3, c4);
+        double A, B, C, ke;
+        cosine_bend_coef(L1sq, L2sq, l1l2, k[i], th0[i], &A, &B, &C, &ke);
+        for (int d = 0; d < 3; d++) {
+            double r1 = c2[d] - c1[d], r2 = c4[d] - c3[d];
+            double Fr1 = ke * (r2 * A - r1 * B);
+            double Fr2 = ke * (r1 * A - r2 * C);
+            f1[d] += -Fr1;
+            f2[d] += Fr1;
+            double f3d = -Fr2;
+            f3[d] += f3d;
+            if (stretch3) stretch3[3 * i + d] = f3d;
+            f4[d] += Fr2;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a5 / a6: linker and motor harmonic springs between cylinder points         */
+/* (LinkerStretchingHarmonic.cpp:191-222, 225-336; MotorGhostStretchingHarmonic.cpp:234-389) */
+/* ------------------------------------------------------------------------- */
+double orc_ls_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0,
+                     const double *p1, const double *p2) {
+    double U = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+        double v1[3], v2[3];
+        midpt(v1, x + bs[4 * i], x + bs[4 * i + 1], p1[i]);
+        midpt(v2, x + bs[4 * i + 2], x + bs[4 * i + 3], p2[i]);
+        double dist = tpdist(v1, v2) - l0[i];
+        double Ui = 0.5 * k[i] * dist * dist;
+        U += Ui;
+    }
+    return U;
+}
+void orc_ls_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0,
+                   const double *p1, const double *p2, double *stretch) {
+    for (int32_t i = 0; i < n; i++) {
+        double v1[3], v2[3];
+        midpt(v1, x + bs[4 * i], x + bs[4 * i + 1], p1[i]);
+        midpt(v2, x + bs[4 * i + 2], x + bs[4 * i + 3], p2[i]);
+        double dist = tpdist(v1, v2);
+        double invL = 1 / dist;
+        double f0 = k[i] * (dist - l0[i]) * invL;
+        double *f1 = f + bs[4 * i], *f2 = f + bs[4 * i + 1], *f3 = f + bs[4 * i + 2], *f4 = f + bs[4 * i + 3];
+        for (int d = 0; d < 3; d++) {
+            double l = v2[d] - v1[d];
+            double Fl = -f0 * (l);
+            f1[d] += -Fl * (1 - p1[i]);
+            f2[d] += -Fl * (p1[i]);
+            f3[d] += Fl * (1 - p2[i]);
+            f4[d] += Fl * (p2[i]);
+        }
+        if (stretch) stretch[i] = f0 / invL;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a7: BranchingStretchingHarmonic (BranchingStretchingHarmonic.cpp:195-221, 224-318) */
+/* ------------------------------------------------------------------------- */
+double orc_bs_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0, const double *p) {
+    double U = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+        double v1[3];
+        midpt(v1, x + bs[3 * i], x + bs[3 * i + 1], p[i]);
+        double dist = tpdist(v1, x + bs[3 * i + 2]) - l0[i];
+        double Ui = 0.5 * k[i] * dist * dist;
+        U += Ui;
+    }
+    return U;
+}
+void orc_bs_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0,
+                   const double *p, double *stretch3) {
+    for (int32_t i = 0; i < n; i++) {
+        const double *c3 = x + bs[3 * i + 2];
+        double *f1 = f + bs[3 * i], *f2 = f + bs[3 * i + 1], *f3 = f + bs[3 * i + 2];
+        double v1[3];
+        midpt(v1, x + bs[3 * i], x + bs[3 * i + 1], p[i]);
+        double dist = tpdist(v1, c3);
+        double invL = 1 / dist;
+        double f0 = k[i] * (dist - l0[i]) * invL;
+        for (int d = 0; d < 3; d++) {
+            double r = c3[d] - v1[d];
+            f1[d] += -f0 * (r) * (p[i] - 1);
+            f2[d] += f0 * (r) * p[i];
+            f3[d] += -f0 * (r);
+            if (stretch3) stretch3[3 * i + d] = -f0 * (r);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a9: CylinderExclVolRepulsion (Volume/CylinderExclVolRepulsion.cpp)          */
+/* ------------------------------------------------------------------------- */
+/* Geometric invariants of a cylinder pair (c1,c2) x (c3,c4), :355-392 / :626-672. */
+typedef struct ev_inv {
+    double a, b, c, d, e, F, g, h, I;
+    double AA, BB, CC, DD, EE, FF, GG, HH, JJ;
+    double ATG1, ATG2, ATG3, ATG4;
+} ev_inv;
+
+static void ev_invariants(const double *c1, const double *c2, const double *c3, const double *c4, ev_inv *v) {
+    v->a = sprod(c1, c2, c1, c2);
+    v->b = sprod(c3, c4, c3, c4);
+    v->c = sprod(c3, c1, c3, c1);
+    v->d = sprod(c1, c2, c3, c4);
+    v->e = sprod(c1, c2, c3, c1);
+    v->F = sprod(c3, c4, c3, c1);
+    v->g = sprod(c1, c2, c1, c4);
+    v->h = sprod(c3, c4, c3, c2);
+    v->I = sprod(c3, c4, c1, c4);
+    double sqD = sqmag2(c1, c4), sqE = sqmag2(c3, c2);
+    double vA[3], vB[3], vC[3], cp[3];
+    for (int k = 0; k < 3; k++) { vA[k] = c1[k] - c2[k]; vB[k] = c3[k] - c4[k]; vC[k] = c3[k] - c1[k]; }
+    double a = v->a, b = v->b, c = v->c, d = v->d, e = v->e, F = v->F, g = v->g, h = v->h, I = v->I;
+    double ac = sqrt(a * c), bc = sqrt(b * c), aD = sqrt(a * sqD), bE = sqrt(b * sqE);
+    v->AA = sqrt(smax(0.0, (ac + e) * (ac - e)));
+    v->BB = sqrt(smax(0.0, (bc + F) * (bc - F)));
+    v->CC = d * e - a * F;
+    v->DD = b * e - d * F;
+    v->EE = sqrt(smax(0.0, (aD + g) * (aD - g)));
+    v->FF = sqrt(smax(0.0, (bE + h) * (bE - h)));
+    v->GG = d * g - a * I;
+    v->HH = v->CC + v->GG - v->DD;
+    cross3(cp, vA, vB);
+    double JJ = dot3(cp, vC);
+    v->JJ = -JJ * JJ;
+    double AA = v->AA, BB = v->BB, EE = v->EE, FF = v->FF;
+    v->ATG1 = atan((a + e) / AA) - atan(e / AA);
+    v->ATG2 = atan((a + e - d) / EE) - atan((e - d) / EE);
+    v->ATG3 = atan((F) / BB) - atan((F - b) / BB);
+    v->ATG4 = atan((d + F) / FF) - atan((d + F - b) / FF);
+}
+
+double orc_ev_pair_energy_analytic(const double *c1, const double *c2, const double *c3, const double *c4, double kr) {
+    ev_inv v;
+    ev_invariants(c1, c2, c3, c4, &v);
+    /* U = 1/2 kR / JJ * (CC/AA ATG1 + GG/EE ATG2 + DD/BB ATG3 + HH/FF ATG4), :393-435 */
+    return 0.5 * kr / v.JJ *
+           (v.CC / v.AA * v.ATG1 + v.GG / v.EE * v.ATG2 + v.DD / v.BB * v.ATG3 + v.HH / v.FF * v.ATG4);
+}
+
+/* 2-D composite Simpson rule over s,t in [0,1] of 1/r^4 as coded in energyN
+ * (:884-999): the running abscissae are advanced before their first use, so
+ * interior nodes sit at (i+1)/N; the reference's sum structure is kept. */
+static inline double r2_st(double a, double b, double c, double d, double e, double F, double s, double t) {
+    return c + 2 * s * e + s * s * a - 2 * t * F - 2 * s * t * d + t * t * b;   /* CylinderExclVolRepulsion.h:53-58 */
+}
+static inline double eint(double a, double b, double c, double d, double e, double F, double s, double t) {
+    double r2 = r2_st(a, b, c, d, e, F, s, t);
+    return (1.0 / (r2 * r2));
+}
+double orc_ev_pair_energy_simpson(const double *c1, const double *c2, const double *c3, const double *c4,
+                                  double kr, int N1, int N2) {
+    double a = sprod(c1, c2, c1, c2), b = sprod(c3, c4, c3, c4), c = sprod(c3, c1, c3, c1);
+    double d = sprod(c1, c2, c3, c4), e = sprod(c1, c2, c3, c1), F = sprod(c3, c4, c3, c1);
+    double ds = 1.0 / ((double)N1), dt = 1.0 / ((double)N2);
+    double T1 = eint(a, b, c, d, e, F, 0.0, 0.0) + eint(a, b, c, d, e, F, 0.0, 1.0) +
+                eint(a, b, c, d, e, F, 1.0, 0.0) + eint(a, b, c, d, e, F, 1.0, 1.0);
+    double T2 = 0.0, T3 = 0.0, T4 = 0.0, s = 0.0, t = 0.0;
+    for (int i = 0; i <= N1; i++) {
+        t = 0.0;
+        s += ds;
+        double fs = (i % 2 == 0) ? 2.0 : 4.0;
+        if (i > 0 && i < N1) T2 += fs * (eint(a, b, c, d, e, F, s, 0.0) + eint(a, b, c, d, e, F, s, 1.0));
+        for (int j = 0; j <= N2; j++) {
+            t += dt;
+            double ft = (j % 2 == 0) ? 2.0 : 4.0;
+            if (j > 0 && j < N2) {
+                if (i == 0) T3 += ft * (eint(a, b, c, d, e, F, 0.0, t) + eint(a, b, c, d, e, F, 1.0, t));
+                else if (i < N1) T4 += fs * ft * eint(a, b, c, d, e, F, s, t);
+            }
+        }
+    }
+    return kr * (ds * dt / 9.0) * (T1 + T2 + T3 + T4);
+}
+
+/* forceN (:1001-1254): Simpson integrals of {1, s, s^2, st, t, t^2}/r^6 and the
+ * force assembly; f12 receives the 4x3 forces. */
+static void fint(double a, double b, double c, double d, double e, double F, double s, double t, double *o) {
+    double r2 = r2_st(a, b, c, d, e, F, s, t);
+    double xx = (1.0 / (r2 * r2 * r2));   /* CylinderExclVolRepulsion.h:60-72 */
+    o[0] = xx; o[1] = s * xx; o[2] = s * s * xx; o[3] = s * t * xx; o[4] = t * xx; o[5] = t * t * xx;
+}
+void orc_ev_pair_force_simpson(const double *c1, const double *c2, const double *c3, const double *c4,
+                               double kr, int N1, int N2, double *f12) {
+    double a = sprod(c1, c2, c1, c2), b = sprod(c3, c4, c3, c4), c = sprod(c3, c1, c3, c1);
+    double d = sprod(c1, c2, c3, c4), e = sprod(c1, c2, c3, c1), F = sprod(c3, c4, c3, c1);
+    double vA[3], vB[3], vC[3];
+    for (int k = 0; k < 3; k++) { vA[k] = c1[k] - c2[k]; vB[k] = c3[k] - c4[k]; vC[k] = c3[k] - c1[k]; }
+    double ds = 1.0 / ((double)N1), dt = 1.0 / ((double)N2);
+    double T1[6] = {0}, T2[6] = {0}, T3[6] = {0}, T4[6] = {0}, o[6];
+    const double lim[2] = {0.0, 1.0};
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            fint(a, b, c, d, e, F, lim[i], lim[j], o);
+            for (int q = 0; q < 6; q++) T1[q] += o[q];
+        }
+    double s = 0.0, t = 0.0;
+    for (int i = 0; i <= N1; i++) {
+        t = 0.0;
+        s += ds;
+        double fs = (i % 2 == 0) ? 2.0 : 4.0;
+        if (i > 0 && i < N1)
+            for (int l = 0; l < 2; l++) {
+                fint(a, b, c, d, e, F, s, lim[l], o);
+                for (int q = 0; q < 6; q++) T2[q] += fs * o[q];
+            }
+        for (int j = 0; j <= N2; j++) {
+            t += dt;
+            double ft = (j % 2 == 0) ? 2.0 : 4.0;
+            if (j > 0 && j < N2) {
+                if (i == 0) {
+                    for (int l = 0; l < 2; l++) {
+                        fint(a, b, c, d, e, F, lim[l], t, o);
+                        for (int q = 0; q < 6; q++) T3[q] += ft * o[q];
+                    }
+                } else if (i < N1) {
+                    fint(a, b, c, d, e, F, s, t, o);
+                    for (int q = 0; q < 6; q++) T4[q] += fs * ft * o[q];
+                }
+            }
+        }
+    }
+    double in[6];
+    for (int q = 0; q < 6; q++) in[q] = (ds * dt / 9.0) * (T1[q] + T2[q] + T3[q] + T4[q]);
+    for (int k = 0; k < 3; k++) {
+        f12[0 + k] = 4.0 * kr * (-in[0] * vC[k] + in[1] * (vC[k] - vA[k]) + in[2] * vA[k] + in[4] * vB[k] - in[3] * vB[k]);
+        f12[3 + k] = 4.0 * kr * (-in[1] * vC[k] - in[2] * vA[k] + in[3] * vB[k]);
+        f12[6 + k] = 4.0 * kr * (in[0] * vC[k] + in[1] * vA[k] - in[4] * (vB[k] + vC[k]) - in[3] * vA[k] + in[5] * vB[k]);
+        f12[9 + k] = 4.0 * kr * (in[4] * vC[k] + in[3] * vA[k] - in[5] * vB[k]);
+    }
+}
+
+static inline int ev_energy_bad(double U) { return fabs(U) == INFINITY || U != U || U < -1.0; }
+
+/* CylinderExclVolRepulsion::energy, :329-575. The per-call side table of pair
+ * energies (cylEnergies, :558-565) has no arithmetic effect and is omitted. */
+double orc_ev_energy(const double *x, int32_t n, const int32_t *bs, const double *krep, const double *eql) {
+    double U = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+        double kr = krep[i] * eql[2 * i] * eql[2 * i + 1];
+        const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
+        double Ui = orc_ev_pair_energy_analytic(c1, c2, c3, c4, kr);
+        if (ev_energy_bad(Ui)) {
+            /* energyN without bead moves; its retry result is discarded by the reference (:989-995) */
+            Ui = orc_ev_pair_energy_simpson(c1, c2, c3, c4, kr, 400, 400);
+            if (ev_energy_bad(Ui)) return -1;
+        }
+        U += Ui;
+    }
+    return U;
+}
+
+/* Analytic force of one pair, :626-860. out[4][3] = forces on c1..c4, with the
+ * direction vectors taken from the (possibly moved) c2. */
+static void ev_pair_force_analytic(const double *c1, const double *c2, const double *c3, const double *c4,
+                                   double kr, double out[12]) {
+    ev_inv v;
+    ev_invariants(c1, c2, c3, c4, &v);
+    const double a = v.a, b = v.b, c = v.c, d = v.d, e = v.e, F = v.F;
+    const double AA = v.AA, BB = v.BB, CC = v.CC, DD = v.DD, EE = v.EE, FF = v.FF, GG = v.GG, HH = v.HH;
+    const double ATG1 = v.ATG1, ATG2 = v.ATG2, ATG3 = v.ATG3, ATG4 = v.ATG4;
+    const double invJJ = 1 / v.JJ;
+    const double sumBlock = CC / AA * ATG1 + GG / EE * ATG2 + DD / BB * ATG3 + HH / FF * ATG4;
+    const double enFactor = kr * invJJ / 2;
+    /* derivatives of the arctan differences */
+    const double A1 = AA * AA / (AA * AA + (a + e) * (a + e));
+    const double A2 = AA * AA / (AA * AA + e * e);
+    const double E1 = EE * EE / (EE * EE + (a + e - d) * (a + e - d));
+    const double E2 = EE * EE / (EE * EE + (e - d) * (e - d));
+    const double B1 = BB * BB / (BB * BB + F * F);
+    const double B2 = BB * BB / (BB * BB + (F - b) * (F - b));
+    const double F1 = FF * FF / (FF * FF + (d + F) * (d + F));
+    const double F2 = FF * FF / (FF * FF + (d + F - b) * (d + F - b));
+    /* partials of the four bracketed blocks w.r.t. their free parameters */
+    const double pA_C = ATG1 / AA;
+    const double pA_A = -(ATG1 * CC) / (AA * AA) + (-A1 * (a + e) + A2 * e) * CC / (AA * AA * AA);
+    const double pA_e = ((A1 - A2) * CC) / (AA * AA);
+    const double pA_a = (A1 * CC) / (AA * AA);
+    const double pE_G = ATG2 / EE;
+    const double pE_E = -(ATG2 * GG) / (EE * EE) + (-E1 * (a + e - d) + E2 * (e - d)) * GG / (EE * EE * EE);
+    const double pE_emd = ((E1 - E2) * GG) / (EE * EE);
+    const double pE_a = (E1 * GG) / (EE * EE);
+    const double pB_D = ATG3 / BB;
+    const double pB_B = -(ATG3 * DD) / (BB * BB) + (-B1 * F + B2 * (F - b)) * DD / (BB * BB * BB);
+    const double pB_fmb = ((B1 - B2) * DD) / (BB * BB);
+    const double pB_b = (B1 * DD) / (BB * BB);
+    const double pF_H = ATG4 / FF;
+    const double pF_F = -(ATG4 * HH) / (FF * FF) + (-F1 * (d + F) + F2 * (d + F - b)) * HH / (FF * FF * FF);
+    const double pF_dfb = ((F1 - F2) * HH) / (FF * FF);
+    const double pF_b = (F1 * HH) / (FF * FF);
+    /* chain rule onto the basis v1 = c2-c1, v2 = c4-c3, v3 = c1-c3 (:703-760) */
+    const double k11 = pA_C * (-2 * F) + pA_A * (c / AA) + pA_a * 2 + pE_G * (2 * (F - b)) + pE_E * ((b + c - 2 * F) / EE) +
+                       pE_a * 2 + pF_H * (-2 * b) + pF_F * (b / FF);
+    const double k12 = pA_C * e + pE_G * (2 * d - e) + pE_E * ((e - d) / EE) + pE_emd * (-1) + pB_D * (-F) +
+                       pF_H * (2 * d + F) + pF_F * (-(d + F) / FF) + pF_dfb;
+    const double k13 = pA_C * d + pA_A * (-e / AA) + pA_e + pE_G * (-d) + pE_E * (-(e - d) / EE) + pE_emd + pB_D * b +
+                       pF_H * (-b) + pF_F * (b / FF);
+    const double k21 = k12;
+    const double k22 = pE_G * (-2 * a) + pE_E * (a / EE) + pB_D * (2 * e) + pB_B * (c / BB) + pB_fmb * (-2) + pB_b * 2 +
+                       pF_H * (-2 * (a + e)) + pF_F * ((a + c + 2 * e) / FF) + pF_dfb * (-2) + pF_b * 2;
+    const double k23 = pA_C * (-a) + pE_G * a + pE_E * (-a / EE) + pB_D * (-d) + pB_B * (-F / BB) + pB_fmb + pF_H * d +
+                       pF_F * (-(d + F) / FF) + pF_dfb;
+    const double k31 = k13;
+    const double k32 = k23;
+    const double k33 = pA_A * (a / AA) + pE_E * (a / EE) + pB_B * (b / BB) + pF_F * (b / FF);
+    /* derivative of JJ (:818-828) */
+    const double J11 = 2 * (-b * c + F * F), J12 = 2 * (c * d - e * F), J13 = 2 * (b * e - d * F);
+    const double J21 = 2 * (c * d - e * F), J22 = 2 * (-a * c + e * e), J23 = 2 * (-d * e + a * F);
+    const double J31 = 2 * (b * e - d * F), J32 = 2 * (-d * e + a * F), J33 = 2 * (d * d - a * b);
+    double dv[4][3];
+    dv[0][0] = enFactor * (-invJJ * sumBlock * (J31 - J11) + (k31 - k11));
+    dv[0][1] = enFactor * (-invJJ * sumBlock * (J32 - J12) + (k32 - k12));
+    dv[0][2] = enFactor * (-invJJ * sumBlock * (J33 - J13) + (k33 - k13));
+    dv[1][0] = enFactor * (-invJJ * sumBlock * (J11) + (k11));
+    dv[1][1] = enFactor * (-invJJ * sumBlock * (J12) + (k12));
+    dv[1][2] = enFactor * (-invJJ * sumBlock * (J13) + (k13));
+    dv[2][0] = enFactor * (-invJJ * sumBlock * (-J31 - J21) + (-k31 - k21));
+    dv[2][1] = enFactor * (-invJJ * sumBlock * (-J32 - J22) + (-k32 - k22));
+    dv[2][2] = enFactor * (-invJJ * sumBlock * (-J33 - J23) + (-k33 - k23));
+    dv[3][0] = enFactor * (-invJJ * sumBlock * (J21) + (k21));
+    dv[3][1] = enFactor * (-invJJ * sumBlock * (J22) + (k22));
+    dv[3][2] = enFactor * (-invJJ * sumBlock * (J23) + (k23));
+    for (int bi = 0; bi < 4; bi++)
+        for (int k = 0; k < 3; k++)
+            out[3 * bi + k] = -(dv[bi][0] * (c2[k] - c1[k]) + dv[bi][1] * (c4[k] - c3[k]) + dv[bi][2] * (c1[k] - c3[k]));
+}
+
+/* forceN with the reference's retry ladder (:1001-1254): first without bead
+ * moves; if still non-finite, with c1 and c2 moved off the plane when
+ * areInPlane holds; if still non-finite the reference exits. */
+static int ev_pair_force_fallback(const double *c1, const double *c2, const double *c3, const double *c4, double kr,
+                                  double out[12]) {
+    orc_ev_pair_force_simpson(c1, c2, c3, c4, kr, 10000, 10000, out);
+    int ok = 1;
+    for (int q = 0; q < 12; q++) ok &= isfinite(out[q]);
+    if (ok) return 0;
+    if (orc_are_in_plane(c1, c2, c3, c4)) {
+        double n2[3], n1[3];
+        move_out_of_plane(c1, c2, c3, c4, n2, 2, 0.01);
+        move_out_of_plane(c1, c2, c3, c4, n1, 1, 0.01);
+        orc_ev_pair_force_simpson(n1, n2, c3, c4, kr, 10000, 10000, out);
+    } else {
+        orc_ev_pair_force_simpson(c1, c2, c3, c4, kr, 10000, 10000, out);
+    }
+    ok = 1;
+    for (int q = 0; q < 12; q++) ok &= isfinite(out[q]);
+    return ok ? 0 : MDY_ERR_FORCE;
+}
+
+/* CylinderExclVolRepulsion::forces, :578-882 */
+int orc_ev_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *krep, const double *eql,
+                  int coplanar_mode) {
+    for (int32_t i = 0; i < n; i++) {
+        double kr = krep[i] * eql[2 * i] * eql[2 * i + 1];
+        const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
+        double nc2[3];
+        const double *c2u = c2;
+        if (coplanar_mode == MDY_COPLANAR_REFERENCE && orc_are_in_plane(c1, c2, c3, c4)) {
+            move_out_of_plane(c1, c2, c3, c4, nc2, 2, 0.01);
+            c2u = nc2;
+        }
+        double out[12];
+        ev_pair_force_analytic(c1, c2u, c3, c4, kr, out);
+        int ok = finite3(out) && finite3(out + 3) && finite3(out + 6) && finite3(out + 9);
+        if (!ok) {
+            int rc = ev_pair_force_fallback(c1, c2, c3, c4, kr, out);
+            if (rc) return rc;
+        }
+        double *f1 = f + bs[4 * i], *f2 = f + bs[4 * i + 1], *f3 = f + bs[4 * i + 2], *f4 = f + bs[4 * i + 3];
+        for (int k = 0; k < 3; k++) {
+            f1[k] += out[k];
+            f2[k] += out[3 + k];
+            f3[k] += out[6 + k];
+            f4[k] += out[9 + k];
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a10: BoundaryCylinderRepulsionExp (Boundary/BoundaryCylinderRepulsionExp.cpp:27-62, 65-110) */
+/* PlaneBoundaryElement::distance (Structure/BoundaryElementImpl.h:71-74)      */
+/* ------------------------------------------------------------------------- */
+static inline double plane_dist(const double *pl, const double *p) {
+    return (pl[0] * p[0] + pl[1] * p[1] + pl[2] * p[2] + pl[3]) /
+           sqrt(pow(pl[0], 2) + pow(pl[1], 2) + pow(pl[2], 2));
+}
+double orc_bd_energy(const double *x, int32_t nplanes, const double *plane, const int32_t *nneigh,
+                     const int32_t *bs, const double *krep, const double *slen) {
+    double U = 0.0;
+    int64_t cum = 0;
+    for (int32_t ib = 0; ib < nplanes; ib++) {
+        for (int32_t ic = 0; ic < nneigh[ib]; ic++) {
+            double r = plane_dist(plane + 4 * ib, x + bs[cum + ic]);
+            double R = -r / slen[cum + ic];
+            double Ui = krep[cum + ic] * exp(R);
+            U += Ui;
+        }
+        cum += nneigh[ib];
+    }
+    return U;
+}
+void orc_bd_forces(const double *x, double *f, int32_t nplanes, const double *plane, const int32_t *nneigh,
+                   const int32_t *bs, const double *krep, const double *slen) {
+    int64_t cum = 0;
+    for (int32_t ib = 0; ib < nplanes; ib++) {
+        const double *pl = plane + 4 * ib;
+        for (int32_t ic = 0; ic < nneigh[ib]; ic++) {
+            double r = plane_dist(pl, x + bs[cum + ic]);
+            double R = -r / slen[cum + ic];
+            double f0 = krep[cum + ic] * exp(R) / slen[cum + ic];
+            double *f1 = f + bs[cum + ic];
+            f1[0] += f0 * pl[0];
+            f1[1] += f0 * pl[1];
+            f1[2] += f0 * pl[2];
+        }
+        cum += nneigh[ib];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a13: neighbor lists                                                       */
+/* ------------------------------------------------------------------------- */
+/* Cylinder database midpoint 0.5*(b1+b2) (Structure/Cylinder.cpp:45-49). */
+static inline void cyl_mid(const mdy_cylinders *cyl, const double *x, int32_t i, double *m) {
+    const double *b1 = x + cyl->bead1[i], *b2 = x + cyl->bead2[i];
+    for (int k = 0; k < 3; k++) m[k] = 0.5 * (b1[k] + b2[k]);
+}
+
+/* Predicate of HybridCylinderCylinderNL::updateNeighborsbin
+ * (HybridNeighborListImpl.cpp:405-441): half list by id, same-filament
+ * neighbours within minCylinderDistanceSameFilament = 2 (SysParams.h:315)
+ * excluded, squared midpoint distance (double) against float rMin^2/rMax^2
+ * (HybridNeighborListImpl.h:78-80). The bin stencil only prunes candidates, so
+ * the brute-force pair set equals the reference's. */
+int64_t orc_nl_pairs(const mdy_cylinders *cyl, const double *x, float rmax, float rmin, int32_t *pairs, int64_t cap) {
+    float rmaxsq = rmax * rmax, rminsq = rmin * rmin;
+    int32_t n = cyl->n;
+    double *mid = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    for (int32_t i = 0; i < n; i++) cyl_mid(cyl, x, i, mid + 3 * i);
+    int64_t cnt = 0;
+    for (int32_t i = 0; i < n; i++) {
+        for (int32_t j = 0; j < n; j++) {
+            if (cyl->id[i] <= cyl->id[j]) continue;
+            if (cyl->filament[i] == cyl->filament[j]) {
+                int ds = abs(cyl->position[i] - cyl->position[j]);
+                if (ds <= 2) continue;
+            }
+            const double *m1 = mid + 3 * i, *m2 = mid + 3 * j;
+            double d2 = 0;
+            for (int k = 0; k < 3; k++) d2 += (m2[k] - m1[k]) * (m2[k] - m1[k]);
+            if (d2 < rminsq || d2 > rmaxsq) continue;
+            if (cnt < cap) { pairs[2 * cnt] = i; pairs[2 * cnt + 1] = j; }
+            cnt++;
+        }
+    }
+    free(mid);
+    return cnt;
+}
+
+/* CylinderExclVolume::vectorize (Volume/CylinderExclVolume.cpp:34-179):
+ * drop pairs whose neighbour is the mother of the cylinder
+ * (cin->getBranchingCylinder() == ci); k is the max over types when they
+ * differ. */
+int64_t orc_ev_vectorize(const mdy_cylinders *cyl, const int32_t *pairs, int64_t npairs, int32_t *bs, double *krep,
+                         double *eql) {
+    int64_t m = 0;
+    for (int64_t p = 0; p < npairs; p++) {
+        int32_t ci = pairs[2 * p], cn = pairs[2 * p + 1];
+        if (cyl->branch_child[cn] == ci) continue;
+        bs[4 * m] = cyl->bead1[ci]; bs[4 * m + 1] = cyl->bead2[ci];
+        bs[4 * m + 2] = cyl->bead1[cn]; bs[4 * m + 3] = cyl->bead2[cn];
+        eql[2 * m] = cyl->eq_length[ci]; eql[2 * m + 1] = cyl->eq_length[cn];
+        if (cyl->type[ci] != cyl->type[cn]) krep[m] = smax(cyl->exvol_k[ci], cyl->exvol_k[cn]);
+        else krep[m] = cyl->exvol_k[ci];
+        m++;
+    }
+    return m;
+}
+
+/* BoundaryCylinderNL::updateNeighbors (NeighborListImpl.cpp:738-751) with the
+ * float cutoff (NeighborList.h:38-39), then BoundaryCylinderRepulsion::vectorize
+ * (BoundaryCylinderRepulsion.cpp:33-88): the first bead of minus-end cylinders,
+ * then the second bead of every listed cylinder. */
+int64_t orc_bd_vectorize(const mdy_cylinders *cyl, const double *x, const mdy_boundary *bnd, int32_t *nneigh,
+                         int32_t *bs, double *krep, double *slen, int64_t cap) {
+    int64_t m = 0;
+    for (int32_t ib = 0; ib < bnd->n_planes; ib++) {
+        const double *pl = bnd->plane + 4 * ib;
+        int32_t cnt = 0;
+        for (int32_t i = 0; i < cyl->n; i++) {
+            double mc[3];
+            const double *b1 = x + cyl->bead1[i], *b2 = x + cyl->bead2[i];
+            /* Cylinder::coordinate = midPointCoordinate(b1, b2, 0.5) */
+            for (int k = 0; k < 3; k++) mc[k] = b1[k] * (1.0 - 0.5) + 0.5 * b2[k];
+            double dist = plane_dist(pl, mc);
+            if (!(dist < bnd->cutoff)) continue;
+            if (cyl->minus_end[i]) {
+                if (m < cap) { bs[m] = cyl->bead1[i]; krep[m] = bnd->k[ib]; slen[m] = bnd->screen[ib]; }
+                m++; cnt++;
+            }
+            if (m < cap) { bs[m] = cyl->bead2[i]; krep[m] = bnd->k[ib]; slen[m] = bnd->screen[ib]; }
+            m++; cnt++;
+        }
+        nneigh[ib] = cnt;
+    }
+    return m;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a11: ForceFieldManager                                                    */
+/* ------------------------------------------------------------------------- */
+int orc_system_setup(orc_system *o, const mdy_system *sys, const double *x) {
+    memset(o, 0, sizeof(*o));
+    o->sys = *sys;
+    const mdy_cylinders *cyl = &sys->cyl;
+    if (sys->exvol_enabled) {
+        int64_t cap = 1024;
+        int32_t *pairs = NULL;
+        int64_t np;
+        for (;;) {
+            pairs = (int32_t *)realloc(pairs, sizeof(int32_t) * 2 * (size_t)cap);
+            np = orc_nl_pairs(cyl, x, sys->exvol_cutoff, 0.0f, pairs, cap);
+            if (np <= cap) break;
+            cap = np;
+        }
+        o->ev_bs = (int32_t *)malloc(sizeof(int32_t) * 4 * (size_t)(np + 1));
+        o->ev_krep = (double *)malloc(sizeof(double) * (size_t)(np + 1));
+        o->ev_eql = (double *)malloc(sizeof(double) * 2 * (size_t)(np + 1));
+        o->ev_n = orc_ev_vectorize(cyl, pairs, np, o->ev_bs, o->ev_krep, o->ev_eql);
+        free(pairs);
+    }
+    if (sys->boundary_enabled && sys->bnd.n_planes > 0) {
+        int64_t cap = 2 * (int64_t)cyl->n * sys->bnd.n_planes + 1;
+        o->bd_nneigh = (int32_t *)malloc(sizeof(int32_t) * (size_t)sys->bnd.n_planes);
+        o->bd_bs = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+        o->bd_krep = (double *)malloc(sizeof(double) * (size_t)cap);
+        o->bd_slen = (double *)malloc(sizeof(double) * (size_t)cap);
+        o->bd_n = orc_bd_vectorize(cyl, x, &sys->bnd, o->bd_nneigh, o->bd_bs, o->bd_krep, o->bd_slen, cap);
+    }
+    return 0;
+}
+
+void orc_system_free(orc_system *o) {
+    free(o->ev_bs); free(o->ev_krep); free(o->ev_eql);
+    free(o->bd_nneigh); free(o->bd_bs); free(o->bd_krep); free(o->bd_slen);
+    memset(o, 0, sizeof(*o));
+}
+
+static void ffm_each(const orc_system *o, const double *x, double *e) {
+    const mdy_ff_tables *t = &o->sys.ff;
+    e[MDY_FF_FS] = orc_fs_energy(x, t->fs_n, t->fs_beadset, t->fs_k, t->fs_l0);
+    e[MDY_FF_FB] = orc_fb_energy(x, t->fb_n, t->fb_beadset, t->fb_k, t->fb_theta0);
+    e[MDY_FF_LS] = orc_ls_energy(x, t->ls_n, t->ls_beadset, t->ls_k, t->ls_l0, t->ls_pos1, t->ls_pos2);
+    e[MDY_FF_MS] = orc_ls_energy(x, t->ms_n, t->ms_beadset, t->ms_k, t->ms_l0, t->ms_pos1, t->ms_pos2);
+    e[MDY_FF_BS] = orc_bs_energy(x, t->bs_n, t->bs_beadset, t->bs_k, t->bs_l0, t->bs_pos);
+    e[MDY_FF_BB] = orc_bb_energy(x, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0);
+    e[MDY_FF_EV] = o->sys.exvol_enabled ? orc_ev_energy(x, (int32_t)o->ev_n, o->ev_bs, o->ev_krep, o->ev_eql) : 0.0;
+    e[MDY_FF_BD] = (o->sys.boundary_enabled && o->sys.bnd.n_planes > 0)
+                       ? orc_bd_energy(x, o->sys.bnd.n_planes, o->sys.bnd.plane, o->bd_nneigh, o->bd_bs, o->bd_krep,
+                                       o->bd_slen)
+                       : 0.0;
+}
+
+/* ForceFieldManager::computeEnergy, ForceFieldManager.cpp:96-207: fixed-order
+ * sum; an energy <= -1 from any force field makes the total +inf. */
+double orc_ffm_energy(const orc_system *o, const double *x, double *per_ff) {
+    double e[MDY_NFF];
+    ffm_each(o, x, e);
+    if (per_ff) memcpy(per_ff, e, sizeof(e));
+    double energy = 0.0;
+    for (int k = 0; k < MDY_NFF; k++) {
+        if (e[k] <= -1) return INFINITY;
+        energy += e[k];
+    }
+    return energy;
+}
+/* ForceFieldManager::computeIndividualEnergies, :209-227 (no sentinel) */
+double orc_ffm_energy_individual(const orc_system *o, const double *x, double *per_ff) {
+    double e[MDY_NFF];
+    ffm_each(o, x, e);
+    if (per_ff) memcpy(per_ff, e, sizeof(e));
+    double total = 0.0;
+    for (int k = 0; k < MDY_NFF; k++) total += e[k];
+    return total;
+}
+/* ForceFieldManager::computeForces, :242-290 */
+int orc_ffm_forces(const orc_system *o, const double *x, double *f, int32_t nvar) {
+    const mdy_ff_tables *t = &o->sys.ff;
+    for (int32_t i = 0; i < nvar; i++) f[i] = 0.0;
+    orc_fs_forces(x, f, t->fs_n, t->fs_beadset, t->fs_k, t->fs_l0);
+    orc_fb_forces(x, f, t->fb_n, t->fb_beadset, t->fb_k, t->fb_theta0);
+    orc_ls_forces(x, f, t->ls_n, t->ls_beadset, t->ls_k, t->ls_l0, t->ls_pos1, t->ls_pos2, NULL);
+    orc_ls_forces(x, f, t->ms_n, t->ms_beadset, t->ms_k, t->ms_l0, t->ms_pos1, t->ms_pos2, NULL);
+    orc_bs_forces(x, f, t->bs_n, t->bs_beadset, t->bs_k, t->bs_l0, t->bs_pos, NULL);
+    orc_bb_forces(x, f, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0, NULL);
+    if (o->sys.exvol_enabled) {
+        int rc = orc_ev_forces(x, f, (int32_t)o->ev_n, o->ev_bs, o->ev_krep, o->ev_eql, o->coplanar_mode);
+        if (rc) return rc;
+    }
+    if (o->sys.boundary_enabled && o->sys.bnd.n_planes > 0)
+        orc_bd_forces(x, f, o->sys.bnd.n_planes, o->sys.bnd.plane, o->bd_nneigh, o->bd_bs, o->bd_krep, o->bd_slen);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a12: CGMethod::minimize<method> + backtrackingLineSearch                  */
+/* (Minimizer/CGMethod.hpp:238-373, 826-1743; CGMethod.cpp:43-360;            */
+/*  MinimizationTypes.hpp:10-64)                                             */
+/* ------------------------------------------------------------------------- */
+typedef struct cg_state {
+    int32_t nvar, ndof;
+    double *force, *forcePrev, *searchDir, *coordLS, *coordMinE;
+    int haveMinE;
+    double minimumE, maxForceBackup, totalEnergy;
+} cg_state;
+
+static double cg_maxF(const cg_state *s) {
+    double m = 0;
+    for (int32_t i = 0; i < s->ndof; i++) m = smax(m, fabs(s->force[i]));
+    return m;
+}
+static double cg_FdotF(const cg_state *s) {          /* full vector, CGMethod.cpp:325-331 */
+    double r = 0.0;
+    for (int32_t i = 0; i < s->nvar; i++) r += s->force[i] * s->force[i];
+    return r;
+}
+static double cg_FdotFprev(const cg_state *s) {
+    double r = 0.0;
+    for (int32_t i = 0; i < s->ndof; i++) r += s->force[i] * s->forcePrev[i];
+    return r;
+}
+static double cg_SdotF(const cg_state *s) {
+    double r = 0.0;
+    for (int32_t i = 0; i < s->ndof; i++) r += s->searchDir[i] * s->force[i];
+    return r;
+}
+static void cg_shift(cg_state *s, double beta) {
+    for (int32_t i = 0; i < s->ndof; i++) s->searchDir[i] = s->force[i] + beta * s->searchDir[i];
+}
+
+static inline int call_limit_exceeded(const mdy_min_result *r) {
+    return (r->energy_call_limit != 0 && r->num_energy_call > r->energy_call_limit) ||
+           (r->force_call_limit != 0 && r->num_force_call > r->force_call_limit);
+}
+static inline int converged(const mdy_min_result *r) {
+    return r->grad_inf_norm < r->grad_tol || r->energy_rel_change < r->energy_rel_tol;
+}
+
+/* backtrackingLineSearch, CGMethod.hpp:238-373 */
+static double line_search(cg_state *s, const double *x, orc_energy_fn efn, void *user, double MAXDIST,
+                          double maxForce, double LAMBDAMAX, double armijo, mdy_min_result *r, double *newEnergy) {
+    *newEnergy = 0;
+    if (maxForce == 0.0) return 0.0;
+    double lambda = smin(LAMBDAMAX, MAXDIST / maxForce);
+    double currentEnergy = efn(user, x, NULL, 0);
+    r->num_energy_call++;
+    double expectedSlope = cg_SdotF(s);
+    double reducedSlope = armijo * expectedSlope;
+    int done = 0;
+    double energyLambda = 0.0;
+    while (!done) {
+        for (int32_t i = 0; i < s->ndof; i++) s->coordLS[i] = x[i] + lambda * s->searchDir[i];
+        energyLambda = efn(user, s->coordLS, NULL, 0);
+        r->num_energy_call++;
+        double ideal = -reducedSlope * lambda;
+        if (lambda < 0 || ideal > 0) { done = 1; lambda = 0.0; }
+        double energyChange = energyLambda - currentEnergy;
+        if (energyChange <= ideal) done = 1;
+        else lambda *= 0.5;
+        if (lambda <= 0.0 || lambda <= 1e-8) {
+            done = 1;
+            lambda = 0.0;
+            energyLambda = currentEnergy;
+        }
+    }
+    s->totalEnergy = energyLambda;
+    *newEnergy = energyLambda;
+    return lambda;
+}
+
+static void reset_cg(cg_state *s, int method, double *lastFDotF) {
+    cg_shift(s, 0);
+    if (method == MDY_CG_FLETCHER_RIEVES || method == MDY_CG_POLAK_RIBIERE) *lastFDotF = cg_FdotF(s);
+    if (method == MDY_CG_POLAK_RIBIERE) memcpy(s->forcePrev, s->force, sizeof(double) * (size_t)s->nvar);
+}
+
+static void trace_push(orc_trace *t, double lam, double e, double mf, double beta, int safe) {
+    if (!t || t->n >= t->cap) return;
+    t->lambda[t->n] = lam; t->energy[t->n] = e; t->maxf[t->n] = mf; t->beta[t->n] = beta; t->safe[t->n] = safe;
+    t->n++;
+}
+
+int orc_minimize(const mdy_cg_params *p, double *x, int32_t nvar, int32_t ndof, orc_energy_fn efn, orc_force_fn ffn,
+                 orc_recover_fn rfn, void *user, double *out_force, mdy_min_result *r, orc_trace *trace,
+                 int32_t iter_cap) {
+    memset(r, 0, sizeof(*r));
+    r->energy_call_limit = p->energy_call_limit;
+    r->grad_tol = p->grad_tol;
+    r->energy_rel_tol = p->energy_rel_tol;
+    r->force_call_limit = p->force_call_limit > 3 * p->n_beads_limit ? p->force_call_limit : 3 * p->n_beads_limit;
+    r->grad_inf_norm = INFINITY;
+    r->energy_rel_change = INFINITY;
+    if (trace) trace->n = 0;
+
+    cg_state s;
+    memset(&s, 0, sizeof(s));
+    s.nvar = nvar; s.ndof = ndof;
+    s.force = (double *)calloc((size_t)nvar + 1, sizeof(double));
+    s.forcePrev = (double *)calloc((size_t)nvar + 1, sizeof(double));
+    s.searchDir = (double *)calloc((size_t)nvar + 1, sizeof(double));
+    s.coordLS = (double *)calloc((size_t)nvar + 1, sizeof(double));
+    s.coordMinE = (double *)calloc((size_t)nvar + 1, sizeof(double));
+    s.minimumE = INFINITY;
+    int status = MDY_OK;
+
+    r->energies_before[MDY_NFF] = efn(user, x, r->energies_before, 1);
+    r->num_energy_call++;
+    status = ffn(user, x, s.force, nvar);
+    r->num_force_call++;
+    if (status) goto done;
+    r->grad_inf_norm = cg_maxF(&s);
+    if (!isfinite(r->grad_inf_norm)) { status = MDY_ERR_NONFINITE; goto done; }
+    double prevEnergy = r->energies_before[MDY_NFF];
+    int safeMode = 0;
+    int numIter = 0;
+    double lastFDotF = 0;
+    reset_cg(&s, p->method, &lastFDotF);
+
+    while (!call_limit_exceeded(r) && !converged(r)) {
+        if (iter_cap > 0 && numIter >= iter_cap) break;
+        double beta = 0, newGrad = 0, prevGrad = 0;
+        numIter++;
+        double newE;
+        double lambda = line_search(&s, x, efn, user, p->max_dist, r->grad_inf_norm, p->lambda_max,
+                                    safeMode ? 0 : 0.4, r, &newE);
+        if (lambda == 0) {
+            if (p->try_to_recover) {
+                if (safeMode) {
+                    if (rfn) {
+                        status = rfn(user, x, s.searchDir, ndof, p->max_dist);
+                        if (status) goto done;
+                    }
+                    r->num_recovery++;
+                    status = ffn(user, x, s.force, nvar);
+                    r->num_force_call++;
+                    if (status) goto done;
+                    r->grad_inf_norm = cg_maxF(&s);
+                    if (!isfinite(r->grad_inf_norm)) { status = MDY_ERR_NONFINITE; goto done; }
+                    reset_cg(&s, p->method, &lastFDotF);
+                    safeMode = 0;
+                } else {
+                    reset_cg(&s, p->method, &lastFDotF);
+                    safeMode = 1;
+                    r->num_safe_mode++;
+                }
+                trace_push(trace, 0.0, newE, r->grad_inf_norm, 0.0, safeMode);
+            } else {
+                r->error_flags |= MDY_ERROR_LINESEARCH;
+                r->num_iter = numIter;
+                trace_push(trace, 0.0, newE, r->grad_inf_norm, 0.0, safeMode);
+                status = MDY_OK;
+                goto done_noafter;
+            }
+        } else {
+            for (int32_t i = 0; i < ndof; i++) x[i] += lambda * s.searchDir[i];
+            status = ffn(user, x, s.force, nvar);
+            r->num_force_call++;
+            if (status) goto done;
+            r->grad_inf_norm = cg_maxF(&s);
+            if (!isfinite(r->grad_inf_norm)) { status = MDY_ERR_NONFINITE; goto done; }
+            r->energy_rel_change = fabs(newE - prevEnergy) / fabs(prevEnergy);
+            prevEnergy = newE;
+            if (p->method == MDY_CG_FLETCHER_RIEVES) newGrad = cg_FdotF(&s);
+            else if (p->method == MDY_CG_POLAK_RIBIERE) { newGrad = cg_FdotF(&s); prevGrad = cg_FdotFprev(&s); }
+            if (p->method == MDY_CG_FLETCHER_RIEVES) beta = newGrad / lastFDotF;
+            else if (p->method == MDY_CG_POLAK_RIBIERE) beta = smax(0.0, (newGrad - prevGrad) / lastFDotF);
+            cg_shift(&s, beta);
+            if (cg_SdotF(&s) <= 0) reset_cg(&s, p->method, &lastFDotF);
+            if (p->method == MDY_CG_POLAK_RIBIERE) memcpy(s.forcePrev, s.force, sizeof(double) * (size_t)nvar);
+            /* forceBelowRelaxedTolerance(10) && numIter > forceCallLimit/2 -> copycoordsifminimumE */
+            int relaxed = 1;
+            for (int32_t i = 0; i < ndof; i++)
+                if (fabs(s.force[i]) > 10 * p->grad_tol) { relaxed = 0; break; }
+            if (relaxed && numIter > r->force_call_limit / 2) {
+                if (s.totalEnergy <= s.minimumE) {
+                    s.minimumE = s.totalEnergy;
+                    s.maxForceBackup = r->grad_inf_norm;
+                    memcpy(s.coordMinE, x, sizeof(double) * (size_t)nvar);
+                    s.haveMinE = 1;
+                }
+            }
+            if (p->method == MDY_CG_FLETCHER_RIEVES || p->method == MDY_CG_POLAK_RIBIERE) lastFDotF = newGrad;
+            trace_push(trace, lambda, newE, r->grad_inf_norm, beta, safeMode);
+        }
+    }
+    r->num_iter = numIter;
+    if (call_limit_exceeded(r)) {
+        if (s.haveMinE) { memcpy(x, s.coordMinE, sizeof(double) * (size_t)nvar); s.haveMinE = 0; }
+    }
+    r->energies_after[MDY_NFF] = efn(user, x, r->energies_after, 1);
+    r->num_energy_call++;
+    if (out_force) memcpy(out_force, s.force, sizeof(double) * (size_t)nvar);
+    goto done_noafter;
+done:
+    r->num_iter = numIter;
+done_noafter:
+    r->status = status;
+    free(s.force); free(s.forcePrev); free(s.searchDir); free(s.coordLS); free(s.coordMinE);
+    return status;
+}
+
+/* MController::moveAlongDirectionLimitingBeadMove_ (Controller/MController.h:273-339) */
+int orc_recover_move(void *user, double *x, const double *dir, int32_t ndof, double maxdist) {
+    (void)user;
+    double step = INFINITY;
+    for (int32_t i = 0; i < ndof; i++) step = smin(step, maxdist / fabs(dir[i]));
+    if (step == INFINITY) return MDY_ERR_RECOVERY;
+    for (int32_t i = 0; i < ndof; i++) x[i] += step * dir[i];
+    return 0;
+}
+
+static double sys_energy(void *user, const double *x, double *per_ff, int individual) {
+    const orc_system *o = (const orc_system *)user;
+    return individual ? orc_ffm_energy_individual(o, x, per_ff) : orc_ffm_energy(o, x, per_ff);
+}
+static int sys_forces(void *user, const double *x, double *f, int32_t nvar) {
+    return orc_ffm_forces((const orc_system *)user, x, f, nvar);
+}
+
+int orc_minimize_system(const orc_system *o, const mdy_cg_params *p, double *x, double *out_force,
+                        mdy_min_result *res, orc_trace *trace, int32_t iter_cap) {
+    int32_t nvar = 3 * o->sys.n_beads;
+    return orc_minimize(p, x, nvar, nvar, sys_energy, sys_forces, orc_recover_move, (void *)o, out_force, res, trace,
+                        iter_cap);
+}

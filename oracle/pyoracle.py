@@ -1,0 +1,246 @@
+"""ctypes binding of the CPU oracle (oracle/libmdy_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from medyan_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PARITY = os.path.join(HERE, "libmdy_oracle.so")       # -O2 -ffp-contract=off (parity)
+LIB_FAST = os.path.join(HERE, "libmdy_oracle_fast.so")    # reference release flags (timing)
+
+_f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+
+
+def build(quiet=True):
+    subprocess.run(["make", "-C", HERE], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+class OrcTrace(C.Structure):
+    _fields_ = [("cap", C.c_int64), ("n", C.c_int64), ("lambda_", _f64p), ("energy", _f64p),
+                ("maxf", _f64p), ("beta", _f64p), ("safe", _i32p)]
+
+
+class OrcSystem(C.Structure):
+    _fields_ = [("sys", _abi.System),
+                ("ev_n", C.c_int64), ("ev_bs", _i32p), ("ev_krep", _f64p), ("ev_eql", _f64p),
+                ("bd_n", C.c_int64), ("bd_nneigh", _i32p), ("bd_bs", _i32p), ("bd_krep", _f64p),
+                ("bd_slen", _f64p), ("coplanar_mode", C.c_int)]
+
+
+ENERGY_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, _f64p, _f64p, C.c_int)
+FORCE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, _f64p, _f64p, C.c_int32)
+RECOVER_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, _f64p, _f64p, C.c_int32, C.c_double)
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Oracle:
+    def __init__(self, fast=False):
+        path = LIB_FAST if fast else LIB_PARITY
+        if not os.path.exists(path):
+            build()
+        self.lib = C.CDLL(path)
+        L = self.lib
+        L.orc_ffm_energy.restype = C.c_double
+        L.orc_ffm_energy_individual.restype = C.c_double
+        L.orc_ev_energy.restype = C.c_double
+        L.orc_ev_pair_energy_analytic.restype = C.c_double
+        L.orc_ev_pair_energy_simpson.restype = C.c_double
+        for fn in ("orc_fs_energy", "orc_fb_energy", "orc_ls_energy", "orc_bs_energy", "orc_bb_energy",
+                   "orc_bd_energy"):
+            getattr(L, fn).restype = C.c_double
+        L.orc_nl_pairs.restype = C.c_int64
+        L.orc_ev_vectorize.restype = C.c_int64
+        L.orc_bd_vectorize.restype = C.c_int64
+        L.orc_minimize.argtypes = [C.POINTER(_abi.CGParams), _f64p, C.c_int32, C.c_int32, ENERGY_FN, FORCE_FN,
+                                   RECOVER_FN, C.c_void_p, _f64p, C.POINTER(_abi.MinResult), C.POINTER(OrcTrace),
+                                   C.c_int32]
+
+    # -- whole system -----------------------------------------------------
+    def system(self, net, coord=None, coplanar_mode=_abi.MDY_COPLANAR_REFERENCE):
+        return OracleSystem(self, net, net.coord if coord is None else coord, coplanar_mode)
+
+    # -- single-pair helpers ----------------------------------------------
+    def ev_pair_energy(self, c1, c2, c3, c4, kr):
+        a = [np.ascontiguousarray(c, dtype=np.float64) for c in (c1, c2, c3, c4)]
+        return self.lib.orc_ev_pair_energy_analytic(*[_p(x) for x in a], C.c_double(kr))
+
+    def ev_pair_energy_simpson(self, c1, c2, c3, c4, kr, n1=400, n2=400):
+        a = [np.ascontiguousarray(c, dtype=np.float64) for c in (c1, c2, c3, c4)]
+        return self.lib.orc_ev_pair_energy_simpson(*[_p(x) for x in a], C.c_double(kr), n1, n2)
+
+    def ev_pair_force_simpson(self, c1, c2, c3, c4, kr, n1, n2):
+        a = [np.ascontiguousarray(c, dtype=np.float64) for c in (c1, c2, c3, c4)]
+        out = np.zeros(12)
+        self.lib.orc_ev_pair_force_simpson(*[_p(x) for x in a], C.c_double(kr), n1, n2, _p(out))
+        return out.reshape(4, 3)
+
+    def are_in_plane(self, c1, c2, c3, c4):
+        a = [np.ascontiguousarray(c, dtype=np.float64) for c in (c1, c2, c3, c4)]
+        return bool(self.lib.orc_are_in_plane(*[_p(x) for x in a]))
+
+    # -- per-FF on explicit tables ----------------------------------------
+    def ev_energy(self, x, bs, krep, eql):
+        x, bs, krep, eql = (np.ascontiguousarray(x, np.float64), np.ascontiguousarray(bs, np.int32),
+                            np.ascontiguousarray(krep, np.float64), np.ascontiguousarray(eql, np.float64))
+        return self.lib.orc_ev_energy(_p(x), len(krep), _p(bs, C.c_int32), _p(krep), _p(eql))
+
+    def ev_forces(self, x, bs, krep, eql, coplanar_mode=0):
+        x, bs, krep, eql = (np.ascontiguousarray(x, np.float64), np.ascontiguousarray(bs, np.int32),
+                            np.ascontiguousarray(krep, np.float64), np.ascontiguousarray(eql, np.float64))
+        f = np.zeros_like(x)
+        rc = self.lib.orc_ev_forces(_p(x), _p(f), len(krep), _p(bs, C.c_int32), _p(krep), _p(eql), coplanar_mode)
+        if rc:
+            raise RuntimeError(_abi.ERRORS.get(rc, rc))
+        return f
+
+    def bd_energy(self, x, plane, nneigh, bs, krep, slen):
+        arrs = [np.ascontiguousarray(a, t) for a, t in ((x, np.float64), (plane, np.float64), (nneigh, np.int32),
+                                                        (bs, np.int32), (krep, np.float64), (slen, np.float64))]
+        x, plane, nneigh, bs, krep, slen = arrs
+        return self.lib.orc_bd_energy(_p(x), len(nneigh), _p(plane), _p(nneigh, C.c_int32), _p(bs, C.c_int32),
+                                      _p(krep), _p(slen))
+
+    def bd_forces(self, x, plane, nneigh, bs, krep, slen):
+        arrs = [np.ascontiguousarray(a, t) for a, t in ((x, np.float64), (plane, np.float64), (nneigh, np.int32),
+                                                        (bs, np.int32), (krep, np.float64), (slen, np.float64))]
+        x, plane, nneigh, bs, krep, slen = arrs
+        f = np.zeros_like(x)
+        self.lib.orc_bd_forces(_p(x), _p(f), len(nneigh), _p(plane), _p(nneigh, C.c_int32), _p(bs, C.c_int32),
+                               _p(krep), _p(slen))
+        return f
+
+    def nl_pairs(self, net, coord=None, rmax=None, rmin=0.0):
+        coord = net.coord if coord is None else np.ascontiguousarray(coord, np.float64)
+        sysc = net.system()
+        rmax = net.exvol_cutoff if rmax is None else rmax
+        cap = 4 * net.n_cyl + 16
+        while True:
+            out = np.zeros(2 * cap, dtype=np.int32)
+            n = self.lib.orc_nl_pairs(C.byref(sysc.cyl), _p(coord), C.c_float(rmax), C.c_float(rmin),
+                                      _p(out, C.c_int32), C.c_int64(cap))
+            if n <= cap:
+                return out[: 2 * n].reshape(-1, 2)
+            cap = n
+
+    # -- generic minimizer over Python callables ----------------------------
+    def minimize(self, params, x, ndof, energy, force, recover=None, iter_cap=0, trace_cap=0):
+        """Run the oracle CGMethod on Python callables (used for the reference's
+        quadratic test problem). energy(x)->float, force(x, f) fills f."""
+        nvar = len(x)
+
+        @ENERGY_FN
+        def efn(user, xp, per_ff, individual):
+            xv = np.ctypeslib.as_array(xp, shape=(nvar,))
+            return float(energy(xv))
+
+        @FORCE_FN
+        def ffn(user, xp, fp, n):
+            xv = np.ctypeslib.as_array(xp, shape=(nvar,))
+            fv = np.ctypeslib.as_array(fp, shape=(n,))
+            force(xv, fv)
+            return 0
+
+        @RECOVER_FN
+        def rfn(user, xp, dp, nd, md):
+            if recover is not None:
+                recover(np.ctypeslib.as_array(xp, shape=(nvar,)), np.ctypeslib.as_array(dp, shape=(nd,)), md)
+            return 0
+
+        xa = np.ascontiguousarray(x, dtype=np.float64).copy()
+        fout = np.zeros(nvar)
+        res = _abi.MinResult()
+        tr, keep = _make_trace(trace_cap)
+        self.lib.orc_minimize(C.byref(params), _p(xa), nvar, ndof, efn, ffn, rfn, None, _p(fout), C.byref(res),
+                              C.byref(tr) if tr is not None else None, iter_cap)
+        return xa, fout, res, _read_trace(tr, keep)
+
+
+def _make_trace(cap):
+    if not cap:
+        return None, None
+    keep = dict(lam=np.zeros(cap), e=np.zeros(cap), m=np.zeros(cap), b=np.zeros(cap),
+                s=np.zeros(cap, dtype=np.int32))
+    tr = OrcTrace(cap, 0, _p(keep["lam"]), _p(keep["e"]), _p(keep["m"]), _p(keep["b"]), _p(keep["s"], C.c_int32))
+    return tr, keep
+
+
+def _read_trace(tr, keep):
+    if tr is None:
+        return None
+    n = tr.n
+    return dict(lambda_=keep["lam"][:n].copy(), energy=keep["e"][:n].copy(), maxf=keep["m"][:n].copy(),
+                beta=keep["b"][:n].copy(), safe=keep["s"][:n].copy())
+
+
+class OracleSystem:
+    """orc_system: the reference ForceFieldManager over one network (neighbor
+    lists built from `coord` at construction, like MController::run)."""
+
+    def __init__(self, oracle, net, coord, coplanar_mode):
+        self.o = oracle
+        self.net = net
+        self.sysc = net.system()
+        self.coord0 = np.ascontiguousarray(coord, dtype=np.float64).copy()
+        self.s = OrcSystem()
+        oracle.lib.orc_system_setup(C.byref(self.s), C.byref(self.sysc), _p(self.coord0))
+        self.s.coplanar_mode = coplanar_mode
+
+    def __del__(self):
+        try:
+            self.o.lib.orc_system_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    @property
+    def ev_table(self):
+        n = self.s.ev_n
+        if n == 0:
+            return np.zeros((0, 4), np.int32), np.zeros(0), np.zeros((0, 2))
+        bs = np.ctypeslib.as_array(self.s.ev_bs, shape=(4 * n,)).reshape(-1, 4).copy()
+        kr = np.ctypeslib.as_array(self.s.ev_krep, shape=(n,)).copy()
+        eq = np.ctypeslib.as_array(self.s.ev_eql, shape=(2 * n,)).reshape(-1, 2).copy()
+        return bs, kr, eq
+
+    @property
+    def bd_table(self):
+        n = self.s.bd_n
+        npl = self.sysc.bnd.n_planes
+        nn = np.ctypeslib.as_array(self.s.bd_nneigh, shape=(npl,)).copy() if npl else np.zeros(0, np.int32)
+        bs = np.ctypeslib.as_array(self.s.bd_bs, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+        return nn, bs
+
+    def energy(self, x=None, individual=False):
+        x = self.coord0 if x is None else np.ascontiguousarray(x, np.float64)
+        per = np.zeros(_abi.MDY_NFF)
+        fn = self.o.lib.orc_ffm_energy_individual if individual else self.o.lib.orc_ffm_energy
+        tot = fn(C.byref(self.s), _p(x), _p(per))
+        return tot, per
+
+    def forces(self, x=None):
+        x = self.coord0 if x is None else np.ascontiguousarray(x, np.float64)
+        f = np.zeros_like(x)
+        rc = self.o.lib.orc_ffm_forces(C.byref(self.s), _p(x), _p(f), len(x))
+        if rc:
+            raise RuntimeError(_abi.ERRORS.get(rc, rc))
+        return f
+
+    def minimize(self, params, x=None, iter_cap=0, trace_cap=0):
+        x = (self.coord0 if x is None else np.ascontiguousarray(x, np.float64)).copy()
+        fout = np.zeros_like(x)
+        res = _abi.MinResult()
+        tr, keep = _make_trace(trace_cap)
+        self.o.lib.orc_minimize_system(C.byref(self.s), C.byref(params), _p(x), _p(fout), C.byref(res),
+                                       C.byref(tr) if tr is not None else None, iter_cap)
+        return x, fout, res, _read_trace(tr, keep)
