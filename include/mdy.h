@@ -227,8 +227,8 @@ int mdy_set_trace(mdy_ctx *ctx, int64_t cap);
 int mdy_last_timing(mdy_ctx *ctx, double *ms_total);
 /* Phase profile of the last minimization: the device timestamp (100 MHz
  * wall clock) at the end of each grid-wide phase and the phase kind
- * (1 energy, 2 energy fallback, 3 forces, 4 force fallback, 5 gather,
- * 6 reduction, 7 restore). Recording is off unless a capacity is set. */
+ * (1 energy, 2 energy fallback, 3 energy and forces, 4 force fallback,
+ * 5 gather, 6 reduction, 7 restore). Recording is off unless a capacity is set. */
 int mdy_set_profile(mdy_ctx *ctx, int64_t cap);
 int mdy_get_profile(mdy_ctx *ctx, int32_t *kind, int64_t *ticks, int64_t cap, int64_t *n);
 

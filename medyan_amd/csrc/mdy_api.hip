@@ -18,10 +18,10 @@ Dev Ctx::dev() const {
     D.n_bd = have_nl ? n_bd : 0;
     D.ev_bs = ev_bs.p; D.ev_kr = ev_kr.p;
     D.fs_bs = fs_bs.p; D.fs_k = fs_k.p; D.fs_l0 = fs_l0.p;
-    D.fb_bs = fb_bs.p; D.fb_k = fb_k.p; D.fb_th = fb_th.p;
+    D.fb_bs = fb_bs.p; D.fb_k = fb_k.p; D.fb_th = fb_th.p; D.fb_ct = fb_ct.p; D.fb_st = fb_st.p;
     D.lm_bs = lm_bs.p; D.lm_k = lm_k.p; D.lm_l0 = lm_l0.p; D.lm_p1 = lm_p1.p; D.lm_p2 = lm_p2.p;
     D.bs_bs = bs_bs.p; D.bs_k = bs_k.p; D.bs_l0 = bs_l0.p; D.bs_p = bs_p.p;
-    D.bb_bs = bb_bs.p; D.bb_k = bb_k.p; D.bb_th = bb_th.p;
+    D.bb_bs = bb_bs.p; D.bb_k = bb_k.p; D.bb_th = bb_th.p; D.bb_ct = bb_ct.p; D.bb_st = bb_st.p;
     D.bd_bead = bd_bead.p; D.bd_plane = bd_plane.p; D.bd_k = bd_k.p; D.bd_sl = bd_sl.p; D.planes = planes.p;
     D.slot_fs = 0;
     D.slot_fb = D.slot_fs + 2 * n_fs;
@@ -119,7 +119,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     Ctx &c = ctx->c;
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
-    release(c.fs_bs); release(c.fs_k); release(c.fs_l0); release(c.fb_bs); release(c.fb_k); release(c.fb_th);
+    release(c.fs_bs); release(c.fs_k); release(c.fs_l0); release(c.fb_bs); release(c.fb_k); release(c.fb_th); release(c.fb_ct); release(c.fb_st); release(c.bb_ct); release(c.bb_st);
     release(c.lm_bs); release(c.lm_k); release(c.lm_l0); release(c.lm_p1); release(c.lm_p2);
     release(c.bs_bs); release(c.bs_k); release(c.bs_l0); release(c.bs_p); release(c.bb_bs); release(c.bb_k);
     release(c.bb_th); release(c.st_off); release(c.st_idx);
@@ -183,6 +183,15 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
     rc |= upload(c.fb_bs, t.fb_beadset, 3 * (size_t)t.fb_n, st);
     rc |= upload(c.fb_k, t.fb_k, t.fb_n, st);
     rc |= upload(c.fb_th, t.fb_theta0, t.fb_n, st);
+    // cos/sin of the equilibrium angles from the host C library, the values the
+    // CPU path computes inside its loops (FilamentBendingCosine.cpp:226, 299)
+    std::vector<double> fct(t.fb_n), fst(t.fb_n), bct(t.bb_n), bst(t.bb_n);
+    for (int i = 0; i < t.fb_n; i++) { fct[i] = std::cos(t.fb_theta0[i]); fst[i] = std::sin(t.fb_theta0[i]); }
+    for (int i = 0; i < t.bb_n; i++) { bct[i] = std::cos(t.bb_theta0[i]); bst[i] = std::sin(t.bb_theta0[i]); }
+    rc |= upload(c.fb_ct, fct.data(), t.fb_n, st);
+    rc |= upload(c.fb_st, fst.data(), t.fb_n, st);
+    rc |= upload(c.bb_ct, bct.data(), t.bb_n, st);
+    rc |= upload(c.bb_st, bst.data(), t.bb_n, st);
     // linkers then motors in one spring table
     {
         const size_t nl = (size_t)t.ls_n + t.ms_n;

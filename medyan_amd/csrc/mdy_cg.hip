@@ -4,13 +4,17 @@
 //
 // One cooperative launch; every block runs the same control flow and derives
 // every decision from grid reductions that all blocks evaluate in the same
-// fixed order, so decisions are identical without a broadcast. Phases are
-// separated by a grid barrier (atomic arrival counter + agent-scope fences).
-// An accepted iteration costs three barriers:
-//   [forces at x+lam*d -> slots] | [gather F, x += lam*d, max|F|, F.F, F.Fprev]
-//   | [beta, s = F + beta*d, s.F, first trial energy of the next line search]
-// The energy at the accepted point is the line-search trial energy of the same
-// coordinates (bitwise), so E(x) is not recomputed.
+// fixed order (exact sums), so decisions are identical without a broadcast.
+// Phases are separated by a flag-array grid barrier.
+//
+// Every line-search trial evaluates energy AND force slots at the trial point,
+// so an accepted trial needs no second pass over the interactions. An
+// accepted iteration costs two barriers:
+//   [gather F at the accepted point, x += lam*d, max|F|, F.F, F.Fprev]
+//   | [beta, s = F + beta*d, s.F, energy+forces at x + lam0*s (the next
+//      line search's first trial)]
+// The energy at the accepted point is the trial energy of the same
+// coordinates (bitwise), so E(x) is never recomputed.
 #include "mdy_eval.hpp"
 #include "mdy_launch.hpp"
 
@@ -23,7 +27,7 @@ struct CgArgs {
     double *x, *F0, *F1, *S0, *S1, *xmin, *contrib;
     double *red;          // 3 * nblk * kRed
     double *qe_u;         // 3 * kQueueCap
-    double *spart;        // nblk * kRed
+    double *spart;        // 6 * nblk * kRed
     int *qe, *qf, *qn;    // 3 * kQueueCap each; qn: 3 x 4 counters
     unsigned long long *bar;  // [0] abort flag; block b arrival flag at [64 + 16 b]
     double *out;          // result record
@@ -51,7 +55,16 @@ enum {
 __device__ __forceinline__ int ld_i(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // phase kinds recorded by the profile
-enum { PK_ENERGY = 1, PK_EFALLBACK, PK_FORCE, PK_FFALLBACK, PK_GATHER, PK_REDUCE, PK_RESTORE };
+enum { PK_ENERGY = 1, PK_EFALLBACK, PK_EVAL, PK_FFALLBACK, PK_GATHER, PK_REDUCE, PK_RESTORE };
+
+// result of one evaluation: e[0..7] per FF, e[8] computeEnergy total
+// (+inf sentinel), e[9] plain total; the force-fallback queue it filled
+struct Eval {
+    double e[kNFF + 2];
+    double xa;     // extra exact accumulator (s.F), if requested
+    int nqf;       // queued force fallbacks
+    int qb;        // queue buffer (0..2)
+};
 
 struct Cg {
     const CgArgs &A;
@@ -59,20 +72,21 @@ struct Cg {
     int G, blk, tid, gtid, gs;
     int ph = 0;                  // phase counter (uniform)
     bool aborted = false;
+    int status = MDY_OK;
 
     __device__ Cg(const CgArgs &a, double *l) : A(a), lds(l) {
         G = gridDim.x; blk = blockIdx.x; tid = threadIdx.x;
         gtid = blk * blockDim.x + tid; gs = G * blockDim.x;
     }
-    __device__ Dev dv(int p) const {
+    __device__ long long *ldsl() const { return reinterpret_cast<long long *>(lds); }
+    __device__ Dev dv(int b) const {
         Dev d = A.D;
-        int b = p % 3;
         d.qe = A.qe + b * kQueueCap; d.qe_n = A.qn + 4 * b;
         d.qf = A.qf + b * kQueueCap; d.qf_n = A.qn + 4 * b + 1;
         return d;
     }
     __device__ double *red(int p) const { return A.red + (size_t)(p % 3) * G * kRed; }
-    __device__ double *qeu(int p) const { return A.qe_u + (size_t)(p % 3) * kQueueCap; }
+    __device__ double *qeu(int b) const { return A.qe_u + (size_t)b * kQueueCap; }
 
     // Grid barrier. Arrival: one agent-scope release per block, then a store of
     // the phase number to the block's own flag line (no shared counter).
@@ -123,66 +137,77 @@ struct Cg {
         return !aborted;
     }
 
-    // ---- energy of all interactions at positions `pos`; e[kNFF]=sentinel total, e[kNFF+1]=plain sum
-    // extra: optional per-thread value reduced alongside (returned in *extra_out)
-    template <class P>
-    __device__ bool energy(const P &pos, double e[kNFF + 2], double extra_in = 0.0, double *extra_out = nullptr) {
-        const int p0 = ph;
-        Dev d = dv(p0);
-        double acc[kNFF + 1];
+    // ---- all interactions at `pos`: energy (exact total; per FF if PER_FF) and,
+    // if FORCES, the force slots. xa: optional per-thread exact accumulator
+    // reduced alongside (its total lands in ev.xa). Returns false on abort;
+    // a queue overflow sets `status`.
+    template <bool PER_FF, bool FORCES, class P>
+    __device__ bool eval(const P &pos, Eval &ev, mdy_xsum *xa) {
+        const int p0 = ph, qb = p0 % 3;
+        Dev d = dv(qb);
+        double acc[kNFF];
 #pragma unroll
         for (int k = 0; k < kNFF; k++) acc[k] = 0.0;
-        acc[kNFF] = extra_in;
+        mdy_xsum xs;
+        mdy_xs_zero(&xs);
         const int total = d.n_items();
-        for (int i = gtid; i < total; i += gs) energy_item(d, i, pos, acc);
-        block_reduce<kNFF + 1>(acc, lds);
-        if (tid < kNFF + 1) red(p0)[(size_t)blk * kRed + tid] = acc[tid];
-        if (!sync(PK_ENERGY)) return false;
-        const int nq = min(ld_i(d.qe_n), kQueueCap);
-        if (nq > 0) {  // quadrature fallback, one block per queued pair
+        for (int i = gtid; i < total; i += gs) item<PER_FF, FORCES>(d, i, pos, acc, xs, A.contrib);
+        double *r = red(p0);
+        if (PER_FF) {
+            block_reduce<kNFF>(acc, lds);
+            if (tid < kNFF) r[(size_t)blk * kRed + tid] = acc[tid];
+        }
+        block_reduce_xsum(xs, ldsl());
+        if (tid == 0) store_xsum(r, blk, kXsE, xs);
+        if (xa) {
+            block_reduce_xsum(*xa, ldsl());
+            if (tid == 0) store_xsum(r, blk, kXsA, *xa);
+        }
+        if (!sync(FORCES ? PK_EVAL : PK_ENERGY)) return false;
+        const int nqe = ld_i(d.qe_n);
+        ev.nqf = FORCES ? ld_i(d.qf_n) : 0;
+        ev.qb = qb;
+        if (nqe > kQueueCap || ev.nqf > kQueueCap) {
+            status = MDY_ERR_CAPACITY;
+            return true;
+        }
+        if (nqe > 0) {  // quadrature fallback energies, one block per queued pair
             const int N = 400;
-            for (int q = blk; q < nq; q += G) {
+            for (int q = blk; q < nqe; q += G) {
                 const int i = ld_i(d.qe + q);
                 int4 b = d.ev_bs[i];
                 SimpsonGeo g = simpson_geo(pos(b.x), pos(b.y), pos(b.z), pos(b.w));
-                double a1[1] = {0.0};
+                mdy_xsum X;
+                mdy_xs_zero(&X);
+                if (tid == 0) mdy_xs_add(&X, simpson_e_corners(g));
                 for (int node = tid; node < (N + 1) * (N + 1); node += blockDim.x)
-                    a1[0] += simpson_e_node(g, node / (N + 1), node % (N + 1), N);
-                block_reduce<1>(a1, lds);
+                    simpson_e_node(g, node / (N + 1), node % (N + 1), N, X);
+                block_reduce_xsum(X, ldsl());
                 if (tid == 0) {
                     const double ds = 1.0 / ((double)N);
-                    qeu(p0)[q] = d.ev_kr[i] * (ds * ds / 9.0) * (simpson_e_corners(g) + a1[0]);
+                    qeu(qb)[q] = d.ev_kr[i] * (ds * ds / 9.0) * mdy_xs_value(&X);
                 }
             }
             if (!sync(PK_EFALLBACK)) return false;
         }
-        energy_epilogue(d, red(p0), G, nq, qeu(p0), lds, e);
-        if (extra_out) {
-            double v[1];
-            // the extra column sits at index kNFF of the same partials
-            v[0] = 0.0;
-            for (int b = tid; b < G; b += blockDim.x) v[0] += __builtin_nontemporal_load(red(p0) + (size_t)b * kRed + kNFF);
-            block_reduce<1>(v, lds);
-            *extra_out = v[0];
+        energy_epilogue(d, r, G, nqe, d.qe, qeu(qb), lds, ev.e, PER_FF);
+        if (xa) {
+            mdy_xsum X;
+            grid_reduce_xsum(r, G, kXsA, X, ldsl());
+            ev.xa = mdy_xs_value(&X);
         }
         return true;
     }
 
-    // ---- forces at `pos` into Fout; optionally x += lam*d for every bead.
-    // Returns max|F_i|, F.F and F.Fprev (Fprev may be null).
+    // ---- quadrature forces for the pairs an accepted evaluation queued
+    // (CylinderExclVolRepulsion.cpp:1001-1254, 1017-1022 retry with the first
+    // cylinder moved off the plane). The slots are written by block 0.
     template <class P>
-    __device__ bool forces(const P &pos, double *Fout, const double *Fprev, bool move, const double *dvec, double lam,
-                           double &maxF, double &FF, double &FFp, int &status) {
-        const int p0 = ph;
-        Dev d = dv(p0);
-        const int total = d.n_items();
-        for (int i = gtid; i < total; i += gs) force_item(d, i, pos, A.contrib);
-        if (!sync(PK_FORCE)) return false;
-        const int nq = ld_i(d.qf_n);
-        if (nq > kQueueCap) { status = MDY_ERR_CAPACITY; return true; }
-        for (int q = 0; q < nq; q++) {
-            const int i = ld_i(d.qf + q);
-            int4 b = d.ev_bs[i];
+    __device__ bool force_fallback(const P &pos, const Eval &ev) {
+        const int *qf = A.qf + ev.qb * kQueueCap;
+        for (int q = 0; q < ev.nqf; q++) {
+            const int i = ld_i(qf + q);
+            int4 b = A.D.ev_bs[i];
             bool done = false;
             for (int moved = 0; moved < 2 && !done; moved++) {
                 d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
@@ -193,57 +218,87 @@ struct Cg {
                 }
                 SimpsonGeo g = simpson_geo(c1, c2, c3, c4);
                 const int N = 10000;
-                double o[6] = {0, 0, 0, 0, 0, 0};
                 const long long nodes = (long long)(N + 1) * (N + 1);
-                for (long long nd = gtid; nd < nodes; nd += gs) simpson_f_node(g, (int)(nd / (N + 1)), (int)(nd % (N + 1)), N, o);
-                block_reduce<6>(o, lds);
-                if (tid < 6) A.spart[(size_t)blk * kRed + tid] = o[tid];
+                for (int comp = 0; comp < 6; comp++) {
+                    mdy_xsum X;
+                    mdy_xs_zero(&X);
+                    if (gtid == 0) simpson_f_corners(g, comp, X);
+                    for (long long nd = gtid; nd < nodes; nd += gs)
+                        simpson_f_node(g, (int)(nd / (N + 1)), (int)(nd % (N + 1)), N, comp, X);
+                    block_reduce_xsum(X, ldsl());
+                    if (tid == 0) store_xsum(A.spart + (size_t)comp * G * kRed, blk, 0, X);
+                }
                 if (!sync(PK_FFALLBACK)) return false;
                 double T[6];
-                grid_partials_reduce<6>(A.spart, G, T, lds);
-                double cn[6] = {0, 0, 0, 0, 0, 0};
-                fint6(g, 0.0, 0.0, 1.0, cn); fint6(g, 0.0, 1.0, 1.0, cn);
-                fint6(g, 1.0, 0.0, 1.0, cn); fint6(g, 1.0, 1.0, 1.0, cn);
-                for (int k = 0; k < 6; k++) T[k] = cn[k] + T[k];
+                for (int comp = 0; comp < 6; comp++) {
+                    mdy_xsum X;
+                    grid_reduce_xsum(A.spart + (size_t)comp * G * kRed, G, 0, X, ldsl());
+                    T[comp] = mdy_xs_value(&X);
+                }
                 d3 f[4];
-                simpson_f_assemble(g, T, d.ev_kr[i], N, f);
+                simpson_f_assemble(g, T, A.D.ev_kr[i], N, f);
                 const bool ok = finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]);
                 if (ok) {
                     if (blk == 0 && tid == 0) {
-                        int s = d.slot_ev + 4 * i;
+                        int s = A.D.slot_ev + 4 * i;
                         put3(A.contrib, s, f[0]); put3(A.contrib, s + 1, f[1]);
                         put3(A.contrib, s + 2, f[2]); put3(A.contrib, s + 3, f[3]);
                     }
                     done = true;
                 } else if (moved || !inplane) {
-                    status = MDY_ERR_FORCE;
+                    status = MDY_ERR_FORCE;   // the reference exits here
                     return true;
                 }
-                if (!sync(PK_FFALLBACK)) return false;   // spart reuse / contrib visibility
+                if (!sync(PK_FFALLBACK)) return false;   // spart reuse / slot visibility
             }
         }
-        // gather + optional move + dot products
-        double v[3] = {0.0, 0.0, 0.0};  // max|F|, F.F, F.Fprev
-        for (int b = gtid; b < d.nb; b += gs) {
-            d3 f = gather_bead(d, A.contrib, b);
+        return true;
+    }
+
+    // ---- per-bead forces from the slots into Fout; x += lam*d when `move`.
+    // Returns max|F_i|, F.F and F.Fprev (Fprev may be null), sums exact.
+    __device__ bool gather(double *Fout, const double *Fprev, bool move, const double *dvec, double lam, double &maxF,
+                           double &FF, double &FFp) {
+        const int p1 = ph;
+        double m[1] = {0.0};
+        mdy_xsum sff, sfp;
+        mdy_xs_zero(&sff);
+        mdy_xs_zero(&sfp);
+        for (int b = gtid; b < A.D.nb; b += gs) {
+            d3 f = gather_bead(A.D, A.contrib, b);
             const int o = 3 * b;
             Fout[o] = f.x; Fout[o + 1] = f.y; Fout[o + 2] = f.z;
-            v[0] = fmax(v[0], fmax(fabs(f.x), fmax(fabs(f.y), fabs(f.z))));
-            v[1] += f.x * f.x; v[1] += f.y * f.y; v[1] += f.z * f.z;
-            if (Fprev) { v[2] += f.x * Fprev[o]; v[2] += f.y * Fprev[o + 1]; v[2] += f.z * Fprev[o + 2]; }
+            m[0] = fmax(m[0], fmax(fabs(f.x), fmax(fabs(f.y), fabs(f.z))));
+            mdy_xs_add(&sff, f.x * f.x); mdy_xs_add(&sff, f.y * f.y); mdy_xs_add(&sff, f.z * f.z);
+            if (Fprev) {
+                mdy_xs_add(&sfp, f.x * Fprev[o]); mdy_xs_add(&sfp, f.y * Fprev[o + 1]);
+                mdy_xs_add(&sfp, f.z * Fprev[o + 2]);
+            }
             if (move) {
                 A.x[o] = A.x[o] + lam * dvec[o];
                 A.x[o + 1] = A.x[o + 1] + lam * dvec[o + 1];
                 A.x[o + 2] = A.x[o + 2] + lam * dvec[o + 2];
             }
         }
-        block_reduce<3>(v, lds, 0x1u);
-        const int p1 = ph;
-        if (tid < 3) red(p1)[(size_t)blk * kRed + tid] = v[tid];
+        block_reduce<1>(m, lds, 0x1u);
+        block_reduce_xsum(sff, ldsl());
+        double *r = red(p1);
+        if (tid == 0) { r[(size_t)blk * kRed] = m[0]; store_xsum(r, blk, 8, sff); }
+        if (Fprev) {
+            block_reduce_xsum(sfp, ldsl());
+            if (tid == 0) store_xsum(r, blk, 16, sfp);
+        }
         if (!sync(PK_GATHER)) return false;
-        double r[3];
-        grid_partials_reduce<3>(red(p1), G, r, lds, 0x1u);
-        maxF = r[0]; FF = r[1]; FFp = r[2];
+        grid_partials_reduce<1>(r, G, m, lds, 0x1u);
+        maxF = m[0];
+        mdy_xsum X;
+        grid_reduce_xsum(r, G, 8, X, ldsl());
+        FF = mdy_xs_value(&X);
+        FFp = 0.0;
+        if (Fprev) {
+            grid_reduce_xsum(r, G, 16, X, ldsl());
+            FFp = mdy_xs_value(&X);
+        }
         return true;
     }
 };
@@ -254,12 +309,11 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
     Cg K(A, lds);
     double *R = A.out;
     const bool leader = (K.blk == 0 && K.tid == 0);
-    int status = MDY_OK;
     unsigned flags = 0;
     int nEC = 0, nFC = 0, nIter = 0, nEE = 0, nSM = 0, nRec = 0;
     long long trn = 0;
     double gradInf = INFINITY, relE = INFINITY;
-    double e[kNFF + 2];
+    Eval ev;
     bool after = true;   // compute energiesAfter / output force
     int cur = 0, sidx = 0;
 #define FB(i) ((i) ? A.F1 : A.F0)
@@ -267,20 +321,23 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
     const double *dvec = nullptr;
     double maxF = 0, FF = 0, FFp = 0, FFcur = 0;
 
-    // energiesBefore = computeIndividualEnergies(x)
-    if (!K.energy(PosX{A.x}, e)) goto aborted;
+    // energiesBefore = computeIndividualEnergies(x); computeForces(x) from the same pass
+    if (!K.eval<true, true>(PosX{A.x}, ev, nullptr)) goto aborted;
+    if (K.status) { after = false; goto finish; }
     nEC++; nEE++;
-    if (leader) for (int k = 0; k < kNFF; k++) R[R_EB + k] = e[k];
-    if (leader) R[R_EB + kNFF] = e[kNFF + 1];
+    if (leader) {
+        for (int k = 0; k < kNFF; k++) R[R_EB + k] = ev.e[k];
+        R[R_EB + kNFF] = ev.e[kNFF + 1];
+    }
     {
-        double E0 = e[kNFF];           // computeEnergy at the same x (sentinel form)
-        bool E0valid = true;
-        double prevEnergy = e[kNFF + 1];
-        if (!K.forces(PosX{A.x}, FB(cur), nullptr, false, nullptr, 0.0, maxF, FF, FFp, status)) goto aborted;
+        double E0 = ev.e[kNFF];           // computeEnergy at the same x (sentinel form)
+        double prevEnergy = ev.e[kNFF + 1];
+        if (ev.nqf && !K.force_fallback(PosX{A.x}, ev)) goto aborted;
+        if (K.status) { after = false; goto finish; }
+        if (!K.gather(FB(cur), nullptr, false, nullptr, 0.0, maxF, FF, FFp)) goto aborted;
         nFC++;
-        if (status) { after = false; goto finish; }
         gradInf = maxF;
-        if (!isfinite(gradInf)) { status = MDY_ERR_NONFINITE; after = false; goto finish; }
+        if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; goto finish; }
         FFcur = FF;
         // resetCGParams: s = F, lastFDotF = F.F, Fprev = F
         dvec = FB(cur);
@@ -289,7 +346,8 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
         double minimumE = INFINITY;
         bool haveMin = false;
         bool specValid = false;
-        double specLam = 0, specE = 0;
+        double specLam = 0;
+        Eval spec;
         auto callLimit = [&]() {
             return (A.energy_call_limit != 0 && nEC > A.energy_call_limit) ||
                    (A.force_call_limit != 0 && nFC > A.force_call_limit);
@@ -302,23 +360,18 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
             double lam = 0.0, newE = 0.0;
             if (maxF != 0.0) {
                 lam = smin(A.lambda_max, A.max_dist / maxF);
-                if (!E0valid) {
-                    if (!K.energy(PosX{A.x}, e)) goto aborted;
-                    nEE++;
-                    E0 = e[kNFF];
-                    E0valid = true;
-                }
-                nEC++;
+                nEC++;   // currentEnergy = computeEnergy(x): E0, known bitwise from the accepting trial
                 const double reducedSlope = armijo * slope;
                 double Et = 0.0;
                 for (;;) {
                     if (specValid && specLam == lam) {
-                        Et = specE;
+                        ev = spec;
                     } else {
-                        if (!K.energy(PosLine{A.x, dvec, lam}, e)) goto aborted;
+                        if (!K.eval<false, true>(PosLine{A.x, dvec, lam}, ev, nullptr)) goto aborted;
                         nEE++;
-                        Et = e[kNFF];
+                        if (K.status) { after = false; goto finish; }
                     }
+                    Et = ev.e[kNFF];
                     specValid = false;
                     nEC++;
                     const double ideal = -reducedSlope * lam;
@@ -345,19 +398,23 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
                         if (!K.sync(PK_REDUCE)) goto aborted;
                         double st[1];
                         grid_partials_reduce<1>(K.red(p1), K.G, st, lds, 0u, 0x1u);
-                        if (st[0] == INFINITY) { status = MDY_ERR_RECOVERY; after = false; goto finish; }
+                        if (st[0] == INFINITY) { K.status = MDY_ERR_RECOVERY; after = false; goto finish; }
+                        const PosLine pl{A.x, dvec, st[0]};
+                        if (!K.eval<false, true>(pl, ev, nullptr)) goto aborted;
+                        nEE++;
+                        if (K.status) { after = false; goto finish; }
+                        if (ev.nqf && !K.force_fallback(pl, ev)) goto aborted;
+                        if (K.status) { after = false; goto finish; }
                         double *Fn = FB(1 - cur);
-                        if (!K.forces(PosLine{A.x, dvec, st[0]}, Fn, FB(cur), true, dvec, st[0], maxF, FF, FFp, status))
-                            goto aborted;
+                        if (!K.gather(Fn, FB(cur), true, dvec, st[0], maxF, FF, FFp)) goto aborted;
                         nFC++; nRec++;
-                        if (status) { after = false; goto finish; }
                         gradInf = maxF;
-                        if (!isfinite(gradInf)) { status = MDY_ERR_NONFINITE; after = false; goto finish; }
+                        if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; goto finish; }
                         cur = 1 - cur;
                         FFcur = FF;
                         dvec = FB(cur); lastFDotF = FF; slope = FF;
                         safeMode = 0;
-                        E0valid = false;
+                        E0 = ev.e[kNFF];   // computeEnergy at the moved coordinates
                     } else {
                         dvec = FB(cur); lastFDotF = FFcur; slope = FFcur;
                         safeMode = 1;
@@ -380,17 +437,18 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
                 }
                 continue;
             }
-            // accepted step: x += lam*d, forces at the new x
+            // accepted step: the force slots hold the forces at x + lam*d
             {
+                const PosLine pl{A.x, dvec, lam};
+                if (ev.nqf && !K.force_fallback(pl, ev)) goto aborted;
+                if (K.status) { after = false; goto finish; }
                 double *Fn = FB(1 - cur);
                 const double *Fp = FB(cur);
-                if (!K.forces(PosLine{A.x, dvec, lam}, Fn, Fp, true, dvec, lam, maxF, FF, FFp, status)) goto aborted;
+                if (!K.gather(Fn, Fp, true, dvec, lam, maxF, FF, FFp)) goto aborted;
                 nFC++;
-                if (status) { after = false; goto finish; }
                 gradInf = maxF;
-                if (!isfinite(gradInf)) { status = MDY_ERR_NONFINITE; after = false; goto finish; }
+                if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; goto finish; }
                 E0 = newE;
-                E0valid = true;
                 relE = fabs(newE - prevEnergy) / fabs(prevEnergy);
                 prevEnergy = newE;
                 const double newGrad = FF, prevGrad = FFp;
@@ -398,32 +456,37 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
                 if (A.method == MDY_CG_FLETCHER_RIEVES) beta = newGrad / lastFDotF;
                 else if (A.method == MDY_CG_POLAK_RIBIERE) beta = smax(0.0, (newGrad - prevGrad) / lastFDotF);
                 // s = F + beta*s, its slope, and (speculatively) the first trial of the next line search
-                const bool more = !(callLimit() || converged()) && maxF != 0.0;
+                const bool cont = !(callLimit() || converged());
+                const bool more = cont && maxF != 0.0;
                 const double lam1 = more ? smin(A.lambda_max, A.max_dist / maxF) : 0.0;
                 double *Sn = SB(1 - sidx);
-                double sdot = 0.0;
-                for (int b = K.gtid; b < A.D.nb; b += K.gs) {
-                    const int o = 3 * b;
-                    for (int k = 0; k < 3; k++) {
-                        double s = Fn[o + k] + beta * dvec[o + k];
-                        Sn[o + k] = s;
-                        sdot += s * Fn[o + k];
-                    }
-                }
                 double sdF = 0.0;
-                if (more) {
-                    if (!K.energy(PosSpec{A.x, Fn, dvec, beta, lam1}, e, sdot, &sdF)) goto aborted;
-                    nEE++;
-                    specValid = true; specLam = lam1; specE = e[kNFF];
-                } else {
-                    double v[1] = {sdot};
-                    block_reduce<1>(v, lds);
-                    const int p1 = K.ph;
-                    if (K.tid == 0) K.red(p1)[(size_t)K.blk * kRed] = v[0];
-                    if (!K.sync(PK_REDUCE)) goto aborted;
-                    double r1[1];
-                    grid_partials_reduce<1>(K.red(p1), K.G, r1, lds);
-                    sdF = r1[0];
+                if (cont) {
+                    mdy_xsum sd;
+                    mdy_xs_zero(&sd);
+                    for (int b = K.gtid; b < A.D.nb; b += K.gs) {
+                        const int o = 3 * b;
+                        for (int k = 0; k < 3; k++) {
+                            double s = Fn[o + k] + beta * dvec[o + k];
+                            Sn[o + k] = s;
+                            mdy_xs_add(&sd, s * Fn[o + k]);
+                        }
+                    }
+                    if (more) {
+                        if (!K.eval<false, true>(PosSpec{A.x, Fn, dvec, beta, lam1}, spec, &sd)) goto aborted;
+                        nEE++;
+                        if (K.status) { after = false; goto finish; }
+                        specValid = true; specLam = lam1;
+                        sdF = spec.xa;
+                    } else {
+                        block_reduce_xsum(sd, K.ldsl());
+                        const int p1 = K.ph;
+                        if (K.tid == 0) store_xsum(K.red(p1), K.blk, kXsA, sd);
+                        if (!K.sync(PK_REDUCE)) goto aborted;
+                        mdy_xsum X;
+                        grid_reduce_xsum(K.red(p1), K.G, kXsA, X, K.ldsl());
+                        sdF = mdy_xs_value(&X);
+                    }
                 }
                 cur = 1 - cur;
                 sidx = 1 - sidx;
@@ -455,28 +518,26 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
     }
 finish:
     if (after) {
-        if (!K.energy(PosX{A.x}, e)) goto aborted;
+        if (!K.eval<true, false>(PosX{A.x}, ev, nullptr)) goto aborted;
         nEC++; nEE++;
         if (leader) {
-            for (int k = 0; k < kNFF; k++) R[R_EA + k] = e[k];
-            R[R_EA + kNFF] = e[kNFF + 1];
+            for (int k = 0; k < kNFF; k++) R[R_EA + k] = ev.e[k];
+            R[R_EA + kNFF] = ev.e[kNFF + 1];
         }
     }
     if (leader) {
         R[R_FLAGS] = flags; R[R_NEC] = nEC; R[R_NFC] = nFC; R[R_NIT] = nIter; R[R_GIN] = gradInf; R[R_ERC] = relE;
-        R[R_NEE] = nEE; R[R_NSM] = nSM; R[R_NREC] = nRec; R[R_STATUS] = status; R[R_TRN] = (double)trn;
+        R[R_NEE] = nEE; R[R_NSM] = nSM; R[R_NREC] = nRec; R[R_STATUS] = K.status; R[R_TRN] = (double)trn;
         R[R_LAST] = after ? 1.0 : 0.0; R[R_FOUT] = (double)cur; R[R_NPH] = (double)K.ph;
     }
     return;
 aborted:
     if (leader) R[R_STATUS] = MDY_ERR_TIMEOUT;
+#undef FB
+#undef SB
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_cg_copy(const double *src, double *dst, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
-}
-
 int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res) {
     hipStream_t st = c.stream;
     const int ndof = 3 * c.nb;
@@ -502,7 +563,7 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     c.coop_blocks = G;
     MDY_TRY(ensure(c.bar, 64 + 16 * (size_t)G));
     MDY_TRY(ensure(c.red, (size_t)3 * G * kRed));
-    MDY_TRY(ensure(c.simpson_part, (size_t)G * kRed));
+    MDY_TRY(ensure(c.simpson_part, (size_t)6 * G * kRed));
     MDY_HIP(hipMemsetAsync(c.bar.p, 0, (64 + 16 * (size_t)G) * sizeof(unsigned long long), st));
     MDY_HIP(hipMemsetAsync(c.qn.p, 0, 16 * sizeof(int), st));
     MDY_HIP(hipMemsetAsync(c.err.p, 0, sizeof(int), st));
@@ -518,12 +579,12 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     A.force_call_limit = p->force_call_limit > 3 * p->n_beads_limit ? p->force_call_limit : 3 * p->n_beads_limit;
     A.max_dist = p->max_dist; A.lambda_max = p->lambda_max; A.grad_tol = p->grad_tol;
     A.energy_rel_tol = p->energy_rel_tol;
-    A.spin_limit = 20000000LL;   // seconds of polling before the watchdog fires
+    A.spin_limit = 20000000LL;   // polls (about 10 s) before the watchdog fires
     hipEvent_t e0, e1;
     MDY_HIP(hipEventCreate(&e0));
     MDY_HIP(hipEventCreate(&e1));
-    MDY_HIP(hipEventRecord(e0, st));
     MDY_HIP(hipMemcpyAsync(c.cg_args.p, &A, sizeof(CgArgs), hipMemcpyHostToDevice, st));
+    MDY_HIP(hipEventRecord(e0, st));
     const CgArgs *Ad = reinterpret_cast<const CgArgs *>(c.cg_args.p);
     void *args[] = {&Ad};
     hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_cg), dim3(G), dim3(kBlock), args, 0, st);

@@ -3,13 +3,16 @@
 // Each function evaluates ONE interaction of one force field from bead
 // positions already in registers; the kernels decide where positions come
 // from (x, or x + lambda * d during a line search) and where forces go.
-// Expressions keep the operation order of the reference lines they cite so
-// that FP64 results reproduce the CPU path to rounding (built with
-// -ffp-contract=off); transcendental functions are the ROCm device library's.
+// Expressions keep the operation order of the reference lines they cite and
+// are built with -ffp-contract=off; arctangent and exponential are the shared
+// mdy_atan / mdy_exp (mdy_xmath.h) and cos/sin of equilibrium angles are
+// precomputed by the host, so the FP64 results equal the CPU path's bits.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+
+#include "mdy_xmath.h"
 
 namespace mdy {
 
@@ -64,7 +67,10 @@ __device__ __forceinline__ void fs_force(d3 c1, d3 c2, double k, double l0, d3 &
 // ---------------------------------------------------------------------------
 // a4 / a8 cosine bending (FilamentBendingCosine.cpp:199-398, BranchingBendingCosine.cpp:196-411)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double cosbend_energy(double l1l2, double L1, double L2, double k, double th0) {
+// cos(theta0) and sin(theta0) arrive precomputed by the host C library (the
+// same values the CPU path uses): ct = cos(th0), st = sin(th0).
+__device__ __forceinline__ double cosbend_energy(double l1l2, double L1, double L2, double k, double th0, double ct,
+                                                 double st) {
     double L1L2 = L1 * L2;
     double x = l1l2 / L1L2;
     if (x < -1.0) x = -1.0;
@@ -72,11 +78,11 @@ __device__ __forceinline__ double cosbend_energy(double l1l2, double L1, double 
     if (are_equal0(th0)) return k * (1 - x);
     double cosA = x;
     double sinA = smax(sqrt(1 - cosA * cosA), 0.0);
-    double cosAmB = cosA * cos(th0) + sinA * sin(th0);
+    double cosAmB = cosA * ct + sinA * st;
     return k * (1 - cosAmB);
 }
-__device__ __forceinline__ void cosbend_coef(double L1sq, double L2sq, double l1l2, double kb, double th0,
-                                             double &A, double &B, double &C, double &ke) {
+__device__ __forceinline__ void cosbend_coef(double L1sq, double L2sq, double l1l2, double kb, double th0, double ct,
+                                             double st, double &A, double &B, double &C, double &ke) {
     A = 1 / sqrt(L1sq * L2sq);
     double x = l1l2 * A;
     B = x / L1sq;
@@ -87,19 +93,20 @@ __device__ __forceinline__ void cosbend_coef(double L1sq, double L2sq, double l1
     else if (x > 1.0) x = 1.0;
     double cosp = x;
     double sinp = sqrt(smax((1 - cosp * cosp), 0.0));
-    double sinpmq = sinp * cos(th0) - cosp * sin(th0);
+    double sinpmq = sinp * ct - cosp * st;
     ke = kb * sinpmq / sinp;
 }
 // filament: beads c1, c2, c3; r1 = c2-c1, r2 = c3-c2
-__device__ __forceinline__ double fb_energy(d3 c1, d3 c2, d3 c3, double k, double th0) {
+__device__ __forceinline__ double fb_energy(d3 c1, d3 c2, d3 c3, double k, double th0, double ct, double st) {
     double L1 = sqrt(sprod(c1, c2, c1, c2));
     double L2 = sqrt(sprod(c2, c3, c2, c3));
     double l1l2 = sprod(c1, c2, c2, c3);
-    return cosbend_energy(l1l2, L1, L2, k, th0);
+    return cosbend_energy(l1l2, L1, L2, k, th0, ct, st);
 }
-__device__ __forceinline__ void fb_force(d3 c1, d3 c2, d3 c3, double k, double th0, d3 &f1, d3 &f2, d3 &f3) {
+__device__ __forceinline__ void fb_force(d3 c1, d3 c2, d3 c3, double k, double th0, double ct, double st, d3 &f1,
+                                         d3 &f2, d3 &f3) {
     double A, B, C, ke;
-    cosbend_coef(sprod(c1, c2, c1, c2), sprod(c2, c3, c2, c3), sprod(c1, c2, c2, c3), k, th0, A, B, C, ke);
+    cosbend_coef(sprod(c1, c2, c1, c2), sprod(c2, c3, c2, c3), sprod(c1, c2, c2, c3), k, th0, ct, st, A, B, C, ke);
     d3 r1 = sub(c2, c1), r2 = sub(c3, c2);
     d3 F1 = d3{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
     d3 F2 = d3{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
@@ -108,16 +115,16 @@ __device__ __forceinline__ void fb_force(d3 c1, d3 c2, d3 c3, double k, double t
     f3 = F2;
 }
 // branch: mother (c1, c2), daughter (c3, c4)
-__device__ __forceinline__ double bb_energy(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0) {
+__device__ __forceinline__ double bb_energy(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0, double ct, double st) {
     double L1 = sqrt(sprod(c1, c2, c1, c2));
     double L2 = sqrt(sprod(c3, c4, c3, c4));
     double l1l2 = sprod(c1, c2, c3, c4);
-    return cosbend_energy(l1l2, L1, L2, k, th0);
+    return cosbend_energy(l1l2, L1, L2, k, th0, ct, st);
 }
-__device__ __forceinline__ void bb_force(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0, d3 &f1, d3 &f2, d3 &f3,
-                                         d3 &f4) {
+__device__ __forceinline__ void bb_force(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0, double ct, double st, d3 &f1,
+                                         d3 &f2, d3 &f3, d3 &f4) {
     double A, B, C, ke;
-    cosbend_coef(sprod(c1, c2, c1, c2), sprod(c3, c4, c3, c4), sprod(c1, c2, c3, c4), k, th0, A, B, C, ke);
+    cosbend_coef(sprod(c1, c2, c1, c2), sprod(c3, c4, c3, c4), sprod(c1, c2, c3, c4), k, th0, ct, st, A, B, C, ke);
     d3 r1 = sub(c2, c1), r2 = sub(c4, c3);
     d3 F1 = d3{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
     d3 F2 = d3{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
@@ -178,12 +185,12 @@ __device__ __forceinline__ double plane_dist(const double *__restrict__ pl, d3 p
 __device__ __forceinline__ double bd_energy(const double *__restrict__ pl, d3 p, double k, double sl) {
     double r = plane_dist(pl, p);
     double R = -r / sl;
-    return k * exp(R);
+    return k * mdy_exp(R);
 }
 __device__ __forceinline__ d3 bd_force(const double *__restrict__ pl, d3 p, double k, double sl) {
     double r = plane_dist(pl, p);
     double R = -r / sl;
-    double f0 = k * exp(R) / sl;
+    double f0 = k * mdy_exp(R) / sl;
     return d3{f0 * pl[0], f0 * pl[1], f0 * pl[2]};
 }
 
@@ -239,24 +246,25 @@ __device__ __forceinline__ void ev_invariants(d3 c1, d3 c2, d3 c3, d3 c4, EvInv 
     d3 cp = cross3(sub(c1, c2), sub(c3, c4));
     double JJ = dot3(cp, sub(c3, c1));
     v.JJ = -JJ * JJ;
-    v.ATG1 = atan((a + e) / v.AA) - atan(e / v.AA);
-    v.ATG2 = atan((a + e - d) / v.EE) - atan((e - d) / v.EE);
-    v.ATG3 = atan((F) / v.BB) - atan((F - b) / v.BB);
-    v.ATG4 = atan((d + F) / v.FF) - atan((d + F - b) / v.FF);
+    v.ATG1 = mdy_atan((a + e) / v.AA) - mdy_atan(e / v.AA);
+    v.ATG2 = mdy_atan((a + e - d) / v.EE) - mdy_atan((e - d) / v.EE);
+    v.ATG3 = mdy_atan((F) / v.BB) - mdy_atan((F - b) / v.BB);
+    v.ATG4 = mdy_atan((d + F) / v.FF) - mdy_atan((d + F - b) / v.FF);
 }
 
+__device__ __forceinline__ double ev_energy_inv(const EvInv &v, double kr) {
+    return 0.5 * kr / v.JJ * (v.CC / v.AA * v.ATG1 + v.GG / v.EE * v.ATG2 + v.DD / v.BB * v.ATG3 + v.HH / v.FF * v.ATG4);
+}
 __device__ __forceinline__ double ev_energy(d3 c1, d3 c2, d3 c3, d3 c4, double kr) {
     EvInv v;
     ev_invariants(c1, c2, c3, c4, v);
-    return 0.5 * kr / v.JJ * (v.CC / v.AA * v.ATG1 + v.GG / v.EE * v.ATG2 + v.DD / v.BB * v.ATG3 + v.HH / v.FF * v.ATG4);
+    return ev_energy_inv(v, kr);
 }
 __device__ __forceinline__ bool ev_energy_bad(double U) { return fabs(U) == INFINITY || U != U || U < -1.0; }
 
 // Analytic pair force; f[0..3] = forces on c1..c4. Direction vectors use the
 // c2 passed in (the moved one in the coplanar case), as the reference does.
-__device__ __forceinline__ void ev_force(d3 c1, d3 c2, d3 c3, d3 c4, double kr, d3 f[4]) {
-    EvInv v;
-    ev_invariants(c1, c2, c3, c4, v);
+__device__ __forceinline__ void ev_force_inv(const EvInv &v, d3 c1, d3 c2, d3 c3, d3 c4, double kr, d3 f[4]) {
     const double a = v.a, b = v.b, c = v.c, d = v.d, e = v.e, F = v.F;
     const double AA = v.AA, BB = v.BB, CC = v.CC, DD = v.DD, EE = v.EE, FF = v.FF, GG = v.GG, HH = v.HH;
     const double ATG1 = v.ATG1, ATG2 = v.ATG2, ATG3 = v.ATG3, ATG4 = v.ATG4;
@@ -323,6 +331,11 @@ __device__ __forceinline__ void ev_force(d3 c1, d3 c2, d3 c3, d3 c4, double kr, 
         f[bi].y = -(dv[bi][0] * u1.y + dv[bi][1] * u2.y + dv[bi][2] * u3.y);
         f[bi].z = -(dv[bi][0] * u1.z + dv[bi][1] * u2.z + dv[bi][2] * u3.z);
     }
+}
+__device__ __forceinline__ void ev_force(d3 c1, d3 c2, d3 c3, d3 c4, double kr, d3 f[4]) {
+    EvInv v;
+    ev_invariants(c1, c2, c3, c4, v);
+    ev_force_inv(v, c1, c2, c3, c4, kr, f);
 }
 __device__ __forceinline__ bool finite_d3(d3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 

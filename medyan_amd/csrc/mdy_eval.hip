@@ -6,36 +6,43 @@
 
 namespace mdy {
 
+// energies: per-FF double sums at [0, kNFF), exact total at kXsE
 template <class P>
 __global__ void __launch_bounds__(kBlock) k_energy(Dev D, P pos, double *red) {
-    __shared__ double lds[17 * kNFF];
+    __shared__ double lds[17 * kNFF + 8];
     double acc[kNFF];
 #pragma unroll
     for (int k = 0; k < kNFF; k++) acc[k] = 0.0;
+    mdy_xsum xs;
+    mdy_xs_zero(&xs);
     const int total = D.n_items(), gs = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs) energy_item(D, i, pos, acc);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs)
+        item<true, false>(D, i, pos, acc, xs, nullptr);
     block_reduce<kNFF>(acc, lds);
+    block_reduce_xsum(xs, reinterpret_cast<long long *>(lds));
     if (threadIdx.x < kNFF) red[(size_t)blockIdx.x * kRed + threadIdx.x] = acc[threadIdx.x];
+    if (threadIdx.x == 0) store_xsum(red, blockIdx.x, kXsE, xs);
 }
 
 // One block per queued pair: 2-D Simpson energy (energyN, 400 x 400 intervals).
 template <class P>
 __global__ void __launch_bounds__(kBlock) k_energy_simpson(Dev D, P pos, double *qe_u) {
-    __shared__ double lds[17];
+    __shared__ long long lds[17 * 8];
     const int nq = min(*D.qe_n, kQueueCap);
     const int N = 400;
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
         const int i = D.qe[q];
         int4 b = D.ev_bs[i];
         SimpsonGeo g = simpson_geo(pos(b.x), pos(b.y), pos(b.z), pos(b.w));
-        double acc[1] = {0.0};
+        mdy_xsum X;
+        mdy_xs_zero(&X);
+        if (threadIdx.x == 0) mdy_xs_add(&X, simpson_e_corners(g));
         for (int node = threadIdx.x; node < (N + 1) * (N + 1); node += blockDim.x)
-            acc[0] += simpson_e_node(g, node / (N + 1), node % (N + 1), N);
-        block_reduce<1>(acc, lds);
+            simpson_e_node(g, node / (N + 1), node % (N + 1), N, X);
+        block_reduce_xsum(X, lds);
         if (threadIdx.x == 0) {
             const double ds = 1.0 / ((double)N);
-            double U = D.ev_kr[i] * (ds * ds / 9.0) * (simpson_e_corners(g) + acc[0]);
-            qe_u[q] = U;
+            qe_u[q] = D.ev_kr[i] * (ds * ds / 9.0) * mdy_xs_value(&X);
         }
         __syncthreads();
     }
@@ -43,9 +50,9 @@ __global__ void __launch_bounds__(kBlock) k_energy_simpson(Dev D, P pos, double 
 
 __global__ void __launch_bounds__(kBlock) k_energy_final(Dev D, const double *red, int nblk, const double *qe_u,
                                                          double *out) {
-    __shared__ double lds[17 * kNFF];
+    __shared__ double lds[17 * kNFF + 8];
     double e[kNFF + 2];
-    energy_epilogue(D, red, nblk, *D.qe_n, qe_u, lds, e);
+    energy_epilogue(D, red, nblk, *D.qe_n, D.qe, qe_u, lds, e, true);
     if (threadIdx.x == 0)
         for (int k = 0; k < kNFF + 2; k++) out[k] = e[k];
 }
@@ -53,16 +60,21 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(Dev D, const double *re
 template <class P>
 __global__ void __launch_bounds__(kBlock) k_force(Dev D, P pos, double *contrib) {
     const int total = D.n_items(), gs = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs) force_item(D, i, pos, contrib);
+    double acc[kNFF];
+    mdy_xsum xs;
+    mdy_xs_zero(&xs);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs)
+        item<false, true>(D, i, pos, acc, xs, contrib);
 }
 
-// Quadrature force (forceN, 10^4 x 10^4 intervals) of one queued pair, split
-// over the whole grid; partial sums per block.
-__global__ void __launch_bounds__(kBlock) k_force_simpson_part(Dev D, const double *x, int q, int moved,
+// Quadrature force (forceN, 10^4 x 10^4 intervals) of one queued pair,
+// integral component `comp`, split over the whole grid; exact partials per
+// block, one record array per component.
+__global__ void __launch_bounds__(kBlock) k_force_simpson_part(Dev D, const double *x, int qi, int moved, int comp,
                                                                double *part) {
-    __shared__ double lds[17 * 6];
+    __shared__ long long lds[17 * 8];
     const int N = 10000;
-    const int i = D.qf[q];
+    const int i = D.qf[qi];
     int4 b = D.ev_bs[i];
     PosX pos{x};
     d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
@@ -71,42 +83,44 @@ __global__ void __launch_bounds__(kBlock) k_force_simpson_part(Dev D, const doub
         c1 = n1; c2 = n2;
     }
     SimpsonGeo g = simpson_geo(c1, c2, c3, c4);
-    double o[6] = {0, 0, 0, 0, 0, 0};
+    mdy_xsum X;
+    mdy_xs_zero(&X);
+    if (blockIdx.x == 0 && threadIdx.x == 0) simpson_f_corners(g, comp, X);
     const long long nodes = (long long)(N + 1) * (N + 1);
     for (long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x; nd < nodes;
          nd += (long long)gridDim.x * blockDim.x)
-        simpson_f_node(g, (int)(nd / (N + 1)), (int)(nd % (N + 1)), N, o);
-    block_reduce<6>(o, lds);
-    if (threadIdx.x < 6) part[(size_t)blockIdx.x * kRed + threadIdx.x] = o[threadIdx.x];
+        simpson_f_node(g, (int)(nd / (N + 1)), (int)(nd % (N + 1)), N, comp, X);
+    block_reduce_xsum(X, lds);
+    if (threadIdx.x == 0) store_xsum(part + (size_t)comp * gridDim.x * kRed, blockIdx.x, 0, X);
 }
 
-// flag[0] = 1 if the result was non-finite (host retries with moved beads or fails)
-__global__ void __launch_bounds__(kBlock) k_force_simpson_final(Dev D, const double *x, int q, int moved,
+// flag[0] = 1 if the result was non-finite, flag[1] = pair in plane
+// (the host retries with moved beads or fails like the reference)
+__global__ void __launch_bounds__(kBlock) k_force_simpson_final(Dev D, const double *x, int qi, int moved,
                                                                 const double *part, int nblk, double *contrib,
                                                                 int *flag) {
-    __shared__ double lds[17 * 6];
+    __shared__ long long lds[17 * 8];
     const int N = 10000;
     double T[6];
-    grid_partials_reduce<6>(part, nblk, T, lds);
+    for (int q = 0; q < 6; q++) {
+        mdy_xsum X;
+        grid_reduce_xsum(part + (size_t)q * nblk * kRed, nblk, 0, X, lds);
+        T[q] = mdy_xs_value(&X);
+    }
     if (threadIdx.x != 0) return;
-    const int i = D.qf[q];
+    const int i = D.qf[qi];
     int4 b = D.ev_bs[i];
     PosX pos{x};
     d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+    flag[1] = are_in_plane(c1, c2, c3, c4) ? 1 : 0;
     if (moved) {
         d3 n2 = moved_off_plane(c1, c2, c3, c2, 0.01), n1 = moved_off_plane(c1, c2, c3, c1, 0.01);
         c1 = n1; c2 = n2;
     }
     SimpsonGeo g = simpson_geo(c1, c2, c3, c4);
-    double corners[6] = {0, 0, 0, 0, 0, 0};
-    fint6(g, 0.0, 0.0, 1.0, corners); fint6(g, 0.0, 1.0, 1.0, corners);
-    fint6(g, 1.0, 0.0, 1.0, corners); fint6(g, 1.0, 1.0, 1.0, corners);
-    for (int k = 0; k < 6; k++) T[k] = corners[k] + T[k];
     d3 f[4];
     simpson_f_assemble(g, T, D.ev_kr[i], N, f);
     bool ok = finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]);
-    // the retry with moved beads happens only when the pair is in plane
-    flag[1] = are_in_plane(pos(b.x), pos(b.y), pos(b.z), pos(b.w)) ? 1 : 0;
     flag[0] = ok ? 0 : 1;
     if (ok) {
         int s = D.slot_ev + 4 * i;
@@ -143,6 +157,7 @@ static int grid_for(const Ctx &c, long long n) {
 int launch_energy(Ctx &c, const double *d_x, double *d_out, hipStream_t st) {
     Dev D = c.dev();
     const int nblk = grid_for(c, D.n_items());
+    MDY_TRY(ensure(c.red, (size_t)kRed * nblk));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c.qn.p, 4);
     hipLaunchKernelGGL(k_energy<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.red.p);
     hipLaunchKernelGGL(k_energy_simpson<PosX>, dim3(64), dim3(kBlock), 0, st, D, PosX{d_x}, c.qe_u.p);
@@ -160,20 +175,22 @@ int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hi
     int h_q[4];
     if (hipMemcpyAsync(h_q, c.qn.p, sizeof(h_q), hipMemcpyDeviceToHost, st) != hipSuccess) return MDY_ERR_HIP;
     if (hipStreamSynchronize(st) != hipSuccess) return MDY_ERR_HIP;
-    int nq = h_q[1] < kQueueCap ? h_q[1] : kQueueCap;
     if (h_q[1] > kQueueCap) return MDY_ERR_CAPACITY;
+    const int nq = h_q[1];
     const int sblk = c.num_cu * 4;
+    MDY_TRY(ensure(c.simpson_part, (size_t)6 * kRed * sblk));
     for (int q = 0; q < nq; q++) {
         for (int moved = 0; moved < 2; moved++) {
-            hipLaunchKernelGGL(k_force_simpson_part, dim3(sblk), dim3(kBlock), 0, st, D, d_x, q, moved,
-                               c.simpson_part.p);
+            for (int comp = 0; comp < 6; comp++)
+                hipLaunchKernelGGL(k_force_simpson_part, dim3(sblk), dim3(kBlock), 0, st, D, d_x, q, moved, comp,
+                                   c.simpson_part.p);
             hipLaunchKernelGGL(k_force_simpson_final, dim3(1), dim3(kBlock), 0, st, D, d_x, q, moved,
                                c.simpson_part.p, sblk, c.contrib.p, c.qn.p + 2);
             int fl[2];
             if (hipMemcpyAsync(fl, c.qn.p + 2, sizeof(fl), hipMemcpyDeviceToHost, st) != hipSuccess) return MDY_ERR_HIP;
             if (hipStreamSynchronize(st) != hipSuccess) return MDY_ERR_HIP;
             if (!fl[0]) break;
-            if (moved || !fl[1]) return MDY_ERR_FORCE;  // reference exits here
+            if (moved || !fl[1]) return MDY_ERR_FORCE;  // the reference exits here
         }
     }
     hipLaunchKernelGGL(k_gather, dim3(grid_for(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_force, accumulate);

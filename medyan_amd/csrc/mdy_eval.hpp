@@ -1,7 +1,7 @@
 // mdy_eval.hpp -- evaluation of all interactions over the unified index space,
-// per-bead force gather, block reductions and the quadrature fallbacks.
-// Used both by the one-shot kernels (mdy_eval.hip) and by the persistent
-// conjugate-gradient kernel (mdy_cg.hip).
+// per-bead force gather, exact block/grid reductions and the quadrature
+// fallbacks. Used both by the one-shot kernels (mdy_eval.hip) and by the
+// persistent conjugate-gradient kernel (mdy_cg.hip).
 #pragma once
 #include "mdy_device.hpp"
 #include "mdy_internal.hpp"
@@ -41,126 +41,136 @@ __device__ __forceinline__ void enqueue(int *q, int *qn, int v, int *err) {
     else set_err(err, MDY_ERR_CAPACITY);
 }
 
-// ---- energies ----------------------------------------------------------------
-template <class P>
-__device__ __forceinline__ void energy_item(const Dev &D, int i, const P &pos, double acc[kNFF]) {
-    if (i < D.n_ev) {
-        int4 b = D.ev_bs[i];
-        double U = ev_energy(pos(b.x), pos(b.y), pos(b.z), pos(b.w), D.ev_kr[i]);
-        if (ev_energy_bad(U)) enqueue(D.qe, D.qe_n, i, D.err);
-        else acc[MDY_FF_EV] += U;
-        return;
-    }
-    i -= D.n_ev;
-    if (i < D.n_fs) {
-        int2 b = D.fs_bs[i];
-        acc[MDY_FF_FS] += fs_energy(pos(b.x), pos(b.y), D.fs_k[i], D.fs_l0[i]);
-        return;
-    }
-    i -= D.n_fs;
-    if (i < D.n_fb) {
-        const int *b = D.fb_bs + 3 * i;
-        acc[MDY_FF_FB] += fb_energy(pos(b[0]), pos(b[1]), pos(b[2]), D.fb_k[i], D.fb_th[i]);
-        return;
-    }
-    i -= D.n_fb;
-    if (i < D.n_lm) {
-        int4 b = D.lm_bs[i];
-        double U = ls_energy(pos(b.x), pos(b.y), pos(b.z), pos(b.w), D.lm_k[i], D.lm_l0[i], D.lm_p1[i], D.lm_p2[i]);
-        if (i < D.n_ls) acc[MDY_FF_LS] += U;
-        else acc[MDY_FF_MS] += U;
-        return;
-    }
-    i -= D.n_lm;
-    if (i < D.n_bs) {
-        const int *b = D.bs_bs + 3 * i;
-        acc[MDY_FF_BS] += bs_energy(pos(b[0]), pos(b[1]), pos(b[2]), D.bs_k[i], D.bs_l0[i], D.bs_p[i]);
-        return;
-    }
-    i -= D.n_bs;
-    if (i < D.n_bb) {
-        int4 b = D.bb_bs[i];
-        acc[MDY_FF_BB] += bb_energy(pos(b.x), pos(b.y), pos(b.z), pos(b.w), D.bb_k[i], D.bb_th[i]);
-        return;
-    }
-    i -= D.n_bb;
-    if (i < D.n_bd) {
-        acc[MDY_FF_BD] += bd_energy(D.planes + 4 * D.bd_plane[i], pos(D.bd_bead[i]), D.bd_k[i], D.bd_sl[i]);
-    }
-}
-
 __device__ __forceinline__ void put3(double *__restrict__ c, int slot, d3 v) {
     c[3 * slot] = v.x; c[3 * slot + 1] = v.y; c[3 * slot + 2] = v.z;
 }
 
-// ---- forces: each interaction writes its per-bead contributions to its slots
-template <class P>
-__device__ __forceinline__ void force_item(const Dev &D, int i, const P &pos, double *__restrict__ contrib) {
+// ---- one interaction: energy (into the exact total and, if PER_FF, into the
+// per-force-field double sums) and, if FORCES, its per-bead force slots. The
+// positions are loaded once for both.
+template <bool PER_FF, bool FORCES, class P>
+__device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double acc[kNFF], mdy_xsum &xs,
+                                     double *__restrict__ contrib) {
     if (i < D.n_ev) {
         int4 b = D.ev_bs[i];
+        const double kr = D.ev_kr[i];
         d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
-        d3 c2u = c2;
-        // coplanar perturbation of the reference force path (CylinderExclVolRepulsion.cpp:610-626)
-        if (D.coplanar_mode == MDY_COPLANAR_REFERENCE && are_in_plane(c1, c2, c3, c4))
-            c2u = moved_off_plane(c1, c2, c3, c2, 0.01);
-        d3 f[4];
-        ev_force(c1, c2u, c3, c4, D.ev_kr[i], f);
-        if (!(finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]))) {
-            enqueue(D.qf, D.qf_n, i, D.err);
-            return;
+        EvInv v;
+        ev_invariants(c1, c2, c3, c4, v);
+        const double U = ev_energy_inv(v, kr);
+        if (ev_energy_bad(U)) enqueue(D.qe, D.qe_n, i, D.err);
+        else {
+            mdy_xs_add(&xs, U);
+            if (PER_FF) acc[MDY_FF_EV] += U;
         }
-        int s = D.slot_ev + 4 * i;
-        put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
+        if (FORCES) {
+            d3 f[4];
+            // coplanar perturbation of the reference force path (CylinderExclVolRepulsion.cpp:610-626)
+            if (D.coplanar_mode == MDY_COPLANAR_REFERENCE && are_in_plane(c1, c2, c3, c4)) {
+                d3 c2u = moved_off_plane(c1, c2, c3, c2, 0.01);
+                ev_force(c1, c2u, c3, c4, kr, f);
+            } else {
+                ev_force_inv(v, c1, c2, c3, c4, kr, f);
+            }
+            if (!(finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]))) {
+                enqueue(D.qf, D.qf_n, i, D.err);
+            } else {
+                int s = D.slot_ev + 4 * i;
+                put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
+            }
+        }
         return;
     }
     i -= D.n_ev;
     if (i < D.n_fs) {
         int2 b = D.fs_bs[i];
-        d3 f1, f2;
-        fs_force(pos(b.x), pos(b.y), D.fs_k[i], D.fs_l0[i], f1, f2);
-        int s = D.slot_fs + 2 * i;
-        put3(contrib, s, f1); put3(contrib, s + 1, f2);
+        d3 c1 = pos(b.x), c2 = pos(b.y);
+        const double U = fs_energy(c1, c2, D.fs_k[i], D.fs_l0[i]);
+        mdy_xs_add(&xs, U);
+        if (PER_FF) acc[MDY_FF_FS] += U;
+        if (FORCES) {
+            d3 f1, f2;
+            fs_force(c1, c2, D.fs_k[i], D.fs_l0[i], f1, f2);
+            int s = D.slot_fs + 2 * i;
+            put3(contrib, s, f1); put3(contrib, s + 1, f2);
+        }
         return;
     }
     i -= D.n_fs;
     if (i < D.n_fb) {
         const int *b = D.fb_bs + 3 * i;
-        d3 f1, f2, f3;
-        fb_force(pos(b[0]), pos(b[1]), pos(b[2]), D.fb_k[i], D.fb_th[i], f1, f2, f3);
-        int s = D.slot_fb + 3 * i;
-        put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
+        d3 c1 = pos(b[0]), c2 = pos(b[1]), c3 = pos(b[2]);
+        const double k = D.fb_k[i], th = D.fb_th[i], ct = D.fb_ct[i], st = D.fb_st[i];
+        const double U = fb_energy(c1, c2, c3, k, th, ct, st);
+        mdy_xs_add(&xs, U);
+        if (PER_FF) acc[MDY_FF_FB] += U;
+        if (FORCES) {
+            d3 f1, f2, f3;
+            fb_force(c1, c2, c3, k, th, ct, st, f1, f2, f3);
+            int s = D.slot_fb + 3 * i;
+            put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
+        }
         return;
     }
     i -= D.n_fb;
     if (i < D.n_lm) {
         int4 b = D.lm_bs[i];
-        d3 f1, f2, f3, f4;
-        ls_force(pos(b.x), pos(b.y), pos(b.z), pos(b.w), D.lm_k[i], D.lm_l0[i], D.lm_p1[i], D.lm_p2[i], f1, f2, f3, f4);
-        int s = D.slot_lm + 4 * i;
-        put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
+        d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+        const double k = D.lm_k[i], l0 = D.lm_l0[i], p1 = D.lm_p1[i], p2 = D.lm_p2[i];
+        const double U = ls_energy(c1, c2, c3, c4, k, l0, p1, p2);
+        mdy_xs_add(&xs, U);
+        if (PER_FF) {
+            if (i < D.n_ls) acc[MDY_FF_LS] += U;
+            else acc[MDY_FF_MS] += U;
+        }
+        if (FORCES) {
+            d3 f1, f2, f3, f4;
+            ls_force(c1, c2, c3, c4, k, l0, p1, p2, f1, f2, f3, f4);
+            int s = D.slot_lm + 4 * i;
+            put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
+        }
         return;
     }
     i -= D.n_lm;
     if (i < D.n_bs) {
         const int *b = D.bs_bs + 3 * i;
-        d3 f1, f2, f3;
-        bs_force(pos(b[0]), pos(b[1]), pos(b[2]), D.bs_k[i], D.bs_l0[i], D.bs_p[i], f1, f2, f3);
-        int s = D.slot_bs + 3 * i;
-        put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
+        d3 c1 = pos(b[0]), c2 = pos(b[1]), c3 = pos(b[2]);
+        const double k = D.bs_k[i], l0 = D.bs_l0[i], p = D.bs_p[i];
+        const double U = bs_energy(c1, c2, c3, k, l0, p);
+        mdy_xs_add(&xs, U);
+        if (PER_FF) acc[MDY_FF_BS] += U;
+        if (FORCES) {
+            d3 f1, f2, f3;
+            bs_force(c1, c2, c3, k, l0, p, f1, f2, f3);
+            int s = D.slot_bs + 3 * i;
+            put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
+        }
         return;
     }
     i -= D.n_bs;
     if (i < D.n_bb) {
         int4 b = D.bb_bs[i];
-        d3 f1, f2, f3, f4;
-        bb_force(pos(b.x), pos(b.y), pos(b.z), pos(b.w), D.bb_k[i], D.bb_th[i], f1, f2, f3, f4);
-        int s = D.slot_bb + 4 * i;
-        put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
+        d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+        const double k = D.bb_k[i], th = D.bb_th[i], ct = D.bb_ct[i], st = D.bb_st[i];
+        const double U = bb_energy(c1, c2, c3, c4, k, th, ct, st);
+        mdy_xs_add(&xs, U);
+        if (PER_FF) acc[MDY_FF_BB] += U;
+        if (FORCES) {
+            d3 f1, f2, f3, f4;
+            bb_force(c1, c2, c3, c4, k, th, ct, st, f1, f2, f3, f4);
+            int s = D.slot_bb + 4 * i;
+            put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
+        }
         return;
     }
     i -= D.n_bb;
     if (i < D.n_bd) {
-        put3(contrib, D.slot_bd + i, bd_force(D.planes + 4 * D.bd_plane[i], pos(D.bd_bead[i]), D.bd_k[i], D.bd_sl[i]));
+        const double *pl = D.planes + 4 * D.bd_plane[i];
+        d3 c = pos(D.bd_bead[i]);
+        const double U = bd_energy(pl, c, D.bd_k[i], D.bd_sl[i]);
+        mdy_xs_add(&xs, U);
+        if (PER_FF) acc[MDY_FF_BD] += U;
+        if (FORCES) put3(contrib, D.slot_bd + i, bd_force(pl, c, D.bd_k[i], D.bd_sl[i]));
     }
 }
 
@@ -179,7 +189,7 @@ __device__ __forceinline__ d3 gather_bead(const Dev &D, const double *__restrict
     return d3{fx, fy, fz};
 }
 
-// ---- deterministic block reductions (fixed butterfly + fixed wave order) ----
+// ---- block reductions (fixed butterfly + fixed wave order: deterministic) ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -195,8 +205,19 @@ __device__ __forceinline__ double wave_min(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     return v;
 }
-// Reduces N values per thread to N values (valid in thread 0, and broadcast to
-// all threads through lds). op: 0 sum, 1 max, 2 min per entry given by mask bits.
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+    return v;
+}
+// N doubles per thread -> N block results (all threads). Entries whose bit is
+// set in max_mask / min_mask are max / min reduced, the rest summed.
+// lds: >= 17 * N doubles.
 template <int N>
 __device__ __forceinline__ void block_reduce(double v[N], double *lds, unsigned max_mask = 0, unsigned min_mask = 0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -220,66 +241,115 @@ __device__ __forceinline__ void block_reduce(double v[N], double *lds, unsigned 
     for (int k = 0; k < N; k++) v[k] = lds[16 * N + k];
     __syncthreads();
 }
+// Exact block sum of per-thread accumulators (integer digit sums are
+// associative, so any tree gives the same result). lds: >= 17 * 8 int64.
+__device__ __forceinline__ void block_reduce_xsum(mdy_xsum &s, long long *lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    mdy_xs_norm(&s);
+#pragma unroll
+    for (int k = 0; k < MDY_XS_N; k++) {
+        long long r = wave_sum_i64(s.d[k]);
+        if (lane == 0) lds[wave * 8 + k] = r;
+    }
+    int f = wave_or(s.flags);
+    if (lane == 0) lds[wave * 8 + 7] = f;
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        int k = threadIdx.x;
+        long long r = lds[k];
+        for (int w = 1; w < nw; w++) r = (k == 7) ? (r | lds[w * 8 + k]) : r + lds[w * 8 + k];
+        lds[128 + k] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MDY_XS_N; k++) s.d[k] = lds[128 + k];
+    s.flags = (int)lds[128 + 7];
+    mdy_xs_norm(&s);
+    __syncthreads();
+}
+// per-block partial records: [blk * kRed + off .. off+7] hold a normalized accumulator
+__device__ __forceinline__ void store_xsum(double *red, int blk, int off, const mdy_xsum &s) {
+    long long *r = reinterpret_cast<long long *>(red) + (size_t)blk * kRed + off;
+#pragma unroll
+    for (int k = 0; k < MDY_XS_N; k++) r[k] = s.d[k];
+    r[7] = s.flags;
+}
+__device__ __forceinline__ void grid_reduce_xsum(const double *red, int nblk, int off, mdy_xsum &s, long long *lds) {
+    mdy_xs_zero(&s);
+    const long long *r = reinterpret_cast<const long long *>(red);
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const long long *q = r + (size_t)b * kRed + off;
+#pragma unroll
+        for (int k = 0; k < MDY_XS_N; k++) s.d[k] += __builtin_nontemporal_load(q + k);
+        s.flags |= (int)__builtin_nontemporal_load(q + 7);
+    }
+    block_reduce_xsum(s, lds);
+}
 
-// Reduce per-block partials red[blk * kRed + k] for k < N over nblk blocks in
-// a fixed order; every calling block obtains bit-identical results.
+// Reduce per-block partials red[blk * kRed + off + k] for k < N over nblk
+// blocks in a fixed order; every calling block obtains bit-identical results.
 template <int N>
 __device__ __forceinline__ void grid_partials_reduce(const double *red, int nblk, double v[N], double *lds,
-                                                     unsigned max_mask = 0, unsigned min_mask = 0) {
+                                                     unsigned max_mask = 0, unsigned min_mask = 0, int off = 0) {
 #pragma unroll
     for (int k = 0; k < N; k++) v[k] = ((max_mask >> k) & 1) ? -INFINITY : ((min_mask >> k) & 1) ? INFINITY : 0.0;
     for (int blk = threadIdx.x; blk < nblk; blk += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < N; k++) {
-            double o = __builtin_nontemporal_load(red + (size_t)blk * kRed + k);
+            double o = __builtin_nontemporal_load(red + (size_t)blk * kRed + off + k);
             v[k] = ((max_mask >> k) & 1) ? fmax(v[k], o) : ((min_mask >> k) & 1) ? fmin(v[k], o) : v[k] + o;
         }
     }
     block_reduce<N>(v, lds, max_mask, min_mask);
 }
 
-// Energy epilogue, shared with the persistent kernel: per-FF totals from the
-// block partials and the (pair-ordered) quadrature fallbacks; out[0..7] per
-// FF, out[8] = computeEnergy total (+inf sentinel), out[9] = sum without sentinel.
-__device__ __forceinline__ void energy_epilogue(const Dev &D, const double *red, int nblk, int nq, const double *qe_u, double *lds,
-                                double e[kNFF + 2]) {
+// Energy epilogue, shared by the one-shot and persistent kernels. The block
+// partials hold per-FF double sums at [0, kNFF) and the exact total at
+// [kXsE, kXsE + 8). Quadrature-fallback energies (queue qe, values qe_u) are
+// added to both. e[0..7] per FF (reported values), e[8] = computeEnergy total
+// (+inf when a force field returned its -1 sentinel), e[9] = the sum without
+// the sentinel.
+constexpr int kXsE = 8;    // exact energy total
+constexpr int kXsA = 16;   // second exact accumulator (slope s.F)
+__device__ __forceinline__ void energy_epilogue(const Dev &D, const double *red, int nblk, int nq, const int *qe,
+                                                const double *qe_u, double *lds, double e[kNFF + 2], bool per_ff) {
     double v[kNFF];
-    grid_partials_reduce<kNFF>(red, nblk, v, lds);
+    if (per_ff) grid_partials_reduce<kNFF>(red, nblk, v, lds);
+    else
+        for (int k = 0; k < kNFF; k++) v[k] = 0.0;
+    mdy_xsum X;
+    grid_reduce_xsum(red, nblk, kXsE, X, reinterpret_cast<long long *>(lds));
     nq = min(nq, kQueueCap);
+    bool bad = false;
     if (nq > 0) {
-        // add fallback energies in increasing pair order (selection; queues are short)
+        // per-FF value: fallback energies added in increasing pair order
         double add = 0.0;
-        bool bad = false;
         int last = -1;
         for (int r = 0; r < nq; r++) {
             int best = -1, bi = 0x7fffffff;
             for (int q = 0; q < nq; q++) {
-                int pi = D.qe[q];
+                int pi = __hip_atomic_load(qe + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (pi > last && pi < bi) { bi = pi; best = q; }
             }
             if (best < 0) break;
             last = bi;
             double U = qe_u[best];
             if (ev_energy_bad(U)) bad = true;
+            else mdy_xs_add(&X, U);
             add += U;
         }
         v[MDY_FF_EV] = bad ? -1.0 : v[MDY_FF_EV] + add;
     }
-    double tot = 0.0, ind = 0.0;
-    bool inf = false;
-    for (int k = 0; k < kNFF; k++) {
-        e[k] = v[k];
-        if (v[k] <= -1) inf = true;
-        if (!inf) tot += v[k];
-        ind += v[k];
-    }
-    e[kNFF] = inf ? INFINITY : tot;
-    e[kNFF + 1] = ind;
+    for (int k = 0; k < kNFF; k++) e[k] = v[k];
+    const double tot = mdy_xs_value(&X);
+    e[kNFF] = bad ? INFINITY : tot;
+    e[kNFF + 1] = bad ? tot - 1.0 : tot;
 }
 
 // ---- Simpson quadrature fallbacks (CylinderExclVolRepulsion.cpp:884-1254) ----
 // The reference advances its abscissae before first use, so interior node i
-// sits at (i+1)/N; node values are formed here as (i+1)*(1/N).
+// sits at (i+1)/N; node values are formed here as (i+1)*(1/N). Sums are exact
+// (mdy_xsum), like the CPU path in exact mode.
 struct SimpsonGeo { double a, b, c, d, e, F; d3 vA, vB, vC; };
 __device__ __forceinline__ SimpsonGeo simpson_geo(d3 c1, d3 c2, d3 c3, d3 c4) {
     SimpsonGeo g;
@@ -288,49 +358,52 @@ __device__ __forceinline__ SimpsonGeo simpson_geo(d3 c1, d3 c2, d3 c3, d3 c4) {
     g.vA = sub(c1, c2); g.vB = sub(c3, c4); g.vC = sub(c3, c1);
     return g;
 }
-// energy integrand contribution of grid node (i, j) with the reference's
-// bookkeeping: returns the weighted term that node adds to Termset2/3/4 (or 0)
-__device__ __forceinline__ double simpson_e_node(const SimpsonGeo &g, int i, int j, int N) {
+__device__ __forceinline__ double eint(const SimpsonGeo &g, double s, double t) {
+    double r2 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, s, t);
+    return (1.0 / (r2 * r2));
+}
+// energy: terms that node (i, j) adds to Termset2/3/4 (CylinderExclVolRepulsion.cpp:923-958)
+__device__ __forceinline__ void simpson_e_node(const SimpsonGeo &g, int i, int j, int N, mdy_xsum &X) {
     const double ds = 1.0 / ((double)N);
     double s = (double)(i + 1) * ds, t = (double)(j + 1) * ds;
     double fs = (i % 2 == 0) ? 2.0 : 4.0, ft = (j % 2 == 0) ? 2.0 : 4.0;
-    double r = 0.0;
-    if (j == 0 && i > 0 && i < N) {  // Termset2 (once per row)
-        double r0 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, s, 0.0), r1 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, s, 1.0);
-        r += fs * ((1.0 / (r0 * r0)) + (1.0 / (r1 * r1)));
-    }
+    if (j == 0 && i > 0 && i < N) mdy_xs_add(&X, fs * (eint(g, s, 0.0) + eint(g, s, 1.0)));
     if (j > 0 && j < N) {
-        if (i == 0) {  // Termset3
-            double r0 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, 0.0, t), r1 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, 1.0, t);
-            r += ft * ((1.0 / (r0 * r0)) + (1.0 / (r1 * r1)));
-        } else if (i < N) {  // Termset4
-            double r2 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, s, t);
-            r += fs * ft * (1.0 / (r2 * r2));
-        }
+        if (i == 0) mdy_xs_add(&X, ft * (eint(g, 0.0, t) + eint(g, 1.0, t)));
+        else if (i < N) mdy_xs_add(&X, fs * ft * eint(g, s, t));
     }
-    return r;
 }
 __device__ __forceinline__ double simpson_e_corners(const SimpsonGeo &g) {
-    double r00 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, 0.0, 0.0), r01 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, 0.0, 1.0);
-    double r10 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, 1.0, 0.0), r11 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, 1.0, 1.0);
-    return (1.0 / (r00 * r00)) + (1.0 / (r01 * r01)) + (1.0 / (r10 * r10)) + (1.0 / (r11 * r11));
+    return eint(g, 0.0, 0.0) + eint(g, 0.0, 1.0) + eint(g, 1.0, 0.0) + eint(g, 1.0, 1.0);
 }
-__device__ __forceinline__ void fint6(const SimpsonGeo &g, double s, double t, double w, double o[6]) {
+// force: component q of the six integrands {1, s, s^2, st, t, t^2}/r^6
+__device__ __forceinline__ double fint_q(const SimpsonGeo &g, double s, double t, int q) {
     double r2 = ev_r2(g.a, g.b, g.c, g.d, g.e, g.F, s, t);
     double x = (1.0 / (r2 * r2 * r2));
-    o[0] += w * x; o[1] += w * (s * x); o[2] += w * (s * s * x);
-    o[3] += w * (s * t * x); o[4] += w * (t * x); o[5] += w * (t * t * x);
+    return q == 0 ? x : q == 1 ? s * x : q == 2 ? s * s * x : q == 3 ? s * t * x : q == 4 ? t * x : t * t * x;
 }
-// force integrand contributions of node (i, j) (Termset2/3/4 bookkeeping)
-__device__ __forceinline__ void simpson_f_node(const SimpsonGeo &g, int i, int j, int N, double o[6]) {
+__device__ __forceinline__ void simpson_f_node(const SimpsonGeo &g, int i, int j, int N, int q, mdy_xsum &X) {
     const double ds = 1.0 / ((double)N);
     double s = (double)(i + 1) * ds, t = (double)(j + 1) * ds;
     double fs = (i % 2 == 0) ? 2.0 : 4.0, ft = (j % 2 == 0) ? 2.0 : 4.0;
-    if (j == 0 && i > 0 && i < N) { fint6(g, s, 0.0, fs, o); fint6(g, s, 1.0, fs, o); }
-    if (j > 0 && j < N) {
-        if (i == 0) { fint6(g, 0.0, t, ft, o); fint6(g, 1.0, t, ft, o); }
-        else if (i < N) fint6(g, s, t, fs * ft, o);
+    if (j == 0 && i > 0 && i < N) {
+        mdy_xs_add(&X, fs * fint_q(g, s, 0.0, q));
+        mdy_xs_add(&X, fs * fint_q(g, s, 1.0, q));
     }
+    if (j > 0 && j < N) {
+        if (i == 0) {
+            mdy_xs_add(&X, ft * fint_q(g, 0.0, t, q));
+            mdy_xs_add(&X, ft * fint_q(g, 1.0, t, q));
+        } else if (i < N) {
+            mdy_xs_add(&X, fs * ft * fint_q(g, s, t, q));
+        }
+    }
+}
+__device__ __forceinline__ void simpson_f_corners(const SimpsonGeo &g, int q, mdy_xsum &X) {
+    mdy_xs_add(&X, fint_q(g, 0.0, 0.0, q));
+    mdy_xs_add(&X, fint_q(g, 0.0, 1.0, q));
+    mdy_xs_add(&X, fint_q(g, 1.0, 0.0, q));
+    mdy_xs_add(&X, fint_q(g, 1.0, 1.0, q));
 }
 __device__ __forceinline__ void simpson_f_assemble(const SimpsonGeo &g, const double T[6], double kr, int N, d3 f[4]) {
     const double ds = 1.0 / ((double)N);

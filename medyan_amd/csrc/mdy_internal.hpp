@@ -13,7 +13,7 @@ namespace mdy {
 constexpr int kNFF = MDY_NFF;
 constexpr int kBlock = 256;          // threads per block of every kernel
 constexpr int kQueueCap = 4096;      // excluded-volume quadrature fallback queue
-constexpr int kRed = 16;             // doubles per block in a reduction slot
+constexpr int kRed = 32;             // 8-byte words per block in a reduction record
 
 // Interaction tables on the device. The evaluation index space is
 // [EV | FS | FB | LM | BS | BB | BD]: excluded volume first so its costly
@@ -26,10 +26,10 @@ struct Dev {
     int n_ev, n_fs, n_fb, n_lm, n_ls, n_bs, n_bb, n_bd;
     const int4 *ev_bs; const double *ev_kr;
     const int2 *fs_bs; const double *fs_k, *fs_l0;
-    const int *fb_bs; const double *fb_k, *fb_th;
+    const int *fb_bs; const double *fb_k, *fb_th, *fb_ct, *fb_st;   // ct/st: cos/sin(theta0) from the host
     const int4 *lm_bs; const double *lm_k, *lm_l0, *lm_p1, *lm_p2;
     const int *bs_bs; const double *bs_k, *bs_l0, *bs_p;
-    const int4 *bb_bs; const double *bb_k, *bb_th;
+    const int4 *bb_bs; const double *bb_k, *bb_th, *bb_ct, *bb_st;
     const int *bd_bead; const int *bd_plane; const double *bd_k, *bd_sl; const double *planes;
     int slot_fs, slot_fb, slot_lm, slot_bs, slot_bb, slot_ev, slot_bd, n_slots;
     const int *st_off, *st_idx;   // static CSR: bead -> bonded/cross-link slots
@@ -62,10 +62,10 @@ struct Ctx {
     std::vector<int> h_static_counts;
     // device tables (static part)
     DBuf<int2> fs_bs; DBuf<double> fs_k, fs_l0;
-    DBuf<int> fb_bs; DBuf<double> fb_k, fb_th;
+    DBuf<int> fb_bs; DBuf<double> fb_k, fb_th, fb_ct, fb_st;
     DBuf<int4> lm_bs; DBuf<double> lm_k, lm_l0, lm_p1, lm_p2;
     DBuf<int> bs_bs; DBuf<double> bs_k, bs_l0, bs_p;
-    DBuf<int4> bb_bs; DBuf<double> bb_k, bb_th;
+    DBuf<int4> bb_bs; DBuf<double> bb_k, bb_th, bb_ct, bb_st;
     DBuf<int> st_off, st_idx;
     int n_fs = 0, n_fb = 0, n_ls = 0, n_ms = 0, n_bs = 0, n_bb = 0;
     // cylinders

@@ -1,0 +1,234 @@
+/*
+ * mdy_xmath.h -- numerics shared by the HIP kernels and the CPU path so that
+ * both produce the same bits:
+ *
+ *   mdy_atan, mdy_exp : arctangent and exponential written from their
+ *     series (no vendor constants): argument reduction to a small interval,
+ *     Taylor polynomial, table/split constants computed by
+ *     gen_xmath_tables.py in 100-digit decimal arithmetic. Built from IEEE
+ *     +,-,*,/ only (no fused multiply-add), so host and device agree exactly.
+ *     Accuracy ~1 ulp (pinned against the C library in the tests).
+ *
+ *   mdy_xsum : order-independent exact summation of doubles. Every term is
+ *     placed on a fixed-point grid of unit 2^-144 (terms are truncated toward
+ *     zero below it) and accumulated in seven signed 64-bit digits of radix
+ *     2^48 with deferred carries; the final value is rounded to nearest even
+ *     once. Any order of additions or merges gives the same bits, and for terms
+ *     on the grid the result equals the correctly rounded sum (math.fsum).
+ *     Magnitudes of 2^191 and above are treated as overflow (+-inf).
+ *
+ * Plain C99 and HIP; include after <math.h>/<stdint.h>.
+ */
+#ifndef MDY_XMATH_H
+#define MDY_XMATH_H
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#define MDY_HD __host__ __device__ static inline __attribute__((always_inline))
+#else
+#define MDY_HD static inline
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* atan                                                                      */
+/* ------------------------------------------------------------------------ */
+/* atan(k/8) = hi + lo, k = 0..8 */
+#define MDY_ATAN_HI(k)                                                                                      \
+    ((k) == 0 ? 0.0 : (k) == 1 ? 0x1.fd5ba9aac2f6ep-4 : (k) == 2 ? 0x1.f5b75f92c80ddp-3                   \
+     : (k) == 3 ? 0x1.6f61941e4def1p-2 : (k) == 4 ? 0x1.dac670561bb4fp-2 : (k) == 5 ? 0x1.1e00babdefeb4p-1 \
+     : (k) == 6 ? 0x1.4978fa3269ee1p-1 : (k) == 7 ? 0x1.700a7c5784634p-1 : 0x1.921fb54442d18p-1)
+#define MDY_ATAN_LO(k)                                                                                        \
+    ((k) == 0 ? 0.0 : (k) == 1 ? -0x1.cd37686760c17p-59 : (k) == 2 ? 0x1.8ab6e3cf7afbdp-57                  \
+     : (k) == 3 ? -0x1.c63aae6f6e918p-56 : (k) == 4 ? 0x1.a2b7f222f65e2p-56 : (k) == 5 ? -0x1.928df287a668fp-58 \
+     : (k) == 6 ? 0x1.2419a87f2a458p-56 : (k) == 7 ? -0x1.8c34d25aadef6p-56 : 0x1.1a62633145c07p-55)
+#define MDY_PIO2_HI 0x1.921fb54442d18p+0
+#define MDY_PIO2_LO 0x1.1a62633145c07p-54
+
+MDY_HD double mdy_atan(double x) {
+    if (x != x) return x;
+    double ax = fabs(x);
+    int inv = ax > 1.0;
+    double y = inv ? 1.0 / ax : ax;              /* y in [0, 1]; 1/inf = 0 */
+    int k = (int)(y * 8.0 + 0.5);                /* nearest table point k/8 */
+    double c = (double)k * 0.125;
+    double t = (y - c) / (1.0 + y * c);          /* |t| <= 1/16: atan(y) = atan(c) + atan(t) */
+    double t2 = t * t;
+    /* atan(t) = t - t^3/3 + t^5/5 - ... ; |t|^17/17 < 2^-70 |t| */
+    double p = -1.0 / 15.0;
+    p = p * t2 + 1.0 / 13.0;
+    p = p * t2 - 1.0 / 11.0;
+    p = p * t2 + 1.0 / 9.0;
+    p = p * t2 - 1.0 / 7.0;
+    p = p * t2 + 1.0 / 5.0;
+    p = p * t2 - 1.0 / 3.0;
+    p = t + t * (t2 * p);
+    double r = MDY_ATAN_HI(k) + (MDY_ATAN_LO(k) + p);
+    if (inv) r = (MDY_PIO2_HI - r) + MDY_PIO2_LO;
+    return x < 0 ? -r : r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* exp                                                                       */
+/* ------------------------------------------------------------------------ */
+#define MDY_LN2_HI 0x1.62e42fee00000p-1
+#define MDY_LN2_LO 0x1.a39ef35793c76p-33
+#define MDY_INVLN2 0x1.71547652b82fep+0
+
+MDY_HD double mdy_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return INFINITY;
+    if (x < -745.2) return 0.0;
+    double kd = rint(x * MDY_INVLN2);
+    int k = (int)kd;
+    double r = (x - kd * MDY_LN2_HI) - kd * MDY_LN2_LO;   /* |r| <= ~0.35 */
+    /* exp(r) = sum r^n / n!, n <= 14 (r^15/15! < 2^-80) */
+    double p = 1.0 / 87178291200.0;
+    p = p * r + 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    /* 2^k * p in two exact steps (k may exceed the normal exponent range) */
+    int k1 = k / 2, k2 = k - k1;
+    return ldexp(ldexp(p, k1), k2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact summation                                                           */
+/* ------------------------------------------------------------------------ */
+#define MDY_XS_N 7
+#define MDY_XS_BIAS 144
+#define MDY_XS_MASK ((int64_t)0xFFFFFFFFFFFFll)
+enum { MDY_XS_NAN = 1, MDY_XS_PINF = 2, MDY_XS_NINF = 4 };
+
+/* Digits carry 16 spare bits: at most 2^15 additions may separate two
+ * normalizations; mdy_xs_add normalizes itself every MDY_XS_RENORM adds. */
+#define MDY_XS_RENORM 16384
+typedef struct mdy_xsum {
+    int64_t d[MDY_XS_N];
+    int32_t flags;
+    int32_t cnt;      /* additions since the last normalization */
+} mdy_xsum;
+
+MDY_HD uint64_t mdy_dbits(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint64_t)__double_as_longlong(v);
+#else
+    uint64_t u;
+    memcpy(&u, &v, sizeof(u));
+    return u;
+#endif
+}
+
+MDY_HD void mdy_xs_zero(mdy_xsum *s) {
+    for (int k = 0; k < MDY_XS_N; k++) s->d[k] = 0;
+    s->flags = 0;
+    s->cnt = 0;
+}
+
+/* carries into [0, 2^48) for all but the top digit */
+MDY_HD void mdy_xs_norm(mdy_xsum *s) {
+    for (int k = 0; k < MDY_XS_N - 1; k++) {
+        int64_t c = s->d[k] >> 48;                /* floor division (arithmetic shift) */
+        s->d[k] &= MDY_XS_MASK;
+        s->d[k + 1] += c;
+    }
+    s->cnt = 0;
+}
+
+MDY_HD void mdy_xs_add(mdy_xsum *s, double v) {
+    if (++s->cnt >= MDY_XS_RENORM) mdy_xs_norm(s);
+    uint64_t b = mdy_dbits(v);
+    int neg = (int)(b >> 63);
+    int ex = (int)((b >> 52) & 0x7ff);
+    uint64_t m = b & 0xFFFFFFFFFFFFFull;
+    if (ex == 0x7ff) {
+        s->flags |= m ? MDY_XS_NAN : (neg ? MDY_XS_NINF : MDY_XS_PINF);
+        return;
+    }
+    int e;
+    if (ex == 0) e = -1074;
+    else { m |= 0x10000000000000ull; e = ex - 1075; }
+    int p = e + MDY_XS_BIAS;                      /* grid position of m's unit bit */
+    if (p < 0) {
+        m = (-p >= 64) ? 0 : (m >> (-p));
+        p = 0;
+    }
+    if (m == 0) return;
+    if (p + 53 > 48 * (MDY_XS_N - 1) + 47) {      /* beyond the accumulator: overflow */
+        s->flags |= neg ? MDY_XS_NINF : MDY_XS_PINF;
+        return;
+    }
+    int j = p / 48, sh = p - 48 * j;
+    int64_t lo = (int64_t)((m << sh) & (uint64_t)MDY_XS_MASK);
+    int64_t mid = (int64_t)((sh == 0 ? (m >> 48) : (m >> (48 - sh))) & (uint64_t)MDY_XS_MASK);
+    int64_t hi = (96 - sh < 64) ? (int64_t)(m >> (96 - sh)) : 0;
+    if (neg) { lo = -lo; mid = -mid; hi = -hi; }
+    for (int k = 0; k < MDY_XS_N; k++) {          /* unrolled selects: no indexed registers */
+        int64_t add = (k == j) ? lo : (k == j + 1) ? mid : (k == j + 2) ? hi : 0;
+        s->d[k] += add;
+    }
+}
+
+MDY_HD void mdy_xs_merge(mdy_xsum *s, const mdy_xsum *o) {
+    for (int k = 0; k < MDY_XS_N; k++) s->d[k] += o->d[k];
+    s->flags |= o->flags;
+}
+
+MDY_HD double mdy_xs_value(const mdy_xsum *in) {
+    if ((in->flags & MDY_XS_NAN) || ((in->flags & MDY_XS_PINF) && (in->flags & MDY_XS_NINF))) return NAN;
+    if (in->flags & MDY_XS_PINF) return INFINITY;
+    if (in->flags & MDY_XS_NINF) return -INFINITY;
+    int64_t d[MDY_XS_N];
+    for (int k = 0; k < MDY_XS_N; k++) d[k] = in->d[k];
+    for (int k = 0; k < MDY_XS_N - 1; k++) { int64_t c = d[k] >> 48; d[k] &= MDY_XS_MASK; d[k + 1] += c; }
+    int neg = d[MDY_XS_N - 1] < 0;
+    if (neg) {
+        for (int k = 0; k < MDY_XS_N; k++) d[k] = -d[k];
+        for (int k = 0; k < MDY_XS_N - 1; k++) { int64_t c = d[k] >> 48; d[k] &= MDY_XS_MASK; d[k + 1] += c; }
+    }
+    int h = -1;
+    for (int k = MDY_XS_N - 1; k >= 0; k--)
+        if (h < 0 && d[k] != 0) h = k;
+    if (h < 0) return 0.0;
+    uint64_t top = (uint64_t)d[h];
+    int nb = 0;
+    while (nb < 64 && (top >> nb) != 0) nb++;     /* bit length of the top digit, 1..63 */
+    uint64_t d1 = h >= 1 ? (uint64_t)d[h - 1] : 0, d2 = h >= 2 ? (uint64_t)d[h - 2] : 0;
+    int need = 64 - nb;                           /* bits taken from below the top digit */
+    uint64_t W = (nb == 64) ? top : (top << need);
+    int sticky = 0;
+    if (need <= 48) {
+        W |= (need == 0) ? 0 : (d1 >> (48 - need));
+        sticky |= (d1 & ((1ull << (48 - need)) - 1)) != 0;
+        sticky |= d2 != 0;
+    } else {
+        W |= (d1 << (need - 48)) | (d2 >> (96 - need));
+        sticky |= (d2 & ((1ull << (96 - need)) - 1)) != 0;
+    }
+    for (int k = 0; k < h - 2; k++) sticky |= d[k] != 0;
+    uint64_t mant = W >> 11;
+    int rbit = (int)((W >> 10) & 1);
+    sticky |= (W & 0x3ff) != 0;
+    int expo = 48 * h + nb - 53 - MDY_XS_BIAS;   /* exponent of mant's unit bit */
+    if (rbit && (sticky || (mant & 1))) {
+        mant++;
+        if (mant == (1ull << 53)) { mant >>= 1; expo++; }
+    }
+    double r = ldexp((double)mant, expo);
+    return neg ? -r : r;
+}
+
+#endif /* MDY_XMATH_H */

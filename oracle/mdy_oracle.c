@@ -20,6 +20,35 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../medyan_amd/csrc/mdy_xmath.h"
+
+/* Numerics mode. Reference mode (default): the C library's atan/exp and
+ * sequential left-to-right sums, as the reference computes them. Exact mode:
+ * the shared mdy_atan/mdy_exp and order-independent exact sums (mdy_xsum),
+ * the numerics the HIP kernels use, so results can be compared bit for bit.
+ * The two modes differ by rounding only (checked in tests/test_oracle_numerics.py). */
+static int g_exact = 0;
+void orc_set_exact(int on) { g_exact = on; }
+int orc_get_exact(void) { return g_exact; }
+static inline double o_atan(double x) { return g_exact ? mdy_atan(x) : atan(x); }
+static inline double o_exp(double x) { return g_exact ? mdy_exp(x) : exp(x); }
+
+/* sum helper: sequential in reference mode, exact in exact mode. In exact
+ * mode every interaction energy is also added to g_total, so the manager's
+ * total is the exact sum of all interaction energies rounded once. */
+typedef struct osum { double seq; mdy_xsum xs; } osum;
+static mdy_xsum *g_total = NULL;
+static inline void os_zero(osum *o) { o->seq = 0.0; mdy_xs_zero(&o->xs); }
+static inline void os_add(osum *o, double v) {
+    if (g_exact) {
+        mdy_xs_add(&o->xs, v);
+        if (g_total) mdy_xs_add(g_total, v);
+    } else {
+        o->seq += v;
+    }
+}
+static inline double os_val(const osum *o) { return g_exact ? mdy_xs_value(&o->xs) : o->seq; }
+
 /* ------------------------------------------------------------------------- */
 /* scalar helpers: src/MathFunctions.h, src/utility.h                        */
 /* ------------------------------------------------------------------------- */
@@ -97,14 +126,15 @@ static inline int finite3(const double *v) { return isfinite(v[0]) && isfinite(v
 /* a3: FilamentStretchingHarmonic  (Filament/FilamentStretchingHarmonic.cpp:185-207, 209-275) */
 /* ------------------------------------------------------------------------- */
 double orc_fs_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0) {
-    double U = 0.0;
+    osum U;
+    os_zero(&U);
     for (int32_t i = 0; i < n; i++) {
         const double *c1 = x + bs[2 * i], *c2 = x + bs[2 * i + 1];
         double dist = tpdist(c1, c2) - l0[i];
         double Ui = 0.5 * k[i] * dist * dist;
-        U += Ui;
+        os_add(&U, Ui);
     }
-    return U;
+    return os_val(&U);
 }
 void orc_fs_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0) {
     for (int32_t i = 0; i < n; i++) {
@@ -156,15 +186,16 @@ static void cosine_bend_coef(double L1sq, double L2sq, double l1l2, double kb, d
     *keff = kb * sinpmq / sinp;
 }
 double orc_fb_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *th0) {
-    double U = 0.0;
+    osum U;
+    os_zero(&U);
     for (int32_t i = 0; i < n; i++) {
         const double *c1 = x + bs[3 * i], *c2 = x + bs[3 * i + 1], *c3 = x + bs[3 * i + 2];
         double L1 = sqrt(sprod(c1, c2, c1, c2));
         double L2 = sqrt(sprod(c2, c3, c2, c3));
         double l1l2 = sprod(c1, c2, c2, c3);
-        U += cosine_bend_energy(l1l2, L1, L2, k[i], th0[i]);
+        os_add(&U, cosine_bend_energy(l1l2, L1, L2, k[i], th0[i]));
     }
-    return U;
+    return os_val(&U);
 }
 void orc_fb_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *th0) {
     for (int32_t i = 0; i < n; i++) {
@@ -184,15 +215,16 @@ void orc_fb_forces(const double *x, double *f, int32_t n, const int32_t *bs, con
     }
 }
 double orc_bb_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *th0) {
-    double U = 0.0;
+    osum U;
+    os_zero(&U);
     for (int32_t i = 0; i < n; i++) {
         const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
         double L1 = sqrt(sprod(c1, c2, c1, c2));
         double L2 = sqrt(sprod(c3, c4, c3, c4));
         double l1l2 = sprod(c1, c2, c3, c4);
-        U += cosine_bend_energy(l1l2, L1, L2, k[i], th0[i]);
+        os_add(&U, cosine_bend_energy(l1l2, L1, L2, k[i], th0[i]));
     }
-    return U;
+    return os_val(&U);
 }
 void orc_bb_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *th0,
                    double *stretch3) {
@@ -222,16 +254,17 @@ void orc_bb_forces(const double *x, double *f, int32_t n, const int32_t *bs, con
 /* ------------------------------------------------------------------------- */
 double orc_ls_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0,
                      const double *p1, const double *p2) {
-    double U = 0.0;
+    osum U;
+    os_zero(&U);
     for (int32_t i = 0; i < n; i++) {
         double v1[3], v2[3];
         midpt(v1, x + bs[4 * i], x + bs[4 * i + 1], p1[i]);
         midpt(v2, x + bs[4 * i + 2], x + bs[4 * i + 3], p2[i]);
         double dist = tpdist(v1, v2) - l0[i];
         double Ui = 0.5 * k[i] * dist * dist;
-        U += Ui;
+        os_add(&U, Ui);
     }
-    return U;
+    return os_val(&U);
 }
 void orc_ls_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0,
                    const double *p1, const double *p2, double *stretch) {
@@ -259,15 +292,16 @@ void orc_ls_forces(const double *x, double *f, int32_t n, const int32_t *bs, con
 /* a7: BranchingStretchingHarmonic (BranchingStretchingHarmonic.cpp:195-221, 224-318) */
 /* ------------------------------------------------------------------------- */
 double orc_bs_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0, const double *p) {
-    double U = 0.0;
+    osum U;
+    os_zero(&U);
     for (int32_t i = 0; i < n; i++) {
         double v1[3];
         midpt(v1, x + bs[3 * i], x + bs[3 * i + 1], p[i]);
         double dist = tpdist(v1, x + bs[3 * i + 2]) - l0[i];
         double Ui = 0.5 * k[i] * dist * dist;
-        U += Ui;
+        os_add(&U, Ui);
     }
-    return U;
+    return os_val(&U);
 }
 void orc_bs_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0,
                    const double *p, double *stretch3) {
@@ -326,10 +360,10 @@ static void ev_invariants(const double *c1, const double *c2, const double *c3, 
     double JJ = dot3(cp, vC);
     v->JJ = -JJ * JJ;
     double AA = v->AA, BB = v->BB, EE = v->EE, FF = v->FF;
-    v->ATG1 = atan((a + e) / AA) - atan(e / AA);
-    v->ATG2 = atan((a + e - d) / EE) - atan((e - d) / EE);
-    v->ATG3 = atan((F) / BB) - atan((F - b) / BB);
-    v->ATG4 = atan((d + F) / FF) - atan((d + F - b) / FF);
+    v->ATG1 = o_atan((a + e) / AA) - o_atan(e / AA);
+    v->ATG2 = o_atan((a + e - d) / EE) - o_atan((e - d) / EE);
+    v->ATG3 = o_atan((F) / BB) - o_atan((F - b) / BB);
+    v->ATG4 = o_atan((d + F) / FF) - o_atan((d + F - b) / FF);
 }
 
 double orc_ev_pair_energy_analytic(const double *c1, const double *c2, const double *c3, const double *c4, double kr) {
@@ -358,21 +392,33 @@ double orc_ev_pair_energy_simpson(const double *c1, const double *c2, const doub
     double T1 = eint(a, b, c, d, e, F, 0.0, 0.0) + eint(a, b, c, d, e, F, 0.0, 1.0) +
                 eint(a, b, c, d, e, F, 1.0, 0.0) + eint(a, b, c, d, e, F, 1.0, 1.0);
     double T2 = 0.0, T3 = 0.0, T4 = 0.0, s = 0.0, t = 0.0;
+    mdy_xsum X;   /* exact mode: one exact accumulator over all node terms */
+    mdy_xs_zero(&X);
+    mdy_xs_add(&X, T1);
     for (int i = 0; i <= N1; i++) {
         t = 0.0;
-        s += ds;
+        s = g_exact ? (double)(i + 1) * ds : s + ds;
         double fs = (i % 2 == 0) ? 2.0 : 4.0;
-        if (i > 0 && i < N1) T2 += fs * (eint(a, b, c, d, e, F, s, 0.0) + eint(a, b, c, d, e, F, s, 1.0));
+        if (i > 0 && i < N1) {
+            double v = fs * (eint(a, b, c, d, e, F, s, 0.0) + eint(a, b, c, d, e, F, s, 1.0));
+            if (g_exact) mdy_xs_add(&X, v); else T2 += v;
+        }
         for (int j = 0; j <= N2; j++) {
-            t += dt;
+            t = g_exact ? (double)(j + 1) * dt : t + dt;
             double ft = (j % 2 == 0) ? 2.0 : 4.0;
             if (j > 0 && j < N2) {
-                if (i == 0) T3 += ft * (eint(a, b, c, d, e, F, 0.0, t) + eint(a, b, c, d, e, F, 1.0, t));
-                else if (i < N1) T4 += fs * ft * eint(a, b, c, d, e, F, s, t);
+                if (i == 0) {
+                    double v = ft * (eint(a, b, c, d, e, F, 0.0, t) + eint(a, b, c, d, e, F, 1.0, t));
+                    if (g_exact) mdy_xs_add(&X, v); else T3 += v;
+                } else if (i < N1) {
+                    double v = fs * ft * eint(a, b, c, d, e, F, s, t);
+                    if (g_exact) mdy_xs_add(&X, v); else T4 += v;
+                }
             }
         }
     }
-    return kr * (ds * dt / 9.0) * (T1 + T2 + T3 + T4);
+    double T = g_exact ? mdy_xs_value(&X) : (T1 + T2 + T3 + T4);
+    return kr * (ds * dt / 9.0) * T;
 }
 
 /* forceN (:1001-1254): Simpson integrals of {1, s, s^2, st, t, t^2}/r^6 and the
@@ -390,40 +436,51 @@ void orc_ev_pair_force_simpson(const double *c1, const double *c2, const double 
     for (int k = 0; k < 3; k++) { vA[k] = c1[k] - c2[k]; vB[k] = c3[k] - c4[k]; vC[k] = c3[k] - c1[k]; }
     double ds = 1.0 / ((double)N1), dt = 1.0 / ((double)N2);
     double T1[6] = {0}, T2[6] = {0}, T3[6] = {0}, T4[6] = {0}, o[6];
+    mdy_xsum X[6];
+    for (int q = 0; q < 6; q++) mdy_xs_zero(&X[q]);
     const double lim[2] = {0.0, 1.0};
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) {
             fint(a, b, c, d, e, F, lim[i], lim[j], o);
-            for (int q = 0; q < 6; q++) T1[q] += o[q];
+            for (int q = 0; q < 6; q++) {
+                if (g_exact) mdy_xs_add(&X[q], o[q]); else T1[q] += o[q];
+            }
         }
     double s = 0.0, t = 0.0;
     for (int i = 0; i <= N1; i++) {
         t = 0.0;
-        s += ds;
+        s = g_exact ? (double)(i + 1) * ds : s + ds;
         double fs = (i % 2 == 0) ? 2.0 : 4.0;
         if (i > 0 && i < N1)
             for (int l = 0; l < 2; l++) {
                 fint(a, b, c, d, e, F, s, lim[l], o);
-                for (int q = 0; q < 6; q++) T2[q] += fs * o[q];
+                for (int q = 0; q < 6; q++) {
+                    if (g_exact) mdy_xs_add(&X[q], fs * o[q]); else T2[q] += fs * o[q];
+                }
             }
         for (int j = 0; j <= N2; j++) {
-            t += dt;
+            t = g_exact ? (double)(j + 1) * dt : t + dt;
             double ft = (j % 2 == 0) ? 2.0 : 4.0;
             if (j > 0 && j < N2) {
                 if (i == 0) {
                     for (int l = 0; l < 2; l++) {
                         fint(a, b, c, d, e, F, lim[l], t, o);
-                        for (int q = 0; q < 6; q++) T3[q] += ft * o[q];
+                        for (int q = 0; q < 6; q++) {
+                            if (g_exact) mdy_xs_add(&X[q], ft * o[q]); else T3[q] += ft * o[q];
+                        }
                     }
                 } else if (i < N1) {
                     fint(a, b, c, d, e, F, s, t, o);
-                    for (int q = 0; q < 6; q++) T4[q] += fs * ft * o[q];
+                    for (int q = 0; q < 6; q++) {
+                        if (g_exact) mdy_xs_add(&X[q], fs * ft * o[q]); else T4[q] += fs * ft * o[q];
+                    }
                 }
             }
         }
     }
     double in[6];
-    for (int q = 0; q < 6; q++) in[q] = (ds * dt / 9.0) * (T1[q] + T2[q] + T3[q] + T4[q]);
+    for (int q = 0; q < 6; q++)
+        in[q] = (ds * dt / 9.0) * (g_exact ? mdy_xs_value(&X[q]) : (T1[q] + T2[q] + T3[q] + T4[q]));
     for (int k = 0; k < 3; k++) {
         f12[0 + k] = 4.0 * kr * (-in[0] * vC[k] + in[1] * (vC[k] - vA[k]) + in[2] * vA[k] + in[4] * vB[k] - in[3] * vB[k]);
         f12[3 + k] = 4.0 * kr * (-in[1] * vC[k] - in[2] * vA[k] + in[3] * vB[k]);
@@ -437,7 +494,8 @@ static inline int ev_energy_bad(double U) { return fabs(U) == INFINITY || U != U
 /* CylinderExclVolRepulsion::energy, :329-575. The per-call side table of pair
  * energies (cylEnergies, :558-565) has no arithmetic effect and is omitted. */
 double orc_ev_energy(const double *x, int32_t n, const int32_t *bs, const double *krep, const double *eql) {
-    double U = 0.0;
+    osum U;
+    os_zero(&U);
     for (int32_t i = 0; i < n; i++) {
         double kr = krep[i] * eql[2 * i] * eql[2 * i + 1];
         const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
@@ -447,9 +505,9 @@ double orc_ev_energy(const double *x, int32_t n, const int32_t *bs, const double
             Ui = orc_ev_pair_energy_simpson(c1, c2, c3, c4, kr, 400, 400);
             if (ev_energy_bad(Ui)) return -1;
         }
-        U += Ui;
+        os_add(&U, Ui);
     }
-    return U;
+    return os_val(&U);
 }
 
 /* Analytic force of one pair, :626-860. out[4][3] = forces on c1..c4, with the
@@ -589,18 +647,19 @@ static inline double plane_dist(const double *pl, const double *p) {
 }
 double orc_bd_energy(const double *x, int32_t nplanes, const double *plane, const int32_t *nneigh,
                      const int32_t *bs, const double *krep, const double *slen) {
-    double U = 0.0;
+    osum U;
+    os_zero(&U);
     int64_t cum = 0;
     for (int32_t ib = 0; ib < nplanes; ib++) {
         for (int32_t ic = 0; ic < nneigh[ib]; ic++) {
             double r = plane_dist(plane + 4 * ib, x + bs[cum + ic]);
             double R = -r / slen[cum + ic];
-            double Ui = krep[cum + ic] * exp(R);
-            U += Ui;
+            double Ui = krep[cum + ic] * o_exp(R);
+            os_add(&U, Ui);
         }
         cum += nneigh[ib];
     }
-    return U;
+    return os_val(&U);
 }
 void orc_bd_forces(const double *x, double *f, int32_t nplanes, const double *plane, const int32_t *nneigh,
                    const int32_t *bs, const double *krep, const double *slen) {
@@ -610,7 +669,7 @@ void orc_bd_forces(const double *x, double *f, int32_t nplanes, const double *pl
         for (int32_t ic = 0; ic < nneigh[ib]; ic++) {
             double r = plane_dist(pl, x + bs[cum + ic]);
             double R = -r / slen[cum + ic];
-            double f0 = krep[cum + ic] * exp(R) / slen[cum + ic];
+            double f0 = krep[cum + ic] * o_exp(R) / slen[cum + ic];
             double *f1 = f + bs[cum + ic];
             f1[0] += f0 * pl[0];
             f1[1] += f0 * pl[1];
@@ -768,23 +827,31 @@ static void ffm_each(const orc_system *o, const double *x, double *e) {
  * sum; an energy <= -1 from any force field makes the total +inf. */
 double orc_ffm_energy(const orc_system *o, const double *x, double *per_ff) {
     double e[MDY_NFF];
+    mdy_xsum X;
+    mdy_xs_zero(&X);
+    g_total = &X;
     ffm_each(o, x, e);
+    g_total = NULL;
     if (per_ff) memcpy(per_ff, e, sizeof(e));
     double energy = 0.0;
     for (int k = 0; k < MDY_NFF; k++) {
         if (e[k] <= -1) return INFINITY;
         energy += e[k];
     }
-    return energy;
+    return g_exact ? mdy_xs_value(&X) : energy;
 }
 /* ForceFieldManager::computeIndividualEnergies, :209-227 (no sentinel) */
 double orc_ffm_energy_individual(const orc_system *o, const double *x, double *per_ff) {
     double e[MDY_NFF];
+    mdy_xsum X;
+    mdy_xs_zero(&X);
+    g_total = &X;
     ffm_each(o, x, e);
+    g_total = NULL;
     if (per_ff) memcpy(per_ff, e, sizeof(e));
     double total = 0.0;
     for (int k = 0; k < MDY_NFF; k++) total += e[k];
-    return total;
+    return g_exact ? mdy_xs_value(&X) : total;
 }
 /* ForceFieldManager::computeForces, :242-290 */
 int orc_ffm_forces(const orc_system *o, const double *x, double *f, int32_t nvar) {
@@ -822,21 +889,15 @@ static double cg_maxF(const cg_state *s) {
     for (int32_t i = 0; i < s->ndof; i++) m = smax(m, fabs(s->force[i]));
     return m;
 }
-static double cg_FdotF(const cg_state *s) {          /* full vector, CGMethod.cpp:325-331 */
-    double r = 0.0;
-    for (int32_t i = 0; i < s->nvar; i++) r += s->force[i] * s->force[i];
-    return r;
+static double cg_dot(const double *a, const double *b, int32_t n) {
+    osum r;
+    os_zero(&r);
+    for (int32_t i = 0; i < n; i++) os_add(&r, a[i] * b[i]);
+    return os_val(&r);
 }
-static double cg_FdotFprev(const cg_state *s) {
-    double r = 0.0;
-    for (int32_t i = 0; i < s->ndof; i++) r += s->force[i] * s->forcePrev[i];
-    return r;
-}
-static double cg_SdotF(const cg_state *s) {
-    double r = 0.0;
-    for (int32_t i = 0; i < s->ndof; i++) r += s->searchDir[i] * s->force[i];
-    return r;
-}
+static double cg_FdotF(const cg_state *s) { return cg_dot(s->force, s->force, s->nvar); }  /* CGMethod.cpp:325-331 */
+static double cg_FdotFprev(const cg_state *s) { return cg_dot(s->force, s->forcePrev, s->ndof); }
+static double cg_SdotF(const cg_state *s) { return cg_dot(s->searchDir, s->force, s->ndof); }
 static void cg_shift(cg_state *s, double beta) {
     for (int32_t i = 0; i < s->ndof; i++) s->searchDir[i] = s->force[i] + beta * s->searchDir[i];
 }
@@ -1035,4 +1096,35 @@ int orc_minimize_system(const orc_system *o, const mdy_cg_params *p, double *x, 
     int32_t nvar = 3 * o->sys.n_beads;
     return orc_minimize(p, x, nvar, nvar, sys_energy, sys_forces, orc_recover_move, (void *)o, out_force, res, trace,
                         iter_cap);
+}
+
+/* ------------------------------------------------------------------------- */
+/* shared numerics, exported for their own tests                             */
+/* ------------------------------------------------------------------------- */
+void orc_xmath_atan(const double *x, double *y, int64_t n) {
+    for (int64_t i = 0; i < n; i++) y[i] = mdy_atan(x[i]);
+}
+void orc_xmath_exp(const double *x, double *y, int64_t n) {
+    for (int64_t i = 0; i < n; i++) y[i] = mdy_exp(x[i]);
+}
+/* exact sum of x[0..n) added in the given order, or (stride > 1) in a
+ * strided interleaving merged afterwards: the result must not depend on it */
+double orc_xsum(const double *x, int64_t n, int64_t stride) {
+    if (stride <= 1) {
+        mdy_xsum s;
+        mdy_xs_zero(&s);
+        for (int64_t i = 0; i < n; i++) mdy_xs_add(&s, x[i]);
+        return mdy_xs_value(&s);
+    }
+    mdy_xsum tot;
+    mdy_xs_zero(&tot);
+    for (int64_t r = stride - 1; r >= 0; r--) {
+        mdy_xsum s;
+        mdy_xs_zero(&s);
+        for (int64_t i = r; i < n; i += stride) mdy_xs_add(&s, x[i]);
+        mdy_xs_norm(&s);
+        mdy_xs_merge(&tot, &s);
+        mdy_xs_norm(&tot);
+    }
+    return mdy_xs_value(&tot);
 }

@@ -36,6 +36,13 @@
 extern "C" {
 #endif
 
+/* ---- numerics mode -------------------------------------------------------- */
+/* 0 (default): the C library's atan/exp and sequential sums, as the reference.
+ * 1: the shared mdy_atan/mdy_exp and exact order-independent sums that the HIP
+ *    kernels use (medyan_amd/csrc/mdy_xmath.h); the modes differ by rounding. */
+void orc_set_exact(int on);
+int  orc_get_exact(void);
+
 /* ---- per force field (reference vectorized layout) ---------------------- */
 double orc_fs_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0);
 void   orc_fs_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0);
@@ -113,6 +120,11 @@ int orc_minimize_system(const orc_system *o, const mdy_cg_params *p, double *x, 
                         mdy_min_result *res, orc_trace *trace, int32_t iter_cap);
 /* the reference recovery move (MController::moveAlongDirectionLimitingBeadMove_) */
 int orc_recover_move(void *user, double *x, const double *dir, int32_t ndof, double maxdist);
+
+/* shared numerics of medyan_amd/csrc/mdy_xmath.h, exported for their tests */
+void   orc_xmath_atan(const double *x, double *y, int64_t n);
+void   orc_xmath_exp(const double *x, double *y, int64_t n);
+double orc_xsum(const double *x, int64_t n, int64_t stride);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,37 @@ class Oracle:
         L.orc_minimize.argtypes = [C.POINTER(_abi.CGParams), _f64p, C.c_int32, C.c_int32, ENERGY_FN, FORCE_FN,
                                    RECOVER_FN, C.c_void_p, _f64p, C.POINTER(_abi.MinResult), C.POINTER(OrcTrace),
                                    C.c_int32]
+        L.orc_set_exact.argtypes = [C.c_int]
+        L.orc_get_exact.restype = C.c_int
+        L.orc_xsum.restype = C.c_double
+        L.orc_xsum.argtypes = [_f64p, C.c_int64, C.c_int64]
+        L.orc_xmath_atan.argtypes = [_f64p, _f64p, C.c_int64]
+        L.orc_xmath_exp.argtypes = [_f64p, _f64p, C.c_int64]
+
+    # -- numerics mode ------------------------------------------------------
+    def set_exact(self, on):
+        """1: the GPU's shared atan/exp and exact sums; 0: libm and sequential sums."""
+        self.lib.orc_set_exact(1 if on else 0)
+
+    @property
+    def exact(self):
+        return bool(self.lib.orc_get_exact())
+
+    def xmath_atan(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.lib.orc_xmath_atan(_p(x), _p(y), x.size)
+        return y
+
+    def xmath_exp(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.lib.orc_xmath_exp(_p(x), _p(y), x.size)
+        return y
+
+    def xsum(self, x, stride=1):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        return self.lib.orc_xsum(_p(x), x.size, int(stride))
 
     # -- whole system -----------------------------------------------------
     def system(self, net, coord=None, coplanar_mode=_abi.MDY_COPLANAR_REFERENCE):
