@@ -219,6 +219,33 @@ class Context:
             raise MdyError(rc, self.lib.mdy_last_error(self.h).decode())
         return x, f, res
 
+    # -- device-pointer forms (coordinates already in device memory) ------------
+    def build_neighbors_dev(self, d_coord: int):
+        """Neighbor lists from device coordinates (a device address, e.g. a
+        torch tensor's data_ptr())."""
+        self._check(self.lib.mdy_build_neighbors_dev(self.h, C.c_void_p(d_coord)))
+
+    def minimize_dev(self, params, d_coord: int, d_force_out: int = 0, trace_cap=0):
+        """Minimizes the device coordinates in place; returns the MinResult."""
+        if isinstance(params, ConjugateGradientParams):
+            params = params.to_c(self.net.n_beads)
+        self._check(self.lib.mdy_set_trace(self.h, int(trace_cap)))
+        res = _abi.MinResult()
+        rc = self.lib.mdy_minimize_dev(self.h, C.byref(params), C.c_void_p(d_coord),
+                                       C.c_void_p(d_force_out) if d_force_out else None, C.byref(res))
+        if rc != 0:
+            raise MdyError(rc, self.lib.mdy_last_error(self.h).decode())
+        return res
+
+    def energy_dev(self, d_coord: int, d_out: int, stream: int = 0):
+        """d_out: MDY_NFF per-FF energies then the computeEnergy total."""
+        self._check(self.lib.mdy_energy_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_out),
+                                            C.c_void_p(stream) if stream else None))
+
+    def forces_dev(self, d_coord: int, d_force: int, accumulate=True, stream: int = 0):
+        self._check(self.lib.mdy_forces_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_force), 1 if accumulate else 0,
+                                            C.c_void_p(stream) if stream else None))
+
     def trace(self):
         n = C.c_int64()
         self._check(self.lib.mdy_get_trace(self.h, None, None, None, None, None, 0, C.byref(n)))
