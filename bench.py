@@ -165,7 +165,7 @@ def main():
     if r.grad_inf_norm >= 1.0 or r.status != 0:
         print(f"minimization did not reach 1 pN: maxF={r.grad_inf_norm} status={r.status}", file=sys.stderr)
         sys.exit(1)
-    n_ev = len(ctx.pairs())
+    n_ev = len(ctx.exvol_table()[1])
     n_bd = len(ctx.boundary_table()[1])
     abytes = algorithmic_bytes(net, n_ev, n_bd, r)
     k_ms = float(np.mean(kernel_ms))

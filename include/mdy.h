@@ -185,9 +185,15 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *sys);
 int mdy_build_neighbors(mdy_ctx *ctx, const double *coord);
 /* Same from device-resident coordinates. */
 int mdy_build_neighbors_dev(mdy_ctx *ctx, const double *d_coord);
-/* Cylinder-cylinder pairs (index of c, index of neighbor n, id(c) > id(n)),
- * sorted by (c, n); mother-daughter pairs already dropped. */
+/* Cylinder-cylinder neighbor pairs (index of c, index of neighbor n,
+ * id(c) > id(n)), sorted by (c, n): the list HybridCylinderCylinderNL holds,
+ * mother-daughter pairs included. */
 int mdy_get_pairs(mdy_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n);
+/* Excluded-volume interaction table built from those pairs
+ * (CylinderExclVolume::vectorize, CylinderExclVolume.cpp:34-179: pairs whose
+ * neighbor is the cylinder's mother dropped): 4 bead offsets per pair and
+ * kR = k * l1 * l2. beadset / krep may be NULL to query the count. */
+int mdy_get_exvol_table(mdy_ctx *ctx, int32_t *beadset, double *krep, int64_t cap, int64_t *n);
 /* Boundary interaction table: per-plane counts and bead offsets (reference order). */
 int mdy_get_boundary_table(mdy_ctx *ctx, int32_t *nneighbors, int32_t *beadset, int64_t cap, int64_t *n);
 

@@ -14,9 +14,16 @@ ERRORS = {
     -5: "MDY_ERR_CAPACITY", -6: "MDY_ERR_NOGPU", -7: "MDY_ERR_LAUNCH", -8: "MDY_ERR_FORCE",
     -9: "MDY_ERR_RECOVERY", -10: "MDY_ERR_TIMEOUT",
 }
+MDY_ERR_ARG = -1
+MDY_ERR_HIP = -2
+MDY_ERR_STATE = -3
 MDY_ERR_NONFINITE = -4
+MDY_ERR_CAPACITY = -5
+MDY_ERR_NOGPU = -6
+MDY_ERR_LAUNCH = -7
 MDY_ERR_FORCE = -8
 MDY_ERR_RECOVERY = -9
+MDY_ERR_TIMEOUT = -10
 
 FF_NAMES = ["FilamentStretching", "FilamentBending", "LinkerStretching", "MotorGhostStretching",
             "BranchingStretching", "BranchingBending", "CylinderExclVolume", "BoundaryCylinderRepulsion"]

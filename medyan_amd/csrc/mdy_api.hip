@@ -314,6 +314,21 @@ int mdy_get_pairs(mdy_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n) {
     return MDY_OK;
 }
 
+int mdy_get_exvol_table(mdy_ctx *ctx, int32_t *beadset, double *krep, int64_t cap, int64_t *n) {
+    if (!ctx || !n) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.have_nl) return fail(ctx, MDY_ERR_STATE, "no neighbor list");
+    *n = c.n_ev;
+    if (!beadset && !krep) return MDY_OK;
+    if (cap < c.n_ev) return fail(ctx, MDY_ERR_CAPACITY, "exvol buffer too small");
+    if (c.n_ev && beadset &&
+        hipMemcpy(beadset, c.ev_bs.p, sizeof(int4) * c.n_ev, hipMemcpyDeviceToHost) != hipSuccess)
+        return hipfail(ctx, "exvol copy");
+    if (c.n_ev && krep && hipMemcpy(krep, c.ev_kr.p, sizeof(double) * c.n_ev, hipMemcpyDeviceToHost) != hipSuccess)
+        return hipfail(ctx, "exvol copy");
+    return MDY_OK;
+}
+
 int mdy_get_boundary_table(mdy_ctx *ctx, int32_t *nneighbors, int32_t *beadset, int64_t cap, int64_t *n) {
     if (!ctx || !n) return MDY_ERR_ARG;
     Ctx &c = ctx->c;

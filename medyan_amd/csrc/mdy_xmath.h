@@ -52,11 +52,14 @@ MDY_HD double mdy_atan(double x) {
     double ax = fabs(x);
     int inv = ax > 1.0;
     double y = inv ? 1.0 / ax : ax;              /* y in [0, 1]; 1/inf = 0 */
-    int k = (int)(y * 8.0 + 0.5);                /* nearest table point k/8 */
+    /* nearest table point c = k/8, except below 3/32 where the series is
+     * used directly: reducing y near 1/16 to t ~ -y would pass the rounding
+     * error of t undiminished into a result of the same size */
+    int k = y < 0.09375 ? 0 : (int)(y * 8.0 + 0.5);
     double c = (double)k * 0.125;
-    double t = (y - c) / (1.0 + y * c);          /* |t| <= 1/16: atan(y) = atan(c) + atan(t) */
+    double t = (y - c) / (1.0 + y * c);          /* |t| <= 3/32: atan(y) = atan(c) + atan(t) */
     double t2 = t * t;
-    /* atan(t) = t - t^3/3 + t^5/5 - ... ; |t|^17/17 < 2^-70 |t| */
+    /* atan(t) = t - t^3/3 + t^5/5 - ... ; |t|^17/17 < 2^-58 |t| */
     double p = -1.0 / 15.0;
     p = p * t2 + 1.0 / 13.0;
     p = p * t2 - 1.0 / 11.0;

@@ -58,6 +58,7 @@ def load_library(path: str = LIB_PATH):
         "mdy_build_neighbors_dev": (C.c_int, [vp, vp]),
         "mdy_get_pairs": (C.c_int, [vp, i32p, C.c_int64, C.POINTER(C.c_int64)]),
         "mdy_get_boundary_table": (C.c_int, [vp, i32p, i32p, C.c_int64, C.POINTER(C.c_int64)]),
+        "mdy_get_exvol_table": (C.c_int, [vp, i32p, f64p, C.c_int64, C.POINTER(C.c_int64)]),
         "mdy_energy": (C.c_int, [vp, f64p, f64p, f64p]),
         "mdy_forces": (C.c_int, [vp, f64p, f64p]),
         "mdy_energy_dev": (C.c_int, [vp, vp, vp, vp]),
@@ -177,6 +178,16 @@ class Context:
         out = np.zeros(2 * max(n.value, 1), dtype=np.int32)
         self._check(self.lib.mdy_get_pairs(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n)))
         return out[: 2 * n.value].reshape(-1, 2)
+
+    def exvol_table(self):
+        """(beadset [n x 4], kR [n]) of the excluded-volume interactions."""
+        n = C.c_int64()
+        self._check(self.lib.mdy_get_exvol_table(self.h, None, None, 0, C.byref(n)))
+        bs = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+        kr = np.zeros(max(n.value, 1))
+        self._check(self.lib.mdy_get_exvol_table(self.h, bs.ctypes.data_as(C.POINTER(C.c_int32)), _f64(kr), n.value,
+                                                 C.byref(n)))
+        return bs[: 4 * n.value].reshape(-1, 4), kr[: n.value]
 
     def boundary_table(self):
         n = C.c_int64()

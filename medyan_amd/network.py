@@ -204,3 +204,88 @@ def generate(lib, workload: Optional[str] = None, **overrides) -> Network:
         return from_c(out.contents)
     finally:
         lib.mdy_network_free(out)
+
+
+def from_filaments(filaments, box, linkers=(), motors=(), branches=(), cyl_size=108.0, k_stretch=100.0,
+                   k_bend=672.0, theta_bend=0.0, k_exvol=1e5, exvol_cutoff=108.0, k_bd=41.0, screen_bd=2.7,
+                   bd_cutoff=125.0, k_br_stretch=100.0, l_br_stretch=6.0, k_br_bend=10.0,
+                   theta_br_bend=70.0 * np.pi / 180.0) -> Network:
+    """A network from explicit filament bead coordinates (one (n_i+1) x 3 array
+    per filament, minus end first), with the generator's rules for the
+    derived tables: one stretching term per cylinder, one bending term per
+    consecutive cylinder pair, cubic boundary planes 25 nm inside the box.
+
+    linkers / motors: tuples (fil1, cyl1, pos1, fil2, cyl2, pos2, k, l0), the
+    cylinder given by its index along its filament. branches: tuples
+    (mother_fil, mother_cyl, pos, daughter_fil); the daughter filament's first
+    cylinder is the branch child.
+    """
+    fil = [np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 3) for f in filaments]
+    first, coord = [], []
+    nb = 0
+    for f in fil:
+        if len(f) < 2:
+            raise ValueError("a filament needs at least two beads")
+        first.append(nb)
+        coord.append(f)
+        nb += len(f)
+    coord = np.concatenate(coord).ravel() if fil else np.zeros(0)
+    b1, b2, cfil, cpos, cminus, cfirst = [], [], [], [], [], []
+    for fi, f in enumerate(fil):
+        cfirst.append(len(b1))
+        for c in range(len(f) - 1):
+            b1.append(3 * (first[fi] + c)); b2.append(3 * (first[fi] + c + 1))
+            cfil.append(fi); cpos.append(c); cminus.append(1 if c == 0 else 0)
+    nc = len(b1)
+    eq = np.full(nc, cyl_size)
+    child = np.full(nc, -1, np.int32)
+    fs_bs = np.array([[b1[i], b2[i]] for i in range(nc)], np.int32).ravel()
+    fb_bs, fb_k = [], []
+    for fi, f in enumerate(fil):
+        for c in range(1, len(f) - 1):
+            prev, cur = cfirst[fi] + c - 1, cfirst[fi] + c
+            fb_bs += [b1[prev], b1[cur], b2[cur]]
+            fb_k.append(k_bend * cyl_size / eq[cur])
+
+    def links(items):
+        bs, k, l0, p1, p2 = [], [], [], [], []
+        for (f1, c1, q1, f2, c2, q2, kk, ll) in items:
+            i1, i2 = cfirst[f1] + c1, cfirst[f2] + c2
+            bs += [b1[i1], b2[i1], b1[i2], b2[i2]]
+            k.append(kk); l0.append(ll); p1.append(q1); p2.append(q2)
+        return (np.array(bs, np.int32), np.array(k, np.float64), np.array(l0, np.float64),
+                np.array(p1, np.float64), np.array(p2, np.float64))
+
+    ls = links(linkers)
+    ms = links(motors)
+    bs_bs, bs_k, bs_l0, bs_p, bb_bs, bb_k, bb_t = [], [], [], [], [], [], []
+    for (mf, mc, q, df) in branches:
+        m, d = cfirst[mf] + mc, cfirst[df]
+        child[m] = d
+        bs_bs += [b1[m], b2[m], b1[d]]
+        bs_k.append(k_br_stretch); bs_l0.append(l_br_stretch); bs_p.append(q)
+        bb_bs += [b1[m], b2[m], b1[d], b2[d]]
+        bb_k.append(k_br_bend); bb_t.append(theta_br_bend)
+    L = float(box)
+    z0, hi = 25.0, L - 25.0
+    pts = [(z0, hi / 2, hi / 2), (hi, hi / 2, hi / 2), (hi / 2, z0, hi / 2), (hi / 2, hi, hi / 2),
+           (hi / 2, hi / 2, z0), (hi / 2, hi / 2, hi)]
+    nrm = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    plane = []
+    for p, n in zip(pts, nrm):
+        plane += [n[0], n[1], n[2], -n[0] * p[0] - n[1] * p[1] - n[2] * p[2]]
+    f64 = lambda a: np.array(a, dtype=np.float64)
+    return Network(
+        coord=coord, n_beads=nb,
+        cyl_bead1=np.array(b1, np.int32), cyl_bead2=np.array(b2, np.int32),
+        cyl_id=np.arange(nc, dtype=np.int64), cyl_filament=np.array(cfil, np.int32),
+        cyl_position=np.array(cpos, np.int32), cyl_type=np.zeros(nc, np.int16), cyl_eq_length=eq,
+        cyl_exvol_k=np.full(nc, k_exvol), cyl_branch_child=child, cyl_minus_end=np.array(cminus, np.uint8),
+        fs_beadset=fs_bs, fs_k=k_stretch * (cyl_size / eq), fs_l0=eq.copy(),
+        fb_beadset=np.array(fb_bs, np.int32), fb_k=f64(fb_k), fb_theta0=np.full(len(fb_k), theta_bend),
+        ls_beadset=ls[0], ls_k=ls[1], ls_l0=ls[2], ls_pos1=ls[3], ls_pos2=ls[4],
+        ms_beadset=ms[0], ms_k=ms[1], ms_l0=ms[2], ms_pos1=ms[3], ms_pos2=ms[4],
+        bs_beadset=np.array(bs_bs, np.int32), bs_k=f64(bs_k), bs_l0=f64(bs_l0), bs_pos=f64(bs_p),
+        bb_beadset=np.array(bb_bs, np.int32), bb_k=f64(bb_k), bb_theta0=f64(bb_t),
+        plane=f64(plane), plane_k=np.full(6, k_bd), plane_screen=np.full(6, screen_bd),
+        bd_cutoff=float(bd_cutoff), exvol_cutoff=float(exvol_cutoff), box=L)

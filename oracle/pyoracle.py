@@ -122,6 +122,33 @@ class Oracle:
         return bool(self.lib.orc_are_in_plane(*[_p(x) for x in a]))
 
     # -- per-FF on explicit tables ----------------------------------------
+    # bonded / cross-link force fields: name in fs, fb, ls, bs, bb; params in
+    # the C order (fs: k, l0; fb: k, theta0; ls: k, l0, pos1, pos2;
+    # bs: k, l0, pos; bb: k, theta0); bead offsets already x3
+    _NPAR = {"fs": 2, "fb": 2, "ls": 4, "bs": 3, "bb": 2}
+    _FSTRETCH = {"ls": 1, "bs": 3, "bb": 3}
+
+    def _ff_args(self, name, x, bs, params):
+        if len(params) != self._NPAR[name]:
+            raise ValueError(f"{name}: {self._NPAR[name]} parameter arrays expected")
+        x = np.ascontiguousarray(x, np.float64)
+        bs = np.ascontiguousarray(bs, np.int32)
+        ps = [np.ascontiguousarray(p, np.float64) for p in params]
+        return x, bs, ps, len(ps[0])
+
+    def ff_energy(self, name, x, bs, *params):
+        x, bs, ps, n = self._ff_args(name, x, bs, params)
+        return getattr(self.lib, f"orc_{name}_energy")(_p(x), n, _p(bs, C.c_int32), *[_p(p) for p in ps])
+
+    def ff_forces(self, name, x, bs, *params):
+        x, bs, ps, n = self._ff_args(name, x, bs, params)
+        f = np.zeros_like(x)
+        extra = []
+        if name in self._FSTRETCH:
+            extra = [None]
+        getattr(self.lib, f"orc_{name}_forces")(_p(x), _p(f), n, _p(bs, C.c_int32), *[_p(p) for p in ps], *extra)
+        return f
+
     def ev_energy(self, x, bs, krep, eql):
         x, bs, krep, eql = (np.ascontiguousarray(x, np.float64), np.ascontiguousarray(bs, np.int32),
                             np.ascontiguousarray(krep, np.float64), np.ascontiguousarray(eql, np.float64))

@@ -1,0 +1,132 @@
+"""Pins of the oracle's CGMethod::minimize restatement.
+
+The reference's own minimizer test (src/TESTS/Mechanics/Minimizer/
+TestCGMethod.cpp:20-92: a 10-D weighted quadratic, three initial points, three
+direction rules) fixes convergence; the Armijo condition fixes monotone
+energies over accepted steps; the exit ladders (call limit, line-search error
+flag, safe mode, recovery, non-finite force) are forced from the host.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from medyan_amd import _abi
+
+OPT = np.array([-500.0, -400.0, -300.0, -200.0, -100.0, 0.0, 100.0, 200.0, 300.0, 400.0])
+W = np.array([3, 1, 0.1, 0.5, 1, 10, 20, 50, 100, 0.1])
+
+
+def quad_energy(x):
+    return float(np.sum(0.5 * W * (x[:10] - OPT) ** 2))
+
+
+def quad_force(x, f):
+    f[:10] = -W * (x[:10] - OPT)
+
+
+def quad_params(method, **kw):
+    p = _abi.default_cg_params(0, grad_tol=0.1, max_dist=10.0, lambda_max=1.0, method=method,
+                               try_to_recover=0, force_call_limit=10000)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+INITS = {
+    "input_0": np.full(10, 10.0),            # vector<floatingpoint>(800, n): every entry equals n = 10
+    "input_1_already_minimized": OPT.copy(),
+    "input_2": np.array([1000.0, 900, 800, 700, 600, 500, 400, 300, 200, 100]),
+}
+
+
+@pytest.mark.parametrize("method", [_abi.MDY_CG_STEEPEST, _abi.MDY_CG_FLETCHER_RIEVES, _abi.MDY_CG_POLAK_RIBIERE])
+@pytest.mark.parametrize("init", list(INITS))
+def test_reference_quadratic(oracle, method, init):
+    p = quad_params(method, energy_rel_tol=0.0)
+    x, f, res, _ = oracle.minimize(p, INITS[init], 10, quad_energy, quad_force)
+    assert res.status == 0 and res.error_flags == 0
+    assert np.all(np.abs(f) <= p.grad_tol)
+    # the exact minimizer is known: |x_i - opt_i| = |F_i| / w_i
+    assert np.all(np.abs(x - OPT) <= p.grad_tol / W * (1 + 1e-12))
+    if init == "input_1_already_minimized":
+        assert res.num_iter == 0
+
+
+def test_armijo_monotone_on_network(oracle, lib):
+    from medyan_amd import network
+    net = network.generate(lib, "net500", n_filaments=12, n_linkers=20, n_motors=4, seed=11)
+    s = oracle.system(net)
+    p = _abi.default_cg_params(net.n_beads)
+    x, f, res, tr = s.minimize(p, trace_cap=100000)
+    assert res.status == 0 and res.error_flags == 0
+    assert res.grad_inf_norm < p.grad_tol or res.energy_rel_change < p.energy_rel_tol
+    acc = tr["lambda_"] > 0
+    e = np.concatenate([[res.energies_before[_abi.MDY_NFF]], tr["energy"][acc]])
+    assert np.all(np.diff(e) <= 1e-9 * np.abs(e[:-1]))
+    # the returned force is the force at the returned coordinates
+    np.testing.assert_array_equal(f, s.forces(x))
+    assert abs(np.abs(f).max() - res.grad_inf_norm) == 0.0
+
+
+def test_call_limit_stops(oracle):
+    p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, force_call_limit=5, n_beads_limit=1, energy_rel_tol=0.0,
+                    grad_tol=1e-12)
+    x, f, res, _ = oracle.minimize(p, INITS["input_2"], 10, quad_energy, quad_force)
+    assert res.force_call_limit == 5
+    assert res.num_force_call == 6          # the loop tests the limit before each iteration
+    assert res.grad_inf_norm > p.grad_tol
+
+
+def uphill_force(x, f):
+    """A force pointing up the energy gradient: no step can satisfy Armijo."""
+    f[:10] = W * (x[:10] - OPT)
+
+
+def test_line_search_error_without_recovery(oracle):
+    p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0)
+    x0 = INITS["input_2"]
+    x, f, res, tr = oracle.minimize(p, x0, 10, quad_energy, uphill_force, trace_cap=10)
+    assert res.error_flags & _abi.MDY_ERROR_LINESEARCH
+    assert res.status == 0 and res.num_iter == 1
+    np.testing.assert_array_equal(x, x0)      # lambda = 0: no move
+    assert tr["lambda_"][0] == 0.0
+
+
+def test_safe_mode_then_recovery(oracle):
+    p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0, try_to_recover=1, force_call_limit=4,
+                    n_beads_limit=1)
+    moves = []
+
+    def recover(x, d, maxdist):
+        step = np.min(maxdist / np.abs(d))
+        moves.append(step)
+        x[:] += step * d
+
+    x, f, res, tr = oracle.minimize(p, INITS["input_2"], 10, quad_energy, uphill_force, recover=recover,
+                                    trace_cap=20)
+    assert res.num_safe_mode >= 1 and res.num_recovery >= 1
+    assert list(tr["safe"][:2]) == [1, 0]      # first failure enters safe mode, the second recovers
+    assert len(moves) == res.num_recovery
+
+
+def test_nonfinite_force_raises(oracle):
+    def bad_force(x, f):
+        quad_force(x, f)
+        f[3] = np.inf
+
+    p = quad_params(_abi.MDY_CG_POLAK_RIBIERE)
+    _, _, res, _ = oracle.minimize(p, INITS["input_2"], 10, quad_energy, bad_force)
+    assert res.status == _abi.MDY_ERR_NONFINITE
+
+
+def test_recovery_move(oracle):
+    """MController::moveAlongDirectionLimitingBeadMove_: step = min_i maxDist/|d_i|."""
+    import pyoracle
+    x = np.array([1.0, 2.0, 3.0])
+    d = np.array([1.0, -2.0, 0.5])
+    rc = oracle.lib.orc_recover_move(None, pyoracle._p(x), pyoracle._p(d), 3, C.c_double(1.0))
+    assert rc == 0
+    np.testing.assert_array_equal(x, [1.5, 1.0, 3.25])
+    z = np.zeros(3)
+    assert oracle.lib.orc_recover_move(None, pyoracle._p(x), pyoracle._p(z), 3, C.c_double(1.0)) == _abi.MDY_ERR_RECOVERY

@@ -1,0 +1,76 @@
+"""Neighbor lists of the oracle against an independent numpy restatement of
+the reference predicate (HybridNeighborListImpl.h:74-200, .cpp:332-471;
+NeighborListImpl.cpp:738-751; CylinderExclVolume.cpp:85,116):
+half list by id(c) > id(n); same-filament pairs with |dpos| <= 2 excluded;
+squared midpoint distance (double) compared with float(rmax)^2 (float
+arithmetic, promoted); mother -> daughter pairs dropped from the
+excluded-volume table; boundary lists by signed plane distance of the
+midpoint below the float cutoff."""
+import numpy as np
+import pytest
+
+from medyan_amd import network
+
+
+def np_pairs(net, coord, rmax):
+    x = coord.reshape(-1, 3)
+    m = 0.5 * (x[net.cyl_bead1 // 3] + x[net.cyl_bead2 // 3])
+    rsq = float(np.float32(rmax) * np.float32(rmax))
+    out = []
+    n = net.n_cyl
+    for i in range(n):
+        d2 = ((m - m[i]) ** 2).sum(axis=1)
+        ok = (net.cyl_id[i] > net.cyl_id) & (d2 <= rsq)
+        same = (net.cyl_filament == net.cyl_filament[i]) & (np.abs(net.cyl_position - net.cyl_position[i]) <= 2)
+        ok &= ~same
+        for j in np.nonzero(ok)[0]:
+            out.append((i, j))
+    return np.array(out, dtype=np.int32).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("kw", [dict(n_filaments=50, seed=1), dict(n_filaments=40, cyl_per_filament=5, n_branches=8, seed=2),
+                                dict(n_filaments=30, cyl_per_filament=3, box=500.0, n_linkers=20, n_motors=2, seed=3)])
+def test_nl_brute_force_matches_numpy(oracle, lib, kw):
+    net = network.generate(lib, "net500", **kw)
+    got = oracle.nl_pairs(net)
+    ref = np_pairs(net, net.coord, net.exvol_cutoff)
+    np.testing.assert_array_equal(got, ref)
+    assert len(got) > 0
+
+
+def test_ev_table_drops_mother_daughter(oracle, lib):
+    net = network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=2)
+    s = oracle.system(net)
+    pairs = oracle.nl_pairs(net)
+    bs, kr, eq = s.ev_table
+    keep = [(i, j) for i, j in pairs if net.cyl_branch_child[j] != i]
+    assert len(keep) == len(bs)
+    assert len(keep) < len(pairs)           # some daughter is within the cutoff of its mother
+    exp = np.array([[net.cyl_bead1[i], net.cyl_bead2[i], net.cyl_bead1[j], net.cyl_bead2[j]] for i, j in keep])
+    np.testing.assert_array_equal(bs, exp)
+    np.testing.assert_array_equal(kr, net.cyl_exvol_k[[i for i, _ in keep]])
+
+
+def test_boundary_table(oracle, lib):
+    """Per plane, in cylinder order: the first bead of a minus-end cylinder,
+    then the second bead of every cylinder whose midpoint lies closer to the
+    plane than the cutoff."""
+    net = network.generate(lib, "net500", n_filaments=50, cyl_per_filament=4, seed=5, box=600.0)
+    s = oracle.system(net)
+    nn, bs = s.bd_table
+    x = net.coord.reshape(-1, 3)
+    m = x[net.cyl_bead1 // 3] * (1.0 - 0.5) + 0.5 * x[net.cyl_bead2 // 3]   # midPointCoordinate(b1, b2, 0.5)
+    exp_nn, exp_bs = [], []
+    pl = net.plane.reshape(-1, 4)
+    for p in range(len(pl)):
+        a, b, c, d = pl[p]
+        dist = (m @ np.array([a, b, c]) + d) / np.sqrt(a * a + b * b + c * c)
+        cnt = 0
+        for i in np.nonzero(dist < np.float32(net.bd_cutoff))[0]:
+            if net.cyl_minus_end[i]:
+                exp_bs.append(net.cyl_bead1[i]); cnt += 1
+            exp_bs.append(net.cyl_bead2[i]); cnt += 1
+        exp_nn.append(cnt)
+    np.testing.assert_array_equal(nn, exp_nn)
+    np.testing.assert_array_equal(bs, np.array(exp_bs, dtype=np.int32))
+    assert sum(exp_nn) > 0
