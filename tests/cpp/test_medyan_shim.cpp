@@ -1,0 +1,78 @@
+// Drives the C++ mirror of MEDYAN's mechanics interface (include/mdy_medyan.hpp)
+// the way MController::run drives the reference (Controller/MController.h:109-211):
+// vectorize, energies, forces, minimize. Needs a HIP device. Prints "ALL OK".
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "mdy_medyan.hpp"
+
+#define REQUIRE(c)                                                         \
+    do {                                                                   \
+        if (!(c)) {                                                        \
+            std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #c); \
+            std::exit(1);                                                  \
+        }                                                                  \
+    } while (0)
+
+using namespace mdy_medyan;
+
+int main() {
+    mdy_gen_params gp;
+    mdy_gen_defaults(&gp);
+    gp.n_filaments = 20;
+    gp.n_linkers = 40;
+    gp.n_motors = 5;
+    gp.seed = 7;
+    mdy_network *net = nullptr;
+    REQUIRE(mdy_generate(&gp, &net) == MDY_OK);
+    const int ndof = 3 * net->sys.n_beads;
+    std::vector<double> coord(net->coord, net->coord + ndof);
+
+    ForceFieldManager ffm(0);
+    ffm.vectorizeAllForceFields(net->sys, coord.data());
+
+    // computeEnergy == sum of computeIndividualEnergies for a valid configuration
+    const double e = ffm.computeEnergy(coord.data());
+    EnergyReport rep = ffm.computeIndividualEnergies(coord.data());
+    REQUIRE(rep.individual.size() == MDY_NFF);
+    REQUIRE(std::fabs(e - rep.total) <= 1e-12 * std::fabs(e));
+
+    // computeForces zero-fills, so a dirty buffer gives the same forces
+    std::vector<double> f1(ndof, 0.0), f2(ndof, 123.0);
+    ffm.computeForces(coord.data(), f1.data(), ndof);
+    ffm.computeForces(coord.data(), f2.data(), ndof);
+    REQUIRE(f1 == f2);
+
+    // minimize to the benchmark tolerance
+    ConjugateGradientParams p;
+    p.gradTol = 1.0;
+    p.maxDist = 1.0;
+    p.lambdaMax = 0.01;
+    p.forceCallLimit = 100000;
+    std::vector<double> out;
+    MinimizationResult r = CGMethod{}.minimize(p, coord, ndof, ffm, &out);
+    REQUIRE(r.success());
+    REQUIRE(r.converged());
+    REQUIRE(r.energiesAfter.total < r.energiesBefore.total);
+    double mf = 0;
+    for (double v : out) mf = std::max(mf, std::fabs(v));
+    REQUIRE(mf == r.gradInfNorm);
+    REQUIRE(std::fabs(ffm.computeEnergy(coord.data()) - r.energiesAfter.total) <= 1e-12 * std::fabs(r.energiesAfter.total));
+
+    // misuse is reported like the reference reports it: by exception
+    bool threw = false;
+    try {
+        std::vector<double> wrong(ndof - 3);
+        CGMethod{}.minimize(p, wrong, ndof - 3, ffm);
+    } catch (const Error &err) {
+        threw = err.code() == MDY_ERR_ARG;
+    }
+    REQUIRE(threw);
+
+    mdy_network_free(net);
+    std::printf("ALL OK: %d CG iterations, max|F| %.4f pN\n", r.numIter, r.gradInfNorm);
+    return 0;
+}

@@ -237,15 +237,21 @@ def test_disabled_force_fields(exact, lib):
     assert bits_equal(ctx.forces(net.coord), s.forces())
 
 
-def test_single_filament(exact):
-    f1 = np.array([[300.0 + 100 * i, 300.0 + 3.0 * (i % 2), 300.0] for i in range(6)])
+def test_single_filament_hairpin(exact):
+    """One filament folded back on itself: pairs on the same filament count
+    only when their positions differ by more than 2, and the exactly parallel
+    legs go through the quadrature fallback."""
+    f1 = np.array([[300.0, 300, 300], [400, 300, 300], [500, 300, 300], [500, 330, 300], [400, 330, 300],
+                   [300, 330, 300]])
     net = network.from_filaments([f1], 1000.0)
     ctx = ctx_for(net)
     s = exact.system(net)
-    assert len(ctx.pairs()) == len(s.ev_table[0]) == 3     # |dpos| > 2 pairs on one filament
+    np.testing.assert_array_equal(ctx.pairs(), [[3, 0], [4, 0], [4, 1]])
+    assert len(s.ev_table[0]) == 3
     p = _abi.default_cg_params(net.n_beads)
     x, f, res = ctx.minimize(p)
     xo, fo, reso, _ = s.minimize(p)
+    assert res.status == reso.status == 0
     assert bits_equal(x, xo) and res.num_iter == reso.num_iter
 
 
