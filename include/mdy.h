@@ -237,6 +237,9 @@ int mdy_last_timing(mdy_ctx *ctx, double *ms_total);
  * 5 gather, 6 reduction, 7 restore). Recording is off unless a capacity is set. */
 int mdy_set_profile(mdy_ctx *ctx, int64_t cap);
 int mdy_get_profile(mdy_ctx *ctx, int32_t *kind, int64_t *ticks, int64_t cap, int64_t *n);
+/* Per phase 5 stamps: barrier exit, latest block arrival at the barrier,
+ * block 0's work start, end of its main loop, and its arrival. */
+int mdy_get_profile_detail(mdy_ctx *ctx, int64_t *rec, int64_t cap, int64_t *n);
 
 /* ------------------------------------------------------------------------- */
 /* Seeded synthetic networks (the generator behind the benchmark workloads)   */

@@ -134,7 +134,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.contrib); release(c.red); release(c.out); release(c.qe); release(c.qf); release(c.qn); release(c.err);
     release(c.qe_u); release(c.simpson_part); release(c.F0); release(c.F1); release(c.S0); release(c.S1);
     release(c.xmin); release(c.bar); release(c.cg_out); release(c.tr_lam); release(c.tr_e); release(c.tr_mf);
-    release(c.tr_beta); release(c.tr_safe); release(c.prof); release(c.prof_kind); release(c.cg_args);
+    release(c.tr_beta); release(c.tr_safe); release(c.prof); release(c.prof_kind); release(c.acc);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
 }
@@ -500,10 +500,25 @@ int mdy_get_profile(mdy_ctx *ctx, int32_t *kind, int64_t *ticks, int64_t cap, in
     Ctx &c = ctx->c;
     long long m = c.prof_n < c.prof_cap ? c.prof_n : c.prof_cap;
     *n = c.prof_n;
-    if (m > cap) m = cap;
+    if (cap < m) m = cap;
     if (m <= 0) return MDY_OK;
+    std::vector<long long> rec(5 * (size_t)m);
     if ((kind && hipMemcpy(kind, c.prof_kind.p, m * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) ||
-        (ticks && hipMemcpy(ticks, c.prof.p, m * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess))
+        (ticks && hipMemcpy(rec.data(), c.prof.p, rec.size() * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess))
+        return hipfail(ctx, "profile copy");
+    if (ticks)
+        for (long long i = 0; i < m; i++) ticks[i] = rec[5 * i];
+    return MDY_OK;
+}
+
+int mdy_get_profile_detail(mdy_ctx *ctx, int64_t *rec, int64_t cap, int64_t *n) {
+    if (!ctx || !n) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    long long m = c.prof_n < c.prof_cap ? c.prof_n : c.prof_cap;
+    *n = m;
+    if (!rec) return MDY_OK;
+    if (cap < m) m = cap;
+    if (m > 0 && hipMemcpy(rec, c.prof.p, 5 * m * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
         return hipfail(ctx, "profile copy");
     return MDY_OK;
 }

@@ -30,8 +30,11 @@ struct CgArgs {
     double *spart;        // 6 * nblk * kRed
     int *qe, *qf, *qn;    // 3 * kQueueCap each; qn: 3 x 4 counters
     unsigned long long *bar;  // [0] abort flag; block b arrival flag at [64 + 16 b]
+    unsigned long long *acc;  // 3 x kAccWords phase accumulators
     double *out;          // result record
-    long long *prof;      // phase profile: timestamp at each barrier exit (block 0)
+    long long *prof;      // phase profile, 5 words per phase: barrier exit (block 0), latest
+                          // arrival (all blocks), block 0's work start, end of its item
+                          // loop, and its arrival
     int *prof_kind;       // phase kind code
     long long prof_cap;
     double *tr_lam, *tr_e, *tr_mf, *tr_beta;
@@ -54,14 +57,18 @@ enum {
 
 __device__ __forceinline__ int ld_i(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
+// phase accumulator sets: 3 rotating sets of kAccWords 64-bit words
+constexpr int kAccWords = 32;
+constexpr int kAcc0 = 0, kAcc1 = 8, kAccMax = 16;
+
 // phase kinds recorded by the profile
 enum { PK_ENERGY = 1, PK_EFALLBACK, PK_EVAL, PK_FFALLBACK, PK_GATHER, PK_REDUCE, PK_RESTORE };
 
-// result of one evaluation: e[0..7] per FF, e[8] computeEnergy total
-// (+inf sentinel), e[9] plain total; the force-fallback queue it filled
+// result of one hot-loop evaluation: the computeEnergy total (+inf sentinel),
+// the extra accumulator, and the force-fallback queue it filled
 struct Eval {
-    double e[kNFF + 2];
-    double xa;     // extra exact accumulator (s.F), if requested
+    double E;
+    double xa;     // s.F, when requested
     int nqf;       // queued force fallbacks
     int qb;        // queue buffer (0..2)
 };
@@ -93,12 +100,17 @@ struct Cg {
     // Departure: every block polls all flags itself (thread t polls blocks t,
     // t+256, ...), then one agent-scope acquire per block. A watchdog on the
     // poll sets the shared abort flag; every waiting block sees it and leaves.
-    __device__ bool sync(int kind) {
+    __attribute__((always_inline)) __device__ bool sync(int kind) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         const unsigned long long epoch = (unsigned long long)(ph + 1);
         unsigned long long *flags = A.bar + 64;
         if (tid == 0) {
+            if (ph < A.prof_cap) {
+                const unsigned long long t = wall_clock64();
+                atomicMax(reinterpret_cast<unsigned long long *>(A.prof) + 5 * (size_t)ph + 1, t);
+                if (blk == 0) A.prof[5 * (size_t)ph + 4] = (long long)t;
+            }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __hip_atomic_store(flags + (size_t)blk * 16, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -126,7 +138,7 @@ struct Cg {
         if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
         if (blk == 0 && tid == 0 && ph < A.prof_cap) {
-            A.prof[ph] = wall_clock64();
+            A.prof[5 * (size_t)ph] = wall_clock64();
             A.prof_kind[ph] = kind;
         }
         ph++;
@@ -134,15 +146,116 @@ struct Cg {
         // recycle the queue counters of buffer (ph+1)%3: last written two phases ago,
         // read by every block before the barrier just passed
         if (blk == 0 && tid < 4) A.qn[4 * ((ph + 1) % 3) + tid] = 0;
+        // and the accumulator set of that phase (read by every block right
+        // after its barrier, which every block has since left)
+        if (blk == 0 && tid < kAccWords)
+            __hip_atomic_store(A.acc + (size_t)((ph + 1) % 3) * kAccWords + tid, 0ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         return !aborted;
     }
 
-    // ---- all interactions at `pos`: energy (exact total; per FF if PER_FF) and,
-    // if FORCES, the force slots. xa: optional per-thread exact accumulator
-    // reduced alongside (its total lands in ev.xa). Returns false on abort;
-    // a queue overflow sets `status`.
-    template <bool PER_FF, bool FORCES, class P>
-    __device__ bool eval(const P &pos, Eval &ev, mdy_xsum *xa) {
+    // profile marks of block 0 inside a phase: 0 = work start, 1 = end of the main loop
+    __device__ void mark(int i) const {
+        if (blk == 0 && tid == 0 && ph < A.prof_cap) A.prof[5 * (size_t)ph + 2 + i] = wall_clock64();
+    }
+
+    // ---- phase accumulators. Every exact sum a decision needs (energy, s.F,
+    // F.F, F.Fprev) is reduced per block, then added digit by digit with
+    // 64-bit integer atomics into one of three rotating sets (phase % 3):
+    // integer addition is associative, so the total is independent of the
+    // arrival order and every block reads the same bits right after the
+    // barrier. Max |F| uses an integer max on the bit pattern (values >= 0).
+    // A set is read right after its phase's barrier and zeroed by block 0 one
+    // barrier later.
+    __device__ unsigned long long *accs(int p) const { return A.acc + (size_t)(p % 3) * kAccWords; }
+    // all threads hold the block total `s` (after block_reduce_xsum)
+    __attribute__((always_inline)) __device__ void acc_push(unsigned long long *base, const mdy_xsum &s) const {
+        if (tid < 8) {
+            const long long v = tid == 0 ? s.d[0] : tid == 1 ? s.d[1] : tid == 2 ? s.d[2] : tid == 3 ? s.d[3]
+                              : tid == 4 ? s.d[4] : tid == 5 ? s.d[5] : tid == 6 ? s.d[6] : (long long)s.flags;
+            if (v != 0) {
+                if (tid < 7) atomicAdd(base + tid, (unsigned long long)v);
+                else atomicOr(base + 7, (unsigned long long)v);
+            }
+        }
+    }
+    __attribute__((always_inline)) __device__ double acc_value(const unsigned long long *base) const {
+        mdy_xsum X;
+#pragma unroll
+        for (int k = 0; k < MDY_XS_N; k++)
+            X.d[k] = (long long)__hip_atomic_load(base + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        X.flags = (int)__hip_atomic_load(base + 7, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        X.cnt = 0;
+        return mdy_xs_value(&X);
+    }
+    __attribute__((always_inline)) __device__ mdy_xsum acc_read(const unsigned long long *base) const {
+        mdy_xsum X;
+#pragma unroll
+        for (int k = 0; k < MDY_XS_N; k++)
+            X.d[k] = (long long)__hip_atomic_load(base + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        X.flags = (int)__hip_atomic_load(base + 7, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        X.cnt = 0;
+        return X;
+    }
+
+    // ---- quadrature fallback energies of the pairs queued in buffer qb, one
+    // block per pair (CylinderExclVolRepulsion.cpp:884-999, 442-481)
+    __attribute__((always_inline)) __device__ bool energy_fallback(const PosAny &pos, const Dev &d, int nqe, int qb) {
+        const int N = 400;
+        for (int q = blk; q < nqe; q += G) {
+            const int i = ld_i(d.qe + q);
+            int4 b = d.ev_bs[i];
+            SimpsonGeo g = simpson_geo(pos(b.x), pos(b.y), pos(b.z), pos(b.w));
+            mdy_xsum Y;
+            mdy_xs_zero(&Y);
+            if (tid == 0) mdy_xs_add(&Y, simpson_e_corners(g));
+            for (int node = tid; node < (N + 1) * (N + 1); node += blockDim.x)
+                simpson_e_node(g, node / (N + 1), node % (N + 1), N, Y);
+            block_reduce_xsum(Y, ldsl());
+            if (tid == 0) {
+                const double ds = 1.0 / ((double)N);
+                qeu(qb)[q] = d.ev_kr[i] * (ds * ds / 9.0) * mdy_xs_value(&Y);
+            }
+        }
+        return sync(PK_EFALLBACK);
+    }
+
+    // ---- hot loop: all interactions at `pos`, energy (exact total) and force
+    // slots. with_sd: the caller has pushed an s.F block total into this
+    // phase's second accumulator; its value lands in ev.xa. Returns false on
+    // abort; a queue overflow sets `status`.
+    __attribute__((always_inline)) __device__ bool eval(const PosAny &pos, Eval &ev, bool with_sd) {
+        const int p0 = ph, qb = p0 % 3;
+        Dev d = dv(qb);
+        mdy_xsum xs;
+        mdy_xs_zero(&xs);
+        const int total = d.n_items();
+        mark(0);
+        for (int i = gtid; i < total; i += gs) item(d, i, pos, nullptr, xs, A.contrib, false, true);
+        mark(1);
+        block_reduce_xsum(xs, ldsl());
+        acc_push(accs(p0) + kAcc0, xs);
+        if (!sync(PK_EVAL)) return false;
+        mdy_xsum X = acc_read(accs(p0) + kAcc0);
+        if (with_sd) ev.xa = acc_value(accs(p0) + kAcc1);
+        const int nqe = ld_i(d.qe_n);
+        ev.nqf = ld_i(d.qf_n);
+        ev.qb = qb;
+        if (nqe > kQueueCap || ev.nqf > kQueueCap) {
+            status = MDY_ERR_CAPACITY;
+            return true;
+        }
+        if (nqe > 0 && !energy_fallback(pos, d, nqe, qb)) return false;
+        double e[kNFF + 2];
+        energy_finish(X, nullptr, nqe, d.qe, qeu(qb), e);
+        ev.E = e[kNFF];
+        return true;
+    }
+
+    // ---- computeIndividualEnergies at x (energies only, per force field):
+    // out[0..7] per force field, out[8] their sum; returns the computeEnergy
+    // total (+inf sentinel) in E and the plain total in Eplain.
+    __attribute__((always_inline)) __device__ bool eval_ff(const PosAny &pos, double *out, double &E, double &Eplain) {
         const int p0 = ph, qb = p0 % 3;
         Dev d = dv(qb);
         double acc[kNFF];
@@ -151,59 +264,36 @@ struct Cg {
         mdy_xsum xs;
         mdy_xs_zero(&xs);
         const int total = d.n_items();
-        for (int i = gtid; i < total; i += gs) item<PER_FF, FORCES>(d, i, pos, acc, xs, A.contrib);
+        for (int i = gtid; i < total; i += gs) item(d, i, pos, acc, xs, nullptr, true, false);
         double *r = red(p0);
-        if (PER_FF) {
-            block_reduce<kNFF>(acc, lds);
-            if (tid < kNFF) r[(size_t)blk * kRed + tid] = acc[tid];
-        }
+        block_reduce<kNFF>(acc, lds);
+        if (tid < kNFF) r[(size_t)blk * kRed + tid] = acc[tid];
         block_reduce_xsum(xs, ldsl());
-        if (tid == 0) store_xsum(r, blk, kXsE, xs);
-        if (xa) {
-            block_reduce_xsum(*xa, ldsl());
-            if (tid == 0) store_xsum(r, blk, kXsA, *xa);
-        }
-        if (!sync(FORCES ? PK_EVAL : PK_ENERGY)) return false;
+        acc_push(accs(p0) + kAcc0, xs);
+        if (!sync(PK_ENERGY)) return false;
+        mdy_xsum X = acc_read(accs(p0) + kAcc0);
         const int nqe = ld_i(d.qe_n);
-        ev.nqf = FORCES ? ld_i(d.qf_n) : 0;
-        ev.qb = qb;
-        if (nqe > kQueueCap || ev.nqf > kQueueCap) {
+        if (nqe > kQueueCap) {
             status = MDY_ERR_CAPACITY;
             return true;
         }
-        if (nqe > 0) {  // quadrature fallback energies, one block per queued pair
-            const int N = 400;
-            for (int q = blk; q < nqe; q += G) {
-                const int i = ld_i(d.qe + q);
-                int4 b = d.ev_bs[i];
-                SimpsonGeo g = simpson_geo(pos(b.x), pos(b.y), pos(b.z), pos(b.w));
-                mdy_xsum X;
-                mdy_xs_zero(&X);
-                if (tid == 0) mdy_xs_add(&X, simpson_e_corners(g));
-                for (int node = tid; node < (N + 1) * (N + 1); node += blockDim.x)
-                    simpson_e_node(g, node / (N + 1), node % (N + 1), N, X);
-                block_reduce_xsum(X, ldsl());
-                if (tid == 0) {
-                    const double ds = 1.0 / ((double)N);
-                    qeu(qb)[q] = d.ev_kr[i] * (ds * ds / 9.0) * mdy_xs_value(&X);
-                }
-            }
-            if (!sync(PK_EFALLBACK)) return false;
+        if (nqe > 0 && !energy_fallback(pos, d, nqe, qb)) return false;
+        double v[kNFF], e[kNFF + 2];
+        grid_partials_reduce<kNFF>(r, G, v, lds);
+        energy_finish(X, v, nqe, d.qe, qeu(qb), e);
+        if (blk == 0 && tid == 0) {
+            for (int k = 0; k < kNFF; k++) out[k] = e[k];
+            out[kNFF] = e[kNFF + 1];
         }
-        energy_epilogue(d, r, G, nqe, d.qe, qeu(qb), lds, ev.e, PER_FF);
-        if (xa) {
-            mdy_xsum X;
-            grid_reduce_xsum(r, G, kXsA, X, ldsl());
-            ev.xa = mdy_xs_value(&X);
-        }
+        E = e[kNFF];
+        Eplain = e[kNFF + 1];
         return true;
     }
 
     // ---- quadrature forces for the pairs an accepted evaluation queued
     // (CylinderExclVolRepulsion.cpp:1001-1254, 1017-1022 retry with the first
     // cylinder moved off the plane). The slots are written by block 0.
-    template <class P>
-    __device__ bool force_fallback(const P &pos, const Eval &ev) {
+    __attribute__((always_inline)) __device__ bool force_fallback(const PosAny &pos, const Eval &ev) {
         const int *qf = A.qf + ev.qb * kQueueCap;
         for (int q = 0; q < ev.nqf; q++) {
             const int i = ld_i(qf + q);
@@ -229,12 +319,16 @@ struct Cg {
                     if (tid == 0) store_xsum(A.spart + (size_t)comp * G * kRed, blk, 0, X);
                 }
                 if (!sync(PK_FFALLBACK)) return false;
-                double T[6];
+                double T0 = 0, T1 = 0, T2 = 0, T3 = 0, T4 = 0, T5 = 0;
+#pragma clang loop unroll(disable)
                 for (int comp = 0; comp < 6; comp++) {
                     mdy_xsum X;
                     grid_reduce_xsum(A.spart + (size_t)comp * G * kRed, G, 0, X, ldsl());
-                    T[comp] = mdy_xs_value(&X);
+                    const double t = mdy_xs_value(&X);
+                    T0 = comp == 0 ? t : T0; T1 = comp == 1 ? t : T1; T2 = comp == 2 ? t : T2;
+                    T3 = comp == 3 ? t : T3; T4 = comp == 4 ? t : T4; T5 = comp == 5 ? t : T5;
                 }
+                const double T[6] = {T0, T1, T2, T3, T4, T5};
                 d3 f[4];
                 simpson_f_assemble(g, T, A.D.ev_kr[i], N, f);
                 const bool ok = finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]);
@@ -257,22 +351,24 @@ struct Cg {
 
     // ---- per-bead forces from the slots into Fout; x += lam*d when `move`.
     // Returns max|F_i|, F.F and F.Fprev (Fprev may be null), sums exact.
-    __device__ bool gather(double *Fout, const double *Fprev, bool move, const double *dvec, double lam, double &maxF,
+    __attribute__((always_inline)) __device__ bool gather(double *Fout, const double *Fprev, bool move, const double *dvec, double lam, double &maxF,
                            double &FF, double &FFp) {
         const int p1 = ph;
         double m[1] = {0.0};
         mdy_xsum sff, sfp;
         mdy_xs_zero(&sff);
         mdy_xs_zero(&sfp);
+        mark(0);
         for (int b = gtid; b < A.D.nb; b += gs) {
             d3 f = gather_bead(A.D, A.contrib, b);
             const int o = 3 * b;
             Fout[o] = f.x; Fout[o + 1] = f.y; Fout[o + 2] = f.z;
             m[0] = fmax(m[0], fmax(fabs(f.x), fmax(fabs(f.y), fabs(f.z))));
-            mdy_xs_add(&sff, f.x * f.x); mdy_xs_add(&sff, f.y * f.y); mdy_xs_add(&sff, f.z * f.z);
-            if (Fprev) {
-                mdy_xs_add(&sfp, f.x * Fprev[o]); mdy_xs_add(&sfp, f.y * Fprev[o + 1]);
-                mdy_xs_add(&sfp, f.z * Fprev[o + 2]);
+#pragma clang loop unroll(disable)
+            for (int k = 0; k < 3; k++) {
+                const double fk = k == 0 ? f.x : k == 1 ? f.y : f.z;
+                mdy_xs_add(&sff, fk * fk);
+                if (Fprev) mdy_xs_add(&sfp, fk * Fprev[o + k]);
             }
             if (move) {
                 A.x[o] = A.x[o] + lam * dvec[o];
@@ -280,250 +376,308 @@ struct Cg {
                 A.x[o + 2] = A.x[o + 2] + lam * dvec[o + 2];
             }
         }
+        mark(1);
         block_reduce<1>(m, lds, 0x1u);
-        block_reduce_xsum(sff, ldsl());
-        double *r = red(p1);
-        if (tid == 0) { r[(size_t)blk * kRed] = m[0]; store_xsum(r, blk, 8, sff); }
-        if (Fprev) {
-            block_reduce_xsum(sfp, ldsl());
-            if (tid == 0) store_xsum(r, blk, 16, sfp);
+        unsigned long long *a = accs(p1);
+        if (tid == 0) atomicMax(a + kAccMax, (unsigned long long)__double_as_longlong(m[0]));
+#pragma clang loop unroll(disable)
+        for (int k = 0; k < 2; k++) {   // F.F, then F.Fprev (one inlined copy of the reduction)
+            mdy_xsum &t = k == 0 ? sff : sfp;
+            block_reduce_xsum(t, ldsl());
+            acc_push(a + (k == 0 ? kAcc0 : kAcc1), t);
         }
         if (!sync(PK_GATHER)) return false;
-        grid_partials_reduce<1>(r, G, m, lds, 0x1u);
-        maxF = m[0];
-        mdy_xsum X;
-        grid_reduce_xsum(r, G, 8, X, ldsl());
-        FF = mdy_xs_value(&X);
-        FFp = 0.0;
-        if (Fprev) {
-            grid_reduce_xsum(r, G, 16, X, ldsl());
-            FFp = mdy_xs_value(&X);
-        }
+        maxF = __longlong_as_double((long long)__hip_atomic_load(a + kAccMax, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT));
+        double v2[2];
+#pragma clang loop unroll(disable)
+        for (int k = 0; k < 2; k++) v2[k] = acc_value(a + (k == 0 ? kAcc0 : kAcc1));
+        FF = v2[0];
+        FFp = Fprev ? v2[1] : 0.0;
         return true;
     }
 };
 
-__global__ void __launch_bounds__(kBlock) k_cg(const CgArgs *__restrict__ Ap) {
+// s = F + beta * d for every bead into Sn, and this block's share of s.F,
+// pushed into the current phase's second accumulator
+__device__ __attribute__((always_inline)) void spec_direction(Cg &K, const double *Fn, const double *dprev,
+                                                              double beta, double *Sn) {
+    mdy_xsum sd;
+    mdy_xs_zero(&sd);
+    for (int b = K.gtid; b < K.A.D.nb; b += K.gs) {
+        const int o = 3 * b;
+        for (int k = 0; k < 3; k++) {
+            double s = Fn[o + k] + beta * dprev[o + k];
+            Sn[o + k] = s;
+            mdy_xs_add(&sd, s * Fn[o + k]);
+        }
+    }
+    block_reduce_xsum(sd, K.ldsl());
+    K.acc_push(K.accs(K.ph) + kAcc1, sd);
+}
+
+// The minimizer as a state machine: every loop turn performs at most one kind
+// of device work (each with a single call site, so each is inlined once) and
+// then advances the CGMethod::minimize control flow
+// (CGMethod.hpp:826-1743, backtrackingLineSearch CGMethod.cpp:43-360).
+// The first trial of a line search that follows an accepted step is taken
+// along s = F + beta*d while s itself is formed in the same phase; if s.F <= 0
+// (resetCGParams) that trial is discarded and redone along F.
+enum Op { OP_NONE, OP_EVAL_FF, OP_EVAL, OP_GATHER, OP_MINSTEP, OP_SDONLY, OP_RESTORE };
+enum Stage {
+    ST_INIT_FF, ST_INIT_F, ST_INIT_G, ST_ITER, ST_TRIAL, ST_SD, ST_LS_DONE, ST_ACCEPT, ST_REC_STEP, ST_REC_EVAL,
+    ST_REC_GATHER, ST_FINAL, ST_FINAL_DONE, ST_FINISH
+};
+
+// The arguments come by value: pointers read from the kernel-argument segment
+// are known to be global, so every access is a global (not flat) access.
+__global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
     __shared__ double lds[17 * 16];
-    const CgArgs &A = *Ap;
+    const CgArgs &A = Av;
     Cg K(A, lds);
     double *R = A.out;
     const bool leader = (K.blk == 0 && K.tid == 0);
+#define FB(i) ((i) ? A.F1 : A.F0)
+#define SB(i) ((i) ? A.S1 : A.S0)
     unsigned flags = 0;
     int nEC = 0, nFC = 0, nIter = 0, nEE = 0, nSM = 0, nRec = 0;
     long long trn = 0;
     double gradInf = INFINITY, relE = INFINITY;
-    Eval ev;
     bool after = true;   // compute energiesAfter / output force
     int cur = 0, sidx = 0;
-#define FB(i) ((i) ? A.F1 : A.F0)
-#define SB(i) ((i) ? A.S1 : A.S0)
     const double *dvec = nullptr;
     double maxF = 0, FF = 0, FFp = 0, FFcur = 0;
+    double E0 = 0, prevEnergy = 0, lastFDotF = 0, slope = 0;
+    int safeMode = 0;
+    double minimumE = INFINITY;
+    bool haveMin = false;
+    double lam = 0, newE = 0, armijo = 0, reducedSlope = 0, sdF = 0, stepRec = 0, beta = 0;
+    bool pendingSpec = false;        // an accepted step's s is not formed yet
+    const double *dprev = nullptr;   // the direction s extends
+    double *Snew = nullptr;          // where s goes
+    Eval ev{0.0, 0.0, 0, 0};
+    double Eff = 0, Effp = 0;
+    // the device work of the next turn and its operands
+    int op = OP_EVAL_FF, stage = ST_INIT_FF;
+    PosAny P = PosAny::at(A.x);
+    double *ffout = R + R_EB;
+    double *gOut = nullptr;
+    const double *gPrev = nullptr;
+    bool gMove = false;
+    auto callLimit = [&]() {
+        return (A.energy_call_limit != 0 && nEC > A.energy_call_limit) ||
+               (A.force_call_limit != 0 && nFC > A.force_call_limit);
+    };
+    auto converged = [&]() { return gradInf < A.grad_tol || relE < A.energy_rel_tol; };
+    auto trace = [&](double l, double e, double b) {
+        if (leader && trn < A.tr_cap) {
+            A.tr_lam[trn] = l; A.tr_e[trn] = e; A.tr_mf[trn] = gradInf; A.tr_beta[trn] = b;
+            A.tr_safe[trn] = safeMode;
+        }
+        trn++;
+    };
 
-    // energiesBefore = computeIndividualEnergies(x); computeForces(x) from the same pass
-    if (!K.eval<true, true>(PosX{A.x}, ev, nullptr)) goto aborted;
-    if (K.status) { after = false; goto finish; }
-    nEC++; nEE++;
-    if (leader) {
-        for (int k = 0; k < kNFF; k++) R[R_EB + k] = ev.e[k];
-        R[R_EB + kNFF] = ev.e[kNFF + 1];
-    }
-    {
-        double E0 = ev.e[kNFF];           // computeEnergy at the same x (sentinel form)
-        double prevEnergy = ev.e[kNFF + 1];
-        if (ev.nqf && !K.force_fallback(PosX{A.x}, ev)) goto aborted;
-        if (K.status) { after = false; goto finish; }
-        if (!K.gather(FB(cur), nullptr, false, nullptr, 0.0, maxF, FF, FFp)) goto aborted;
-        nFC++;
-        gradInf = maxF;
-        if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; goto finish; }
-        FFcur = FF;
-        // resetCGParams: s = F, lastFDotF = F.F, Fprev = F
-        dvec = FB(cur);
-        double lastFDotF = FF, slope = FF;
-        int safeMode = 0;
-        double minimumE = INFINITY;
-        bool haveMin = false;
-        bool specValid = false;
-        double specLam = 0;
-        Eval spec;
-        auto callLimit = [&]() {
-            return (A.energy_call_limit != 0 && nEC > A.energy_call_limit) ||
-                   (A.force_call_limit != 0 && nFC > A.force_call_limit);
-        };
-        auto converged = [&]() { return gradInf < A.grad_tol || relE < A.energy_rel_tol; };
+    for (;;) {
+        // ---- device work
+        bool ok = true;
+        switch (op) {
+        case OP_EVAL_FF:
+            ok = K.eval_ff(P, ffout, Eff, Effp);
+            break;
+        case OP_EVAL:
+        case OP_SDONLY: {
+            const int p1 = K.ph;
+            if (P.mode == PosAny::SPEC) spec_direction(K, P.F, P.d, P.beta, Snew);
+            if (op == OP_EVAL) {
+                ok = K.eval(P, ev, P.mode == PosAny::SPEC);
+                nEE++;
+            } else {   // s.F only (max|F| = 0: no trial)
+                ok = K.sync(PK_REDUCE);
+                if (ok) sdF = K.acc_value(K.accs(p1) + kAcc1);
+            }
+            break;
+        }
+        case OP_GATHER:
+            if (ev.nqf) ok = K.force_fallback(P, ev);
+            if (ok && !K.status) ok = K.gather(gOut, gPrev, gMove, P.d, P.lam, maxF, FF, FFp);
+            break;
+        case OP_MINSTEP: {
+            // MController::moveAlongDirectionLimitingBeadMove_: step = min maxDist/|d_i|
+            double v[1] = {INFINITY};
+            for (int o = K.gtid; o < A.D.ndof; o += K.gs) v[0] = smin(v[0], A.max_dist / fabs(dvec[o]));
+            block_reduce<1>(v, lds, 0u, 0x1u);
+            const int p1 = K.ph;
+            if (K.tid == 0) K.red(p1)[(size_t)K.blk * kRed] = v[0];
+            ok = K.sync(PK_REDUCE);
+            if (ok) {
+                grid_partials_reduce<1>(K.red(p1), K.G, v, lds, 0u, 0x1u);
+                stepRec = v[0];
+            }
+            break;
+        }
+        case OP_RESTORE:
+            for (int o = K.gtid; o < A.D.ndof; o += K.gs) A.x[o] = A.xmin[o];
+            ok = K.sync(PK_RESTORE);
+            break;
+        default:
+            break;
+        }
+        if (!ok) goto aborted;
+        op = OP_NONE;
+        if (K.status && stage != ST_FINISH) { after = false; stage = ST_FINISH; }
 
-        while (!callLimit() && !converged()) {
+        // ---- control
+        switch (stage) {
+        case ST_INIT_FF:   // energiesBefore = computeIndividualEnergies(x); E(x) for the first line search
+            nEC++; nEE++;
+            E0 = Eff; prevEnergy = Effp;
+            op = OP_EVAL; P = PosAny::at(A.x); stage = ST_INIT_F;
+            break;
+        case ST_INIT_F:    // forces at x
+            op = OP_GATHER; gOut = FB(cur); gPrev = nullptr; gMove = false; stage = ST_INIT_G;
+            break;
+        case ST_INIT_G:
+            nFC++;
+            gradInf = maxF;
+            if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
+            FFcur = FF;
+            dvec = FB(cur); lastFDotF = FF; slope = FF;   // resetCGParams
+            stage = ST_ITER;
+            break;
+        case ST_ITER:
+            if (callLimit() || converged()) {
+                if (callLimit() && haveMin) op = OP_RESTORE;
+                stage = ST_FINAL;
+                break;
+            }
             nIter++;
-            const double armijo = safeMode ? 0.0 : 0.4;
-            double lam = 0.0, newE = 0.0;
+            armijo = safeMode ? 0.0 : 0.4;
+            lam = 0.0; newE = 0.0;
             if (maxF != 0.0) {
                 lam = smin(A.lambda_max, A.max_dist / maxF);
                 nEC++;   // currentEnergy = computeEnergy(x): E0, known bitwise from the accepting trial
-                const double reducedSlope = armijo * slope;
-                double Et = 0.0;
-                for (;;) {
-                    if (specValid && specLam == lam) {
-                        ev = spec;
-                    } else {
-                        if (!K.eval<false, true>(PosLine{A.x, dvec, lam}, ev, nullptr)) goto aborted;
-                        nEE++;
-                        if (K.status) { after = false; goto finish; }
-                    }
-                    Et = ev.e[kNFF];
-                    specValid = false;
-                    nEC++;
-                    const double ideal = -reducedSlope * lam;
-                    bool done = false;
-                    if (lam < 0 || ideal > 0) { done = true; lam = 0.0; }
-                    const double change = Et - E0;
-                    if (change <= ideal) done = true;
-                    else lam *= 0.5;
-                    if (lam <= 0.0 || lam <= 1e-8) { done = true; lam = 0.0; Et = E0; }
-                    if (done) break;
-                }
-                newE = Et;
+                op = OP_EVAL;
+                P = pendingSpec ? PosAny::spec(A.x, FB(cur), dprev, beta, lam) : PosAny::line(A.x, dvec, lam);
+                reducedSlope = armijo * slope;
+                stage = ST_TRIAL;
+            } else if (pendingSpec) {
+                op = OP_SDONLY; P = PosAny::spec(A.x, FB(cur), dprev, beta, 0.0); stage = ST_SD;
+            } else {
+                stage = ST_LS_DONE;
             }
-            specValid = false;
+            break;
+        case ST_SD:        // s.F without a trial (max|F| = 0)
+            pendingSpec = false;
+            if (sdF <= 0) { dvec = FB(cur); lastFDotF = FFcur; slope = FFcur; }
+            else { dvec = Snew; slope = sdF; }
+            stage = ST_LS_DONE;
+            break;
+        case ST_TRIAL: {   // one backtracking step
+            if (P.mode == PosAny::SPEC) {
+                pendingSpec = false;
+                sdF = ev.xa;
+                if (sdF <= 0) {   // resetCGParams: redo this trial along F
+                    dvec = FB(cur); lastFDotF = FFcur; slope = FFcur;
+                    reducedSlope = armijo * slope;
+                    op = OP_EVAL; P = PosAny::line(A.x, dvec, lam);
+                    break;
+                }
+                dvec = Snew; slope = sdF;
+                reducedSlope = armijo * slope;
+            }
+            double Et = ev.E;
+            nEC++;
+            const double ideal = -reducedSlope * lam;
+            bool done = false;
+            if (lam < 0 || ideal > 0) { done = true; lam = 0.0; }
+            const double change = Et - E0;
+            if (change <= ideal) done = true;
+            else lam *= 0.5;
+            if (lam <= 0.0 || lam <= 1e-8) { done = true; lam = 0.0; Et = E0; }
+            if (!done) { op = OP_EVAL; P = PosAny::line(A.x, dvec, lam); break; }
+            newE = Et;
+            stage = ST_LS_DONE;
+            break;
+        }
+        case ST_LS_DONE:
             if (lam == 0.0) {
-                if (A.try_recover) {
-                    if (safeMode) {
-                        // MController::moveAlongDirectionLimitingBeadMove_: step = min maxDist/|d_i|
-                        double v[1] = {INFINITY};
-                        for (int o = K.gtid; o < A.D.ndof; o += K.gs) v[0] = smin(v[0], A.max_dist / fabs(dvec[o]));
-                        block_reduce<1>(v, lds, 0u, 0x1u);
-                        const int p1 = K.ph;
-                        if (K.tid == 0) K.red(p1)[(size_t)K.blk * kRed] = v[0];
-                        if (!K.sync(PK_REDUCE)) goto aborted;
-                        double st[1];
-                        grid_partials_reduce<1>(K.red(p1), K.G, st, lds, 0u, 0x1u);
-                        if (st[0] == INFINITY) { K.status = MDY_ERR_RECOVERY; after = false; goto finish; }
-                        const PosLine pl{A.x, dvec, st[0]};
-                        if (!K.eval<false, true>(pl, ev, nullptr)) goto aborted;
-                        nEE++;
-                        if (K.status) { after = false; goto finish; }
-                        if (ev.nqf && !K.force_fallback(pl, ev)) goto aborted;
-                        if (K.status) { after = false; goto finish; }
-                        double *Fn = FB(1 - cur);
-                        if (!K.gather(Fn, FB(cur), true, dvec, st[0], maxF, FF, FFp)) goto aborted;
-                        nFC++; nRec++;
-                        gradInf = maxF;
-                        if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; goto finish; }
-                        cur = 1 - cur;
-                        FFcur = FF;
-                        dvec = FB(cur); lastFDotF = FF; slope = FF;
-                        safeMode = 0;
-                        E0 = ev.e[kNFF];   // computeEnergy at the moved coordinates
-                    } else {
-                        dvec = FB(cur); lastFDotF = FFcur; slope = FFcur;
-                        safeMode = 1;
-                        nSM++;
-                    }
-                    if (leader && trn < A.tr_cap) {
-                        A.tr_lam[trn] = 0.0; A.tr_e[trn] = newE; A.tr_mf[trn] = gradInf; A.tr_beta[trn] = 0.0;
-                        A.tr_safe[trn] = safeMode;
-                    }
-                    trn++;
-                } else {
+                if (!A.try_recover) {
                     flags |= MDY_ERROR_LINESEARCH;
-                    if (leader && trn < A.tr_cap) {
-                        A.tr_lam[trn] = 0.0; A.tr_e[trn] = newE; A.tr_mf[trn] = gradInf; A.tr_beta[trn] = 0.0;
-                        A.tr_safe[trn] = safeMode;
-                    }
-                    trn++;
+                    trace(0.0, newE, 0.0);
                     after = false;   // the reference returns before energiesAfter / ptrOutForce
-                    goto finish;
-                }
-                continue;
-            }
-            // accepted step: the force slots hold the forces at x + lam*d
-            {
-                const PosLine pl{A.x, dvec, lam};
-                if (ev.nqf && !K.force_fallback(pl, ev)) goto aborted;
-                if (K.status) { after = false; goto finish; }
-                double *Fn = FB(1 - cur);
-                const double *Fp = FB(cur);
-                if (!K.gather(Fn, Fp, true, dvec, lam, maxF, FF, FFp)) goto aborted;
-                nFC++;
-                gradInf = maxF;
-                if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; goto finish; }
-                E0 = newE;
-                relE = fabs(newE - prevEnergy) / fabs(prevEnergy);
-                prevEnergy = newE;
-                const double newGrad = FF, prevGrad = FFp;
-                double beta = 0.0;
-                if (A.method == MDY_CG_FLETCHER_RIEVES) beta = newGrad / lastFDotF;
-                else if (A.method == MDY_CG_POLAK_RIBIERE) beta = smax(0.0, (newGrad - prevGrad) / lastFDotF);
-                // s = F + beta*s, its slope, and (speculatively) the first trial of the next line search
-                const bool cont = !(callLimit() || converged());
-                const bool more = cont && maxF != 0.0;
-                const double lam1 = more ? smin(A.lambda_max, A.max_dist / maxF) : 0.0;
-                double *Sn = SB(1 - sidx);
-                double sdF = 0.0;
-                if (cont) {
-                    mdy_xsum sd;
-                    mdy_xs_zero(&sd);
-                    for (int b = K.gtid; b < A.D.nb; b += K.gs) {
-                        const int o = 3 * b;
-                        for (int k = 0; k < 3; k++) {
-                            double s = Fn[o + k] + beta * dvec[o + k];
-                            Sn[o + k] = s;
-                            mdy_xs_add(&sd, s * Fn[o + k]);
-                        }
-                    }
-                    if (more) {
-                        if (!K.eval<false, true>(PosSpec{A.x, Fn, dvec, beta, lam1}, spec, &sd)) goto aborted;
-                        nEE++;
-                        if (K.status) { after = false; goto finish; }
-                        specValid = true; specLam = lam1;
-                        sdF = spec.xa;
-                    } else {
-                        block_reduce_xsum(sd, K.ldsl());
-                        const int p1 = K.ph;
-                        if (K.tid == 0) store_xsum(K.red(p1), K.blk, kXsA, sd);
-                        if (!K.sync(PK_REDUCE)) goto aborted;
-                        mdy_xsum X;
-                        grid_reduce_xsum(K.red(p1), K.G, kXsA, X, K.ldsl());
-                        sdF = mdy_xs_value(&X);
-                    }
-                }
-                cur = 1 - cur;
-                sidx = 1 - sidx;
-                FFcur = newGrad;
-                if (sdF <= 0) {  // resetCGParams
-                    dvec = FB(cur); lastFDotF = newGrad; slope = newGrad;
-                    specValid = false;
+                    stage = ST_FINISH;
+                } else if (safeMode) {
+                    op = OP_MINSTEP; stage = ST_REC_STEP;
                 } else {
-                    dvec = Sn; slope = sdF;
+                    dvec = FB(cur); lastFDotF = FFcur; slope = FFcur;
+                    safeMode = 1;
+                    nSM++;
+                    trace(0.0, newE, 0.0);
+                    stage = ST_ITER;
                 }
-                // forceBelowRelaxedTolerance(10) && numIter > forceCallLimit/2 -> keep lowest-energy coordinates
-                if (maxF <= 10 * A.grad_tol && nIter > A.force_call_limit / 2 && newE <= minimumE) {
-                    minimumE = newE;
-                    haveMin = true;
-                    for (int o = K.gtid; o < A.D.ndof; o += K.gs) A.xmin[o] = A.x[o];
-                }
-                if (A.method != MDY_CG_STEEPEST) lastFDotF = newGrad;
-                if (leader && trn < A.tr_cap) {
-                    A.tr_lam[trn] = lam; A.tr_e[trn] = newE; A.tr_mf[trn] = gradInf; A.tr_beta[trn] = beta;
-                    A.tr_safe[trn] = safeMode;
-                }
-                trn++;
+            } else {   // accepted: the force slots hold the forces at x + lam*d
+                op = OP_GATHER; P = PosAny::line(A.x, dvec, lam);
+                gOut = FB(1 - cur); gPrev = FB(cur); gMove = true;
+                stage = ST_ACCEPT;
             }
+            break;
+        case ST_REC_STEP:
+            if (stepRec == INFINITY) { K.status = MDY_ERR_RECOVERY; after = false; stage = ST_FINISH; break; }
+            op = OP_EVAL; P = PosAny::line(A.x, dvec, stepRec); stage = ST_REC_EVAL;
+            break;
+        case ST_REC_EVAL:
+            op = OP_GATHER; gOut = FB(1 - cur); gPrev = FB(cur); gMove = true; stage = ST_REC_GATHER;
+            break;
+        case ST_REC_GATHER:
+            nFC++; nRec++;
+            gradInf = maxF;
+            if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
+            cur = 1 - cur;
+            FFcur = FF;
+            dvec = FB(cur); lastFDotF = FF; slope = FF;
+            safeMode = 0;
+            E0 = ev.E;   // computeEnergy at the moved coordinates
+            trace(0.0, newE, 0.0);
+            stage = ST_ITER;
+            break;
+        case ST_ACCEPT: {
+            nFC++;
+            gradInf = maxF;
+            if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
+            E0 = newE;
+            relE = fabs(newE - prevEnergy) / fabs(prevEnergy);
+            prevEnergy = newE;
+            const double newGrad = FF, prevGrad = FFp;
+            beta = 0.0;
+            if (A.method == MDY_CG_FLETCHER_RIEVES) beta = newGrad / lastFDotF;
+            else if (A.method == MDY_CG_POLAK_RIBIERE) beta = smax(0.0, (newGrad - prevGrad) / lastFDotF);
+            cur = 1 - cur;
+            FFcur = newGrad;
+            dprev = dvec; Snew = SB(1 - sidx); sidx = 1 - sidx;
+            pendingSpec = true;   // s = F + beta*d is formed with the next trial
+            // forceBelowRelaxedTolerance(10) && numIter > forceCallLimit/2 -> keep lowest-energy coordinates
+            if (maxF <= 10 * A.grad_tol && nIter > A.force_call_limit / 2 && newE <= minimumE) {
+                minimumE = newE;
+                haveMin = true;
+                for (int o = K.gtid; o < A.D.ndof; o += K.gs) A.xmin[o] = A.x[o];
+            }
+            if (A.method != MDY_CG_STEEPEST) lastFDotF = newGrad;
+            trace(lam, newE, beta);
+            stage = ST_ITER;
+            break;
         }
-        if (callLimit() && haveMin) {
-            for (int o = K.gtid; o < A.D.ndof; o += K.gs) A.x[o] = A.xmin[o];
-            if (!K.sync(PK_RESTORE)) goto aborted;
+        case ST_FINAL:
+            if (after) { op = OP_EVAL_FF; P = PosAny::at(A.x); ffout = R + R_EA; stage = ST_FINAL_DONE; }
+            else stage = ST_FINISH;
+            break;
+        case ST_FINAL_DONE:
+            nEC++; nEE++;
+            stage = ST_FINISH;
+            break;
+        default:
+            break;
         }
-    }
-finish:
-    if (after) {
-        if (!K.eval<true, false>(PosX{A.x}, ev, nullptr)) goto aborted;
-        nEC++; nEE++;
-        if (leader) {
-            for (int k = 0; k < kNFF; k++) R[R_EA + k] = ev.e[k];
-            R[R_EA + kNFF] = ev.e[kNFF + 1];
-        }
+        if (stage == ST_FINISH && op == OP_NONE) break;
     }
     if (leader) {
         R[R_FLAGS] = flags; R[R_NEC] = nEC; R[R_NFC] = nFC; R[R_NIT] = nIter; R[R_GIN] = gradInf; R[R_ERC] = relE;
@@ -544,9 +698,8 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     MDY_TRY(ensure(c.F0, ndof)); MDY_TRY(ensure(c.F1, ndof));
     MDY_TRY(ensure(c.S0, ndof)); MDY_TRY(ensure(c.S1, ndof));
     MDY_TRY(ensure(c.xmin, ndof));
-    MDY_TRY(ensure(c.prof, c.prof_cap + 1));
+    MDY_TRY(ensure(c.prof, 5 * (size_t)c.prof_cap + 5));
     MDY_TRY(ensure(c.prof_kind, c.prof_cap + 1));
-    MDY_TRY(ensure(c.cg_args, sizeof(CgArgs) / sizeof(double) + 1));
     MDY_TRY(ensure(c.cg_out, R_SIZE));
     MDY_TRY(ensure(c.qe, 3 * kQueueCap)); MDY_TRY(ensure(c.qf, 3 * kQueueCap));
     MDY_TRY(ensure(c.qn, 16));
@@ -562,16 +715,19 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     const int G = per_cu * c.num_cu;
     c.coop_blocks = G;
     MDY_TRY(ensure(c.bar, 64 + 16 * (size_t)G));
+    MDY_TRY(ensure(c.acc, 3 * (size_t)kAccWords));
     MDY_TRY(ensure(c.red, (size_t)3 * G * kRed));
     MDY_TRY(ensure(c.simpson_part, (size_t)6 * G * kRed));
     MDY_HIP(hipMemsetAsync(c.bar.p, 0, (64 + 16 * (size_t)G) * sizeof(unsigned long long), st));
     MDY_HIP(hipMemsetAsync(c.qn.p, 0, 16 * sizeof(int), st));
+    MDY_HIP(hipMemsetAsync(c.acc.p, 0, 3 * kAccWords * sizeof(unsigned long long), st));
     MDY_HIP(hipMemsetAsync(c.err.p, 0, sizeof(int), st));
+    if (c.prof_cap > 0) MDY_HIP(hipMemsetAsync(c.prof.p, 0, (5 * (size_t)c.prof_cap + 5) * sizeof(long long), st));
     CgArgs A;
     A.D = c.dev();
     A.x = d_x; A.F0 = c.F0.p; A.F1 = c.F1.p; A.S0 = c.S0.p; A.S1 = c.S1.p; A.xmin = c.xmin.p;
     A.contrib = c.contrib.p; A.red = c.red.p; A.qe_u = c.qe_u.p; A.spart = c.simpson_part.p;
-    A.qe = c.qe.p; A.qf = c.qf.p; A.qn = c.qn.p; A.bar = c.bar.p; A.out = c.cg_out.p;
+    A.qe = c.qe.p; A.qf = c.qf.p; A.qn = c.qn.p; A.bar = c.bar.p; A.acc = c.acc.p; A.out = c.cg_out.p;
     A.tr_lam = c.tr_lam.p; A.tr_e = c.tr_e.p; A.tr_mf = c.tr_mf.p; A.tr_beta = c.tr_beta.p; A.tr_safe = c.tr_safe.p;
     A.tr_cap = c.trace_cap;
     A.prof = c.prof.p; A.prof_kind = c.prof_kind.p; A.prof_cap = c.prof_cap;
@@ -583,10 +739,8 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     hipEvent_t e0, e1;
     MDY_HIP(hipEventCreate(&e0));
     MDY_HIP(hipEventCreate(&e1));
-    MDY_HIP(hipMemcpyAsync(c.cg_args.p, &A, sizeof(CgArgs), hipMemcpyHostToDevice, st));
     MDY_HIP(hipEventRecord(e0, st));
-    const CgArgs *Ad = reinterpret_cast<const CgArgs *>(c.cg_args.p);
-    void *args[] = {&Ad};
+    void *args[] = {&A};
     hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_cg), dim3(G), dim3(kBlock), args, 0, st);
     if (le != hipSuccess) {
         (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);

@@ -17,7 +17,7 @@ __global__ void __launch_bounds__(kBlock) k_energy(Dev D, P pos, double *red) {
     mdy_xs_zero(&xs);
     const int total = D.n_items(), gs = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs)
-        item<true, false>(D, i, pos, acc, xs, nullptr);
+        item(D, i, pos, acc, xs, nullptr, true, false);
     block_reduce<kNFF>(acc, lds);
     block_reduce_xsum(xs, reinterpret_cast<long long *>(lds));
     if (threadIdx.x < kNFF) red[(size_t)blockIdx.x * kRed + threadIdx.x] = acc[threadIdx.x];
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, P pos, double *contrib)
     mdy_xsum xs;
     mdy_xs_zero(&xs);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs)
-        item<false, true>(D, i, pos, acc, xs, contrib);
+        item(D, i, pos, acc, xs, contrib, false, true);
 }
 
 // Quadrature force (forceN, 10^4 x 10^4 intervals) of one queued pair,

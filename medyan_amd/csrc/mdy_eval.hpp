@@ -33,6 +33,30 @@ struct PosSpec {  // x + lam * (F + beta * d): trial along the not-yet-stored ne
     __device__ __forceinline__ d3 operator()(int o) const { return d3{c(o), c(o + 1), c(o + 2)}; }
 };
 
+// one functor for all three (uniform runtime mode): a single inlined copy of
+// the interaction code serves every evaluation of the persistent kernel
+struct PosAny {
+    enum { X = 0, LINE = 1, SPEC = 2 };
+    const double *__restrict__ x;
+    const double *__restrict__ d;
+    const double *__restrict__ F;
+    double lam, beta;
+    int mode;
+    __device__ __forceinline__ double c(int o) const {
+        if (mode == X) return x[o];
+        const double s = (mode == SPEC) ? F[o] + beta * d[o] : d[o];
+        return x[o] + lam * s;
+    }
+    __device__ __forceinline__ d3 operator()(int o) const { return d3{c(o), c(o + 1), c(o + 2)}; }
+    static __device__ PosAny at(const double *x) { return PosAny{x, nullptr, nullptr, 0.0, 0.0, X}; }
+    static __device__ PosAny line(const double *x, const double *d, double lam) {
+        return PosAny{x, d, nullptr, lam, 0.0, LINE};
+    }
+    static __device__ PosAny spec(const double *x, const double *F, const double *d, double beta, double lam) {
+        return PosAny{x, d, F, lam, beta, SPEC};
+    }
+};
+
 __device__ __forceinline__ void set_err(int *err, int code) { atomicCAS(err, 0, code); }
 
 __device__ __forceinline__ void enqueue(int *q, int *qn, int v, int *err) {
@@ -47,31 +71,41 @@ __device__ __forceinline__ void put3(double *__restrict__ c, int slot, d3 v) {
 
 // ---- one interaction: energy (into the exact total and, if PER_FF, into the
 // per-force-field double sums) and, if FORCES, its per-bead force slots. The
-// positions are loaded once for both.
-template <bool PER_FF, bool FORCES, class P>
+// positions are loaded once for both. Written with one accumulation site and
+// one copy of the excluded-volume invariants and force, so the inlined code
+// stays small (it runs from the instruction cache of every CU).
+template <class P>
 __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double acc[kNFF], mdy_xsum &xs,
-                                     double *__restrict__ contrib) {
+                                     double *__restrict__ contrib, const bool PER_FF, const bool FORCES) {
+    double U = 0.0;
+    int ff = 0;
+    bool add = true;
     if (i < D.n_ev) {
         int4 b = D.ev_bs[i];
         const double kr = D.ev_kr[i];
-        d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+        const d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+        ff = MDY_FF_EV;
         EvInv v;
-        ev_invariants(c1, c2, c3, c4, v);
-        const double U = ev_energy_inv(v, kr);
-        if (ev_energy_bad(U)) enqueue(D.qe, D.qe_n, i, D.err);
-        else {
-            mdy_xs_add(&xs, U);
-            if (PER_FF) acc[MDY_FF_EV] += U;
+        d3 c2e = c2;
+        // pass 0: the pair as given (energy, and force unless coplanar);
+        // pass 1: the coplanar perturbation of the reference force path,
+        // c2 moved off the plane (CylinderExclVolRepulsion.cpp:610-626)
+#pragma clang loop unroll(disable)
+        for (int pass = 0; pass < 2; pass++) {
+            ev_invariants(c1, c2e, c3, c4, v);
+            if (pass == 0) {
+                U = ev_energy_inv(v, kr);
+                if (!FORCES || !(D.coplanar_mode == MDY_COPLANAR_REFERENCE && are_in_plane(c1, c2, c3, c4))) break;
+                c2e = moved_off_plane(c1, c2, c3, c2, 0.01);
+            }
+        }
+        if (ev_energy_bad(U)) {
+            enqueue(D.qe, D.qe_n, i, D.err);
+            add = false;
         }
         if (FORCES) {
             d3 f[4];
-            // coplanar perturbation of the reference force path (CylinderExclVolRepulsion.cpp:610-626)
-            if (D.coplanar_mode == MDY_COPLANAR_REFERENCE && are_in_plane(c1, c2, c3, c4)) {
-                d3 c2u = moved_off_plane(c1, c2, c3, c2, 0.01);
-                ev_force(c1, c2u, c3, c4, kr, f);
-            } else {
-                ev_force_inv(v, c1, c2, c3, c4, kr, f);
-            }
+            ev_force_inv(v, c1, c2e, c3, c4, kr, f);
             if (!(finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]))) {
                 enqueue(D.qf, D.qf_n, i, D.err);
             } else {
@@ -79,110 +113,117 @@ __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double a
                 put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
             }
         }
-        return;
-    }
-    i -= D.n_ev;
-    if (i < D.n_fs) {
+    } else if ((i -= D.n_ev) < D.n_fs) {
         int2 b = D.fs_bs[i];
         d3 c1 = pos(b.x), c2 = pos(b.y);
-        const double U = fs_energy(c1, c2, D.fs_k[i], D.fs_l0[i]);
-        mdy_xs_add(&xs, U);
-        if (PER_FF) acc[MDY_FF_FS] += U;
+        ff = MDY_FF_FS;
+        U = fs_energy(c1, c2, D.fs_k[i], D.fs_l0[i]);
         if (FORCES) {
             d3 f1, f2;
             fs_force(c1, c2, D.fs_k[i], D.fs_l0[i], f1, f2);
             int s = D.slot_fs + 2 * i;
             put3(contrib, s, f1); put3(contrib, s + 1, f2);
         }
-        return;
-    }
-    i -= D.n_fs;
-    if (i < D.n_fb) {
+    } else if ((i -= D.n_fs) < D.n_fb) {
         const int *b = D.fb_bs + 3 * i;
         d3 c1 = pos(b[0]), c2 = pos(b[1]), c3 = pos(b[2]);
         const double k = D.fb_k[i], th = D.fb_th[i], ct = D.fb_ct[i], st = D.fb_st[i];
-        const double U = fb_energy(c1, c2, c3, k, th, ct, st);
-        mdy_xs_add(&xs, U);
-        if (PER_FF) acc[MDY_FF_FB] += U;
+        ff = MDY_FF_FB;
+        U = fb_energy(c1, c2, c3, k, th, ct, st);
         if (FORCES) {
             d3 f1, f2, f3;
             fb_force(c1, c2, c3, k, th, ct, st, f1, f2, f3);
             int s = D.slot_fb + 3 * i;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
         }
-        return;
-    }
-    i -= D.n_fb;
-    if (i < D.n_lm) {
+    } else if ((i -= D.n_fb) < D.n_lm) {
         int4 b = D.lm_bs[i];
         d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
         const double k = D.lm_k[i], l0 = D.lm_l0[i], p1 = D.lm_p1[i], p2 = D.lm_p2[i];
-        const double U = ls_energy(c1, c2, c3, c4, k, l0, p1, p2);
-        mdy_xs_add(&xs, U);
-        if (PER_FF) {
-            if (i < D.n_ls) acc[MDY_FF_LS] += U;
-            else acc[MDY_FF_MS] += U;
-        }
+        ff = (i < D.n_ls) ? MDY_FF_LS : MDY_FF_MS;
+        U = ls_energy(c1, c2, c3, c4, k, l0, p1, p2);
         if (FORCES) {
             d3 f1, f2, f3, f4;
             ls_force(c1, c2, c3, c4, k, l0, p1, p2, f1, f2, f3, f4);
             int s = D.slot_lm + 4 * i;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
         }
-        return;
-    }
-    i -= D.n_lm;
-    if (i < D.n_bs) {
+    } else if ((i -= D.n_lm) < D.n_bs) {
         const int *b = D.bs_bs + 3 * i;
         d3 c1 = pos(b[0]), c2 = pos(b[1]), c3 = pos(b[2]);
         const double k = D.bs_k[i], l0 = D.bs_l0[i], p = D.bs_p[i];
-        const double U = bs_energy(c1, c2, c3, k, l0, p);
-        mdy_xs_add(&xs, U);
-        if (PER_FF) acc[MDY_FF_BS] += U;
+        ff = MDY_FF_BS;
+        U = bs_energy(c1, c2, c3, k, l0, p);
         if (FORCES) {
             d3 f1, f2, f3;
             bs_force(c1, c2, c3, k, l0, p, f1, f2, f3);
             int s = D.slot_bs + 3 * i;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
         }
-        return;
-    }
-    i -= D.n_bs;
-    if (i < D.n_bb) {
+    } else if ((i -= D.n_bs) < D.n_bb) {
         int4 b = D.bb_bs[i];
         d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
         const double k = D.bb_k[i], th = D.bb_th[i], ct = D.bb_ct[i], st = D.bb_st[i];
-        const double U = bb_energy(c1, c2, c3, c4, k, th, ct, st);
-        mdy_xs_add(&xs, U);
-        if (PER_FF) acc[MDY_FF_BB] += U;
+        ff = MDY_FF_BB;
+        U = bb_energy(c1, c2, c3, c4, k, th, ct, st);
         if (FORCES) {
             d3 f1, f2, f3, f4;
             bb_force(c1, c2, c3, c4, k, th, ct, st, f1, f2, f3, f4);
             int s = D.slot_bb + 4 * i;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
         }
-        return;
-    }
-    i -= D.n_bb;
-    if (i < D.n_bd) {
+    } else if ((i -= D.n_bb) < D.n_bd) {
         const double *pl = D.planes + 4 * D.bd_plane[i];
         d3 c = pos(D.bd_bead[i]);
-        const double U = bd_energy(pl, c, D.bd_k[i], D.bd_sl[i]);
-        mdy_xs_add(&xs, U);
-        if (PER_FF) acc[MDY_FF_BD] += U;
+        ff = MDY_FF_BD;
+        U = bd_energy(pl, c, D.bd_k[i], D.bd_sl[i]);
         if (FORCES) put3(contrib, D.slot_bd + i, bd_force(pl, c, D.bd_k[i], D.bd_sl[i]));
+    } else {
+        add = false;
+    }
+    if (add) {
+        mdy_xs_add(&xs, U);
+        if (PER_FF) {
+#pragma unroll
+            for (int k = 0; k < kNFF; k++) acc[k] += (k == ff) ? U : 0.0;
+        }
     }
 }
 
 // Per-bead force: the slots of bead b in reference order, starting from 0.0
 // (ForceFieldManager::computeForces zero-fills, then each force field adds).
+// The first kGS static and kGD dynamic slot indices and values are loaded
+// together before any addition, so a bead costs three dependent memory round
+// trips (offsets, indices, values) instead of two per slot.
+constexpr int kGS = 8, kGD = 8;
 __device__ __forceinline__ d3 gather_bead(const Dev &D, const double *__restrict__ contrib, int b) {
+    const int s0 = D.st_off[b], s1 = D.st_off[b + 1], d0 = D.dy_off[b], d1 = D.dy_off[b + 1];
+    int is[kGS], id[kGD];
+#pragma unroll
+    for (int j = 0; j < kGS; j++) is[j] = (s0 + j < s1) ? D.st_idx[s0 + j] : 0;
+#pragma unroll
+    for (int j = 0; j < kGD; j++) id[j] = (d0 + j < d1) ? D.dy_idx[d0 + j] : 0;
+    double vs[kGS][3], vd[kGD][3];
+#pragma unroll
+    for (int j = 0; j < kGS; j++) {
+        if (s0 + j < s1) { vs[j][0] = contrib[3 * is[j]]; vs[j][1] = contrib[3 * is[j] + 1]; vs[j][2] = contrib[3 * is[j] + 2]; }
+    }
+#pragma unroll
+    for (int j = 0; j < kGD; j++) {
+        if (d0 + j < d1) { vd[j][0] = contrib[3 * id[j]]; vd[j][1] = contrib[3 * id[j] + 1]; vd[j][2] = contrib[3 * id[j] + 2]; }
+    }
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int k = D.st_off[b], e = D.st_off[b + 1]; k < e; k++) {
+#pragma unroll
+    for (int j = 0; j < kGS; j++)
+        if (s0 + j < s1) { fx += vs[j][0]; fy += vs[j][1]; fz += vs[j][2]; }
+    for (int k = s0 + kGS; k < s1; k++) {
         int s = D.st_idx[k];
         fx += contrib[3 * s]; fy += contrib[3 * s + 1]; fz += contrib[3 * s + 2];
     }
-    for (int k = D.dy_off[b], e = D.dy_off[b + 1]; k < e; k++) {
+#pragma unroll
+    for (int j = 0; j < kGD; j++)
+        if (d0 + j < d1) { fx += vd[j][0]; fy += vd[j][1]; fz += vd[j][2]; }
+    for (int k = d0 + kGD; k < d1; k++) {
         int s = D.dy_idx[k];
         fx += contrib[3 * s]; fy += contrib[3 * s + 1]; fz += contrib[3 * s + 2];
     }
@@ -303,26 +344,21 @@ __device__ __forceinline__ void grid_partials_reduce(const double *red, int nblk
     block_reduce<N>(v, lds, max_mask, min_mask);
 }
 
-// Energy epilogue, shared by the one-shot and persistent kernels. The block
-// partials hold per-FF double sums at [0, kNFF) and the exact total at
-// [kXsE, kXsE + 8). Quadrature-fallback energies (queue qe, values qe_u) are
-// added to both. e[0..7] per FF (reported values), e[8] = computeEnergy total
-// (+inf when a force field returned its -1 sentinel), e[9] = the sum without
-// the sentinel.
-constexpr int kXsE = 8;    // exact energy total
-constexpr int kXsA = 16;   // second exact accumulator (slope s.F)
-__device__ __forceinline__ void energy_epilogue(const Dev &D, const double *red, int nblk, int nq, const int *qe,
-                                                const double *qe_u, double *lds, double e[kNFF + 2], bool per_ff) {
+// Energy epilogue, shared by the one-shot and persistent kernels. X holds the
+// exact total of the analytic energies; v (per_ff) the per-force-field double
+// sums. Quadrature-fallback energies (queue qe, values qe_u) are added to
+// both, in increasing pair order. e[0..7] per FF (reported values), e[8] =
+// computeEnergy total (+inf when a force field returned its -1 sentinel),
+// e[9] = the sum without the sentinel.
+constexpr int kXsE = 8;    // exact energy total in a reduction record
+constexpr int kXsA = 16;   // second exact accumulator in a reduction record
+__device__ __forceinline__ void energy_finish(mdy_xsum X, const double v_in[kNFF], int nq, const int *qe,
+                                              const double *qe_u, double e[kNFF + 2]) {
     double v[kNFF];
-    if (per_ff) grid_partials_reduce<kNFF>(red, nblk, v, lds);
-    else
-        for (int k = 0; k < kNFF; k++) v[k] = 0.0;
-    mdy_xsum X;
-    grid_reduce_xsum(red, nblk, kXsE, X, reinterpret_cast<long long *>(lds));
+    for (int k = 0; k < kNFF; k++) v[k] = v_in ? v_in[k] : 0.0;
     nq = min(nq, kQueueCap);
     bool bad = false;
     if (nq > 0) {
-        // per-FF value: fallback energies added in increasing pair order
         double add = 0.0;
         int last = -1;
         for (int r = 0; r < nq; r++) {
@@ -344,6 +380,15 @@ __device__ __forceinline__ void energy_epilogue(const Dev &D, const double *red,
     const double tot = mdy_xs_value(&X);
     e[kNFF] = bad ? INFINITY : tot;
     e[kNFF + 1] = bad ? tot - 1.0 : tot;
+}
+// the same from per-block records: per-FF doubles at [0, kNFF), exact total at kXsE
+__device__ __forceinline__ void energy_epilogue(const Dev &D, const double *red, int nblk, int nq, const int *qe,
+                                                const double *qe_u, double *lds, double e[kNFF + 2], bool per_ff) {
+    double v[kNFF];
+    if (per_ff) grid_partials_reduce<kNFF>(red, nblk, v, lds);
+    mdy_xsum X;
+    grid_reduce_xsum(red, nblk, kXsE, X, reinterpret_cast<long long *>(lds));
+    energy_finish(X, per_ff ? v : nullptr, nq, qe, qe_u, e);
 }
 
 // ---- Simpson quadrature fallbacks (CylinderExclVolRepulsion.cpp:884-1254) ----
@@ -367,11 +412,13 @@ __device__ __forceinline__ void simpson_e_node(const SimpsonGeo &g, int i, int j
     const double ds = 1.0 / ((double)N);
     double s = (double)(i + 1) * ds, t = (double)(j + 1) * ds;
     double fs = (i % 2 == 0) ? 2.0 : 4.0, ft = (j % 2 == 0) ? 2.0 : 4.0;
-    if (j == 0 && i > 0 && i < N) mdy_xs_add(&X, fs * (eint(g, s, 0.0) + eint(g, s, 1.0)));
+    double v = 0.0;
+    if (j == 0 && i > 0 && i < N) v = fs * (eint(g, s, 0.0) + eint(g, s, 1.0));
     if (j > 0 && j < N) {
-        if (i == 0) mdy_xs_add(&X, ft * (eint(g, 0.0, t) + eint(g, 1.0, t)));
-        else if (i < N) mdy_xs_add(&X, fs * ft * eint(g, s, t));
+        if (i == 0) v = ft * (eint(g, 0.0, t) + eint(g, 1.0, t));
+        else if (i < N) v = fs * ft * eint(g, s, t);
     }
+    mdy_xs_add(&X, v);   // adding an exact 0.0 changes nothing
 }
 __device__ __forceinline__ double simpson_e_corners(const SimpsonGeo &g) {
     return eint(g, 0.0, 0.0) + eint(g, 0.0, 1.0) + eint(g, 1.0, 0.0) + eint(g, 1.0, 1.0);
@@ -386,18 +433,21 @@ __device__ __forceinline__ void simpson_f_node(const SimpsonGeo &g, int i, int j
     const double ds = 1.0 / ((double)N);
     double s = (double)(i + 1) * ds, t = (double)(j + 1) * ds;
     double fs = (i % 2 == 0) ? 2.0 : 4.0, ft = (j % 2 == 0) ? 2.0 : 4.0;
+    double v1 = 0.0, v2 = 0.0;   // the node's terms (an exact 0.0 adds nothing)
     if (j == 0 && i > 0 && i < N) {
-        mdy_xs_add(&X, fs * fint_q(g, s, 0.0, q));
-        mdy_xs_add(&X, fs * fint_q(g, s, 1.0, q));
+        v1 = fs * fint_q(g, s, 0.0, q);
+        v2 = fs * fint_q(g, s, 1.0, q);
     }
     if (j > 0 && j < N) {
         if (i == 0) {
-            mdy_xs_add(&X, ft * fint_q(g, 0.0, t, q));
-            mdy_xs_add(&X, ft * fint_q(g, 1.0, t, q));
+            v1 = ft * fint_q(g, 0.0, t, q);
+            v2 = ft * fint_q(g, 1.0, t, q);
         } else if (i < N) {
-            mdy_xs_add(&X, fs * ft * fint_q(g, s, t, q));
+            v1 = fs * ft * fint_q(g, s, t, q);
         }
     }
+    mdy_xs_add(&X, v1);
+    mdy_xs_add(&X, v2);
 }
 __device__ __forceinline__ void simpson_f_corners(const SimpsonGeo &g, int q, mdy_xsum &X) {
     mdy_xs_add(&X, fint_q(g, 0.0, 0.0, q));

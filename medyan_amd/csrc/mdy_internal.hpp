@@ -95,12 +95,11 @@ struct Ctx {
     DBuf<double> qe_u, simpson_part;
     // CG state
     DBuf<double> F0, F1, S0, S1, xmin;
-    DBuf<unsigned long long> bar;
+    DBuf<unsigned long long> bar, acc;
     DBuf<double> cg_out;          // result block
     DBuf<double> tr_lam, tr_e, tr_mf, tr_beta; DBuf<int> tr_safe;
     long long trace_cap = 0, trace_n = 0;
     DBuf<long long> prof; DBuf<int> prof_kind; long long prof_cap = 0, prof_n = 0;
-    DBuf<double> cg_args;         // CgArgs image read by the persistent kernel
     double last_ms = 0;
     int coop_blocks = 0;
     Dev dev() const;

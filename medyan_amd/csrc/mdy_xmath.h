@@ -35,30 +35,39 @@
 /* ------------------------------------------------------------------------ */
 /* atan                                                                      */
 /* ------------------------------------------------------------------------ */
-/* atan(k/8) = hi + lo, k = 0..8 */
-#define MDY_ATAN_HI(k)                                                                                      \
-    ((k) == 0 ? 0.0 : (k) == 1 ? 0x1.fd5ba9aac2f6ep-4 : (k) == 2 ? 0x1.f5b75f92c80ddp-3                   \
-     : (k) == 3 ? 0x1.6f61941e4def1p-2 : (k) == 4 ? 0x1.dac670561bb4fp-2 : (k) == 5 ? 0x1.1e00babdefeb4p-1 \
-     : (k) == 6 ? 0x1.4978fa3269ee1p-1 : (k) == 7 ? 0x1.700a7c5784634p-1 : 0x1.921fb54442d18p-1)
-#define MDY_ATAN_LO(k)                                                                                        \
-    ((k) == 0 ? 0.0 : (k) == 1 ? -0x1.cd37686760c17p-59 : (k) == 2 ? 0x1.8ab6e3cf7afbdp-57                  \
-     : (k) == 3 ? -0x1.c63aae6f6e918p-56 : (k) == 4 ? 0x1.a2b7f222f65e2p-56 : (k) == 5 ? -0x1.928df287a668fp-58 \
-     : (k) == 6 ? 0x1.2419a87f2a458p-56 : (k) == 7 ? -0x1.8c34d25aadef6p-56 : 0x1.1a62633145c07p-55)
+/* atan(k/8) = hi + lo, k = 1..8 (k = 0: 0) */
+#define MDY_ATAN_H1 0x1.fd5ba9aac2f6ep-4
+#define MDY_ATAN_H2 0x1.f5b75f92c80ddp-3
+#define MDY_ATAN_H3 0x1.6f61941e4def1p-2
+#define MDY_ATAN_H4 0x1.dac670561bb4fp-2
+#define MDY_ATAN_H5 0x1.1e00babdefeb4p-1
+#define MDY_ATAN_H6 0x1.4978fa3269ee1p-1
+#define MDY_ATAN_H7 0x1.700a7c5784634p-1
+#define MDY_ATAN_H8 0x1.921fb54442d18p-1
+#define MDY_ATAN_L1 -0x1.cd37686760c17p-59
+#define MDY_ATAN_L2 0x1.8ab6e3cf7afbdp-57
+#define MDY_ATAN_L3 -0x1.c63aae6f6e918p-56
+#define MDY_ATAN_L4 0x1.a2b7f222f65e2p-56
+#define MDY_ATAN_L5 -0x1.928df287a668fp-58
+#define MDY_ATAN_L6 0x1.2419a87f2a458p-56
+#define MDY_ATAN_L7 -0x1.8c34d25aadef6p-56
+#define MDY_ATAN_L8 0x1.1a62633145c07p-55
 #define MDY_PIO2_HI 0x1.921fb54442d18p+0
 #define MDY_PIO2_LO 0x1.1a62633145c07p-54
 
+/* Straight-line code (selects, no branches): on the device every lane runs
+ * the same instructions, which keeps the inlined copies short. */
 MDY_HD double mdy_atan(double x) {
-    if (x != x) return x;
-    double ax = fabs(x);
-    int inv = ax > 1.0;
-    double y = inv ? 1.0 / ax : ax;              /* y in [0, 1]; 1/inf = 0 */
+    const double ax = fabs(x);
+    const int inv = ax > 1.0;
+    const double y = inv ? 1.0 / ax : ax;        /* y in [0, 1]; 1/inf = 0 */
     /* nearest table point c = k/8, except below 3/32 where the series is
      * used directly: reducing y near 1/16 to t ~ -y would pass the rounding
      * error of t undiminished into a result of the same size */
-    int k = y < 0.09375 ? 0 : (int)(y * 8.0 + 0.5);
-    double c = (double)k * 0.125;
-    double t = (y - c) / (1.0 + y * c);          /* |t| <= 3/32: atan(y) = atan(c) + atan(t) */
-    double t2 = t * t;
+    const int k = (y < 0.09375 || !(y <= 1.0)) ? 0 : (int)(y * 8.0 + 0.5);
+    const double c = (double)k * 0.125;
+    const double t = (y - c) / (1.0 + y * c);    /* |t| <= 3/32: atan(y) = atan(c) + atan(t) */
+    const double t2 = t * t;
     /* atan(t) = t - t^3/3 + t^5/5 - ... ; |t|^17/17 < 2^-58 |t| */
     double p = -1.0 / 15.0;
     p = p * t2 + 1.0 / 13.0;
@@ -68,9 +77,20 @@ MDY_HD double mdy_atan(double x) {
     p = p * t2 + 1.0 / 5.0;
     p = p * t2 - 1.0 / 3.0;
     p = t + t * (t2 * p);
-    double r = MDY_ATAN_HI(k) + (MDY_ATAN_LO(k) + p);
-    if (inv) r = (MDY_PIO2_HI - r) + MDY_PIO2_LO;
-    return x < 0 ? -r : r;
+    double hi = 0.0, lo = 0.0;
+    hi = k == 1 ? MDY_ATAN_H1 : hi; lo = k == 1 ? MDY_ATAN_L1 : lo;
+    hi = k == 2 ? MDY_ATAN_H2 : hi; lo = k == 2 ? MDY_ATAN_L2 : lo;
+    hi = k == 3 ? MDY_ATAN_H3 : hi; lo = k == 3 ? MDY_ATAN_L3 : lo;
+    hi = k == 4 ? MDY_ATAN_H4 : hi; lo = k == 4 ? MDY_ATAN_L4 : lo;
+    hi = k == 5 ? MDY_ATAN_H5 : hi; lo = k == 5 ? MDY_ATAN_L5 : lo;
+    hi = k == 6 ? MDY_ATAN_H6 : hi; lo = k == 6 ? MDY_ATAN_L6 : lo;
+    hi = k == 7 ? MDY_ATAN_H7 : hi; lo = k == 7 ? MDY_ATAN_L7 : lo;
+    hi = k == 8 ? MDY_ATAN_H8 : hi; lo = k == 8 ? MDY_ATAN_L8 : lo;
+    double r = hi + (lo + p);
+    const double ri = (MDY_PIO2_HI - r) + MDY_PIO2_LO;
+    r = inv ? ri : r;
+    r = x < 0 ? -r : r;
+    return (x != x) ? x : r;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -202,17 +222,24 @@ MDY_HD double mdy_xs_value(const mdy_xsum *in) {
         for (int k = 0; k < MDY_XS_N; k++) d[k] = -d[k];
         for (int k = 0; k < MDY_XS_N - 1; k++) { int64_t c = d[k] >> 48; d[k] &= MDY_XS_MASK; d[k + 1] += c; }
     }
+    /* highest nonzero digit h and the two below it, selected without
+     * indexing (the digits stay in registers on the device) */
     int h = -1;
-    for (int k = MDY_XS_N - 1; k >= 0; k--)
-        if (h < 0 && d[k] != 0) h = k;
+    for (int k = 0; k < MDY_XS_N; k++)
+        if (d[k] != 0) h = k;
     if (h < 0) return 0.0;
-    uint64_t top = (uint64_t)d[h];
+    uint64_t top = 0, d1 = 0, d2 = 0;
+    int sticky = 0;
+    for (int k = 0; k < MDY_XS_N; k++) {
+        top = (k == h) ? (uint64_t)d[k] : top;
+        d1 = (k == h - 1) ? (uint64_t)d[k] : d1;
+        d2 = (k == h - 2) ? (uint64_t)d[k] : d2;
+        sticky |= (k < h - 2) && d[k] != 0;
+    }
     int nb = 0;
     while (nb < 64 && (top >> nb) != 0) nb++;     /* bit length of the top digit, 1..63 */
-    uint64_t d1 = h >= 1 ? (uint64_t)d[h - 1] : 0, d2 = h >= 2 ? (uint64_t)d[h - 2] : 0;
     int need = 64 - nb;                           /* bits taken from below the top digit */
     uint64_t W = (nb == 64) ? top : (top << need);
-    int sticky = 0;
     if (need <= 48) {
         W |= (need == 0) ? 0 : (d1 >> (48 - need));
         sticky |= (d1 & ((1ull << (48 - need)) - 1)) != 0;
@@ -221,7 +248,6 @@ MDY_HD double mdy_xs_value(const mdy_xsum *in) {
         W |= (d1 << (need - 48)) | (d2 >> (96 - need));
         sticky |= (d2 & ((1ull << (96 - need)) - 1)) != 0;
     }
-    for (int k = 0; k < h - 2; k++) sticky |= d[k] != 0;
     uint64_t mant = W >> 11;
     int rbit = (int)((W >> 10) & 1);
     sticky |= (W & 0x3ff) != 0;

@@ -71,6 +71,7 @@ def load_library(path: str = LIB_PATH):
         "mdy_last_timing": (C.c_int, [vp, f64p]),
         "mdy_set_profile": (C.c_int, [vp, C.c_int64]),
         "mdy_get_profile": (C.c_int, [vp, i32p, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]),
+        "mdy_get_profile_detail": (C.c_int, [vp, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]),
         "mdy_gen_defaults": (None, [C.POINTER(_abi.GenParams)]),
         "mdy_generate": (C.c_int, [C.POINTER(_abi.GenParams), C.POINTER(C.POINTER(_abi.Network))]),
         "mdy_network_free": (None, [C.POINTER(_abi.Network)]),
@@ -282,6 +283,17 @@ class Context:
                                              ticks.ctypes.data_as(C.POINTER(C.c_int64)), m, C.byref(n)))
         m = min(m, int(n.value))
         return kind[:m], ticks[:m]
+
+    def profile_detail(self):
+        """[n x 5] stamps per phase: barrier exit, latest arrival, block 0's
+        work start, end of its main loop, its arrival (100 MHz ticks)."""
+        n = C.c_int64()
+        self._check(self.lib.mdy_get_profile_detail(self.h, None, 0, C.byref(n)))
+        m = int(n.value)
+        rec = np.zeros(5 * max(m, 1), dtype=np.int64)
+        self._check(self.lib.mdy_get_profile_detail(self.h, rec.ctypes.data_as(C.POINTER(C.c_int64)), m,
+                                                    C.byref(n)))
+        return rec[: 5 * m].reshape(-1, 5)
 
     def last_ms(self):
         t = C.c_double()
