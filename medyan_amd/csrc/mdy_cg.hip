@@ -73,7 +73,9 @@ struct Eval {
     int qb;        // queue buffer (0..2)
 };
 
+template <bool MIXED>
 struct Cg {
+    using Slot = typename SlotT<MIXED>::T;   // force records: double, or float in the mixed form
     const CgArgs &A;
     double *lds;
     int G, blk, tid, gtid, gs;
@@ -86,6 +88,7 @@ struct Cg {
         gtid = blk * blockDim.x + tid; gs = G * blockDim.x;
     }
     __device__ long long *ldsl() const { return reinterpret_cast<long long *>(lds); }
+    __device__ Slot *slots() const { return reinterpret_cast<Slot *>(A.contrib); }
     __device__ Dev dv(int b) const {
         Dev d = A.D;
         d.qe = A.qe + b * kQueueCap; d.qe_n = A.qn + 4 * b;
@@ -231,7 +234,7 @@ struct Cg {
         mdy_xs_zero(&xs);
         const int total = d.n_items();
         mark(0);
-        for (int i = gtid; i < total; i += gs) item(d, i, pos, nullptr, xs, A.contrib, false, true);
+        sweep<MIXED>(d, pos, gtid, gs, nullptr, xs, slots(), false, true);
         mark(1);
         block_reduce_xsum(xs, ldsl());
         acc_push(accs(p0) + kAcc0, xs);
@@ -264,7 +267,7 @@ struct Cg {
         mdy_xsum xs;
         mdy_xs_zero(&xs);
         const int total = d.n_items();
-        for (int i = gtid; i < total; i += gs) item(d, i, pos, acc, xs, nullptr, true, false);
+        sweep<MIXED>(d, pos, gtid, gs, acc, xs, nullptr, true, false);
         double *r = red(p0);
         block_reduce<kNFF>(acc, lds);
         if (tid < kNFF) r[(size_t)blk * kRed + tid] = acc[tid];
@@ -335,8 +338,8 @@ struct Cg {
                 if (ok) {
                     if (blk == 0 && tid == 0) {
                         int s = A.D.slot_ev + 4 * i;
-                        put3(A.contrib, s, f[0]); put3(A.contrib, s + 1, f[1]);
-                        put3(A.contrib, s + 2, f[2]); put3(A.contrib, s + 3, f[3]);
+                        put3(slots(), s, f[0]); put3(slots(), s + 1, f[1]);
+                        put3(slots(), s + 2, f[2]); put3(slots(), s + 3, f[3]);
                     }
                     done = true;
                 } else if (moved || !inplane) {
@@ -360,7 +363,7 @@ struct Cg {
         mdy_xs_zero(&sfp);
         mark(0);
         for (int b = gtid; b < A.D.nb; b += gs) {
-            d3 f = gather_bead(A.D, A.contrib, b);
+            d3 f = gather_bead(A.D, slots(), b);
             const int o = 3 * b;
             Fout[o] = f.x; Fout[o + 1] = f.y; Fout[o + 2] = f.z;
             m[0] = fmax(m[0], fmax(fabs(f.x), fmax(fabs(f.y), fabs(f.z))));
@@ -400,7 +403,8 @@ struct Cg {
 
 // s = F + beta * d for every bead into Sn, and this block's share of s.F,
 // pushed into the current phase's second accumulator
-__device__ __attribute__((always_inline)) void spec_direction(Cg &K, const double *Fn, const double *dprev,
+template <bool MIXED>
+__device__ __attribute__((always_inline)) void spec_direction(Cg<MIXED> &K, const double *Fn, const double *dprev,
                                                               double beta, double *Sn) {
     mdy_xsum sd;
     mdy_xs_zero(&sd);
@@ -431,10 +435,11 @@ enum Stage {
 
 // The arguments come by value: pointers read from the kernel-argument segment
 // are known to be global, so every access is a global (not flat) access.
+template <bool MIXED>
 __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
     __shared__ double lds[17 * 16];
     const CgArgs &A = Av;
-    Cg K(A, lds);
+    Cg<MIXED> K(A, lds);
     double *R = A.out;
     const bool leader = (K.blk == 0 && K.tid == 0);
 #define FB(i) ((i) ? A.F1 : A.F0)
@@ -709,7 +714,9 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     MDY_TRY(ensure(c.tr_safe, c.trace_cap + 1));
     // persistent grid: all blocks resident
     int per_cu = 0;
-    MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_cg), kBlock, 0));
+    const bool mixed = c.opts.precision == MDY_PREC_MIXED;
+    const void *kern = mixed ? reinterpret_cast<const void *>(k_cg<true>) : reinterpret_cast<const void *>(k_cg<false>);
+    MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
     if (per_cu < 1) return MDY_ERR_LAUNCH;
     if (c.opts.blocks_per_cu > 0 && c.opts.blocks_per_cu < per_cu) per_cu = c.opts.blocks_per_cu;
     const int G = per_cu * c.num_cu;
@@ -741,7 +748,7 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     MDY_HIP(hipEventCreate(&e1));
     MDY_HIP(hipEventRecord(e0, st));
     void *args[] = {&A};
-    hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_cg), dim3(G), dim3(kBlock), args, 0, st);
+    hipError_t le = hipLaunchCooperativeKernel(kern, dim3(G), dim3(kBlock), args, 0, st);
     if (le != hipSuccess) {
         (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
         return MDY_ERR_LAUNCH;

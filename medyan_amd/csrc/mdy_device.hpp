@@ -17,6 +17,9 @@
 namespace mdy {
 
 struct d3 { double x, y, z; };
+struct f3 { float x, y, z; };   // force records of the mixed-precision form
+template <class R> struct vec3_of { using T = d3; };
+template <> struct vec3_of<float> { using T = f3; };
 
 __device__ __forceinline__ d3 ld3(const double *__restrict__ p) { return d3{p[0], p[1], p[2]}; }
 __device__ __forceinline__ d3 sub(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -264,54 +267,60 @@ __device__ __forceinline__ bool ev_energy_bad(double U) { return fabs(U) == INFI
 
 // Analytic pair force; f[0..3] = forces on c1..c4. Direction vectors use the
 // c2 passed in (the moved one in the coplanar case), as the reference does.
-__device__ __forceinline__ void ev_force_inv(const EvInv &v, d3 c1, d3 c2, d3 c3, d3 c4, double kr, d3 f[4]) {
-    const double a = v.a, b = v.b, c = v.c, d = v.d, e = v.e, F = v.F;
-    const double AA = v.AA, BB = v.BB, CC = v.CC, DD = v.DD, EE = v.EE, FF = v.FF, GG = v.GG, HH = v.HH;
-    const double ATG1 = v.ATG1, ATG2 = v.ATG2, ATG3 = v.ATG3, ATG4 = v.ATG4;
-    const double invJJ = 1 / v.JJ;
-    const double sumBlock = CC / AA * ATG1 + GG / EE * ATG2 + DD / BB * ATG3 + HH / FF * ATG4;
-    const double enFactor = kr * invJJ / 2;
-    const double A1 = AA * AA / (AA * AA + (a + e) * (a + e));
-    const double A2 = AA * AA / (AA * AA + e * e);
-    const double E1 = EE * EE / (EE * EE + (a + e - d) * (a + e - d));
-    const double E2 = EE * EE / (EE * EE + (e - d) * (e - d));
-    const double B1 = BB * BB / (BB * BB + F * F);
-    const double B2 = BB * BB / (BB * BB + (F - b) * (F - b));
-    const double F1 = FF * FF / (FF * FF + (d + F) * (d + F));
-    const double F2 = FF * FF / (FF * FF + (d + F - b) * (d + F - b));
-    const double pA_C = ATG1 / AA;
-    const double pA_A = -(ATG1 * CC) / (AA * AA) + (-A1 * (a + e) + A2 * e) * CC / (AA * AA * AA);
-    const double pA_e = ((A1 - A2) * CC) / (AA * AA);
-    const double pA_a = (A1 * CC) / (AA * AA);
-    const double pE_G = ATG2 / EE;
-    const double pE_E = -(ATG2 * GG) / (EE * EE) + (-E1 * (a + e - d) + E2 * (e - d)) * GG / (EE * EE * EE);
-    const double pE_emd = ((E1 - E2) * GG) / (EE * EE);
-    const double pE_a = (E1 * GG) / (EE * EE);
-    const double pB_D = ATG3 / BB;
-    const double pB_B = -(ATG3 * DD) / (BB * BB) + (-B1 * F + B2 * (F - b)) * DD / (BB * BB * BB);
-    const double pB_fmb = ((B1 - B2) * DD) / (BB * BB);
-    const double pB_b = (B1 * DD) / (BB * BB);
-    const double pF_H = ATG4 / FF;
-    const double pF_F = -(ATG4 * HH) / (FF * FF) + (-F1 * (d + F) + F2 * (d + F - b)) * HH / (FF * FF * FF);
-    const double pF_dfb = ((F1 - F2) * HH) / (FF * FF);
-    const double pF_b = (F1 * HH) / (FF * FF);
-    const double k11 = pA_C * (-2 * F) + pA_A * (c / AA) + pA_a * 2 + pE_G * (2 * (F - b)) +
-                       pE_E * ((b + c - 2 * F) / EE) + pE_a * 2 + pF_H * (-2 * b) + pF_F * (b / FF);
-    const double k12 = pA_C * e + pE_G * (2 * d - e) + pE_E * ((e - d) / EE) + pE_emd * (-1) + pB_D * (-F) +
-                       pF_H * (2 * d + F) + pF_F * (-(d + F) / FF) + pF_dfb;
-    const double k13 = pA_C * d + pA_A * (-e / AA) + pA_e + pE_G * (-d) + pE_E * (-(e - d) / EE) + pE_emd +
-                       pB_D * b + pF_H * (-b) + pF_F * (b / FF);
-    const double k22 = pE_G * (-2 * a) + pE_E * (a / EE) + pB_D * (2 * e) + pB_B * (c / BB) + pB_fmb * (-2) +
-                       pB_b * 2 + pF_H * (-2 * (a + e)) + pF_F * ((a + c + 2 * e) / FF) + pF_dfb * (-2) + pF_b * 2;
-    const double k23 = pA_C * (-a) + pE_G * a + pE_E * (-a / EE) + pB_D * (-d) + pB_B * (-F / BB) + pB_fmb +
-                       pF_H * d + pF_F * (-(d + F) / FF) + pF_dfb;
-    const double k33 = pA_A * (a / AA) + pE_E * (a / EE) + pB_B * (b / BB) + pF_F * (b / FF);
-    const double k21 = k12, k31 = k13, k32 = k23;
-    const double J11 = 2 * (-b * c + F * F), J12 = 2 * (c * d - e * F), J13 = 2 * (b * e - d * F);
-    const double J21 = 2 * (c * d - e * F), J22 = 2 * (-a * c + e * e), J23 = 2 * (-d * e + a * F);
-    const double J31 = 2 * (b * e - d * F), J32 = 2 * (-d * e + a * F), J33 = 2 * (d * d - a * b);
-    const double sb = -invJJ * sumBlock;
-    double dv[4][3];
+// R = double: the FP64 path; R = float: the mixed-precision form, whose
+// derivative blocks run in FP32 from the FP64 invariants.
+template <class R>
+__device__ __forceinline__ void ev_force_inv_t(const EvInv &v, d3 c1, d3 c2, d3 c3, d3 c4, double kr_,
+                                               typename vec3_of<R>::T f[4]) {
+    const R a = (R)v.a, b = (R)v.b, c = (R)v.c, d = (R)v.d, e = (R)v.e, F = (R)v.F;
+    const R AA = (R)v.AA, BB = (R)v.BB, CC = (R)v.CC, DD = (R)v.DD, EE = (R)v.EE, FF = (R)v.FF, GG = (R)v.GG,
+            HH = (R)v.HH;
+    const R ATG1 = (R)v.ATG1, ATG2 = (R)v.ATG2, ATG3 = (R)v.ATG3, ATG4 = (R)v.ATG4;
+    const R kr = (R)kr_;
+    const R invJJ = 1 / (R)v.JJ;
+    const R sumBlock = CC / AA * ATG1 + GG / EE * ATG2 + DD / BB * ATG3 + HH / FF * ATG4;
+    const R enFactor = kr * invJJ / 2;
+    const R A1 = AA * AA / (AA * AA + (a + e) * (a + e));
+    const R A2 = AA * AA / (AA * AA + e * e);
+    const R E1 = EE * EE / (EE * EE + (a + e - d) * (a + e - d));
+    const R E2 = EE * EE / (EE * EE + (e - d) * (e - d));
+    const R B1 = BB * BB / (BB * BB + F * F);
+    const R B2 = BB * BB / (BB * BB + (F - b) * (F - b));
+    const R F1 = FF * FF / (FF * FF + (d + F) * (d + F));
+    const R F2 = FF * FF / (FF * FF + (d + F - b) * (d + F - b));
+    const R pA_C = ATG1 / AA;
+    const R pA_A = -(ATG1 * CC) / (AA * AA) + (-A1 * (a + e) + A2 * e) * CC / (AA * AA * AA);
+    const R pA_e = ((A1 - A2) * CC) / (AA * AA);
+    const R pA_a = (A1 * CC) / (AA * AA);
+    const R pE_G = ATG2 / EE;
+    const R pE_E = -(ATG2 * GG) / (EE * EE) + (-E1 * (a + e - d) + E2 * (e - d)) * GG / (EE * EE * EE);
+    const R pE_emd = ((E1 - E2) * GG) / (EE * EE);
+    const R pE_a = (E1 * GG) / (EE * EE);
+    const R pB_D = ATG3 / BB;
+    const R pB_B = -(ATG3 * DD) / (BB * BB) + (-B1 * F + B2 * (F - b)) * DD / (BB * BB * BB);
+    const R pB_fmb = ((B1 - B2) * DD) / (BB * BB);
+    const R pB_b = (B1 * DD) / (BB * BB);
+    const R pF_H = ATG4 / FF;
+    const R pF_F = -(ATG4 * HH) / (FF * FF) + (-F1 * (d + F) + F2 * (d + F - b)) * HH / (FF * FF * FF);
+    const R pF_dfb = ((F1 - F2) * HH) / (FF * FF);
+    const R pF_b = (F1 * HH) / (FF * FF);
+    const R k11 = pA_C * (-2 * F) + pA_A * (c / AA) + pA_a * 2 + pE_G * (2 * (F - b)) +
+                  pE_E * ((b + c - 2 * F) / EE) + pE_a * 2 + pF_H * (-2 * b) + pF_F * (b / FF);
+    const R k12 = pA_C * e + pE_G * (2 * d - e) + pE_E * ((e - d) / EE) + pE_emd * (-1) + pB_D * (-F) +
+                  pF_H * (2 * d + F) + pF_F * (-(d + F) / FF) + pF_dfb;
+    const R k13 = pA_C * d + pA_A * (-e / AA) + pA_e + pE_G * (-d) + pE_E * (-(e - d) / EE) + pE_emd +
+                  pB_D * b + pF_H * (-b) + pF_F * (b / FF);
+    const R k22 = pE_G * (-2 * a) + pE_E * (a / EE) + pB_D * (2 * e) + pB_B * (c / BB) + pB_fmb * (-2) +
+                  pB_b * 2 + pF_H * (-2 * (a + e)) + pF_F * ((a + c + 2 * e) / FF) + pF_dfb * (-2) + pF_b * 2;
+    const R k23 = pA_C * (-a) + pE_G * a + pE_E * (-a / EE) + pB_D * (-d) + pB_B * (-F / BB) + pB_fmb +
+                  pF_H * d + pF_F * (-(d + F) / FF) + pF_dfb;
+    const R k33 = pA_A * (a / AA) + pE_E * (a / EE) + pB_B * (b / BB) + pF_F * (b / FF);
+    const R k21 = k12, k31 = k13, k32 = k23;
+    const R J11 = 2 * (-b * c + F * F), J12 = 2 * (c * d - e * F), J13 = 2 * (b * e - d * F);
+    const R J21 = 2 * (c * d - e * F), J22 = 2 * (-a * c + e * e), J23 = 2 * (-d * e + a * F);
+    const R J31 = 2 * (b * e - d * F), J32 = 2 * (-d * e + a * F), J33 = 2 * (d * d - a * b);
+    const R sb = -invJJ * sumBlock;
+    R dv[4][3];
     dv[0][0] = enFactor * (sb * (J31 - J11) + (k31 - k11));
     dv[0][1] = enFactor * (sb * (J32 - J12) + (k32 - k12));
     dv[0][2] = enFactor * (sb * (J33 - J13) + (k33 - k13));
@@ -324,13 +333,18 @@ __device__ __forceinline__ void ev_force_inv(const EvInv &v, d3 c1, d3 c2, d3 c3
     dv[3][0] = enFactor * (sb * (J21) + (k21));
     dv[3][1] = enFactor * (sb * (J22) + (k22));
     dv[3][2] = enFactor * (sb * (J23) + (k23));
-    const d3 u1 = sub(c2, c1), u2 = sub(c4, c3), u3 = sub(c1, c3);
+    const d3 u1d = sub(c2, c1), u2d = sub(c4, c3), u3d = sub(c1, c3);
+    const R u1x = (R)u1d.x, u1y = (R)u1d.y, u1z = (R)u1d.z, u2x = (R)u2d.x, u2y = (R)u2d.y, u2z = (R)u2d.z;
+    const R u3x = (R)u3d.x, u3y = (R)u3d.y, u3z = (R)u3d.z;
 #pragma unroll
     for (int bi = 0; bi < 4; bi++) {
-        f[bi].x = -(dv[bi][0] * u1.x + dv[bi][1] * u2.x + dv[bi][2] * u3.x);
-        f[bi].y = -(dv[bi][0] * u1.y + dv[bi][1] * u2.y + dv[bi][2] * u3.y);
-        f[bi].z = -(dv[bi][0] * u1.z + dv[bi][1] * u2.z + dv[bi][2] * u3.z);
+        f[bi].x = -(dv[bi][0] * u1x + dv[bi][1] * u2x + dv[bi][2] * u3x);
+        f[bi].y = -(dv[bi][0] * u1y + dv[bi][1] * u2y + dv[bi][2] * u3y);
+        f[bi].z = -(dv[bi][0] * u1z + dv[bi][1] * u2z + dv[bi][2] * u3z);
     }
+}
+__device__ __forceinline__ void ev_force_inv(const EvInv &v, d3 c1, d3 c2, d3 c3, d3 c4, double kr, d3 f[4]) {
+    ev_force_inv_t<double>(v, c1, c2, c3, c4, kr, f);
 }
 __device__ __forceinline__ void ev_force(d3 c1, d3 c2, d3 c3, d3 c4, double kr, d3 f[4]) {
     EvInv v;
@@ -338,6 +352,7 @@ __device__ __forceinline__ void ev_force(d3 c1, d3 c2, d3 c3, d3 c4, double kr, 
     ev_force_inv(v, c1, c2, c3, c4, kr, f);
 }
 __device__ __forceinline__ bool finite_d3(d3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+__device__ __forceinline__ bool finite_d3(f3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 
 // Simpson integrands (CylinderExclVolRepulsion.h:53-72)
 __device__ __forceinline__ double ev_r2(double a, double b, double c, double d, double e, double F, double s, double t) {

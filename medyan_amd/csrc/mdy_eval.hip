@@ -57,14 +57,15 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(Dev D, const double *re
         for (int k = 0; k < kNFF + 2; k++) out[k] = e[k];
 }
 
-template <class P>
-__global__ void __launch_bounds__(kBlock) k_force(Dev D, P pos, double *contrib) {
+template <bool MIXED, class P>
+__global__ void __launch_bounds__(kBlock) k_force(Dev D, P pos, double *contrib_) {
+    auto *contrib = reinterpret_cast<typename SlotT<MIXED>::T *>(contrib_);
     const int total = D.n_items(), gs = gridDim.x * blockDim.x;
     double acc[kNFF];
     mdy_xsum xs;
     mdy_xs_zero(&xs);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs)
-        item(D, i, pos, acc, xs, contrib, false, true);
+        item<MIXED>(D, i, pos, acc, xs, contrib, false, true);
 }
 
 // Quadrature force (forceN, 10^4 x 10^4 intervals) of one queued pair,
@@ -96,9 +97,11 @@ __global__ void __launch_bounds__(kBlock) k_force_simpson_part(Dev D, const doub
 
 // flag[0] = 1 if the result was non-finite, flag[1] = pair in plane
 // (the host retries with moved beads or fails like the reference)
+template <bool MIXED>
 __global__ void __launch_bounds__(kBlock) k_force_simpson_final(Dev D, const double *x, int qi, int moved,
-                                                                const double *part, int nblk, double *contrib,
+                                                                const double *part, int nblk, double *contrib_,
                                                                 int *flag) {
+    auto *contrib = reinterpret_cast<typename SlotT<MIXED>::T *>(contrib_);
     __shared__ long long lds[17 * 8];
     const int N = 10000;
     double T[6];
@@ -128,7 +131,9 @@ __global__ void __launch_bounds__(kBlock) k_force_simpson_final(Dev D, const dou
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_gather(Dev D, const double *contrib, double *force, int accumulate) {
+template <bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_gather(Dev D, const double *contrib_, double *force, int accumulate) {
+    const auto *contrib = reinterpret_cast<const typename SlotT<MIXED>::T *>(contrib_);
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < D.nb; b += gridDim.x * blockDim.x) {
         d3 f = gather_bead(D, contrib, b);
         if (accumulate) {
@@ -169,7 +174,9 @@ int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hi
     Dev D = c.dev();
     const int nblk = grid_for(c, D.n_items());
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c.qn.p, 4);
-    hipLaunchKernelGGL(k_force<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
+    const bool mixed = c.opts.precision == MDY_PREC_MIXED;
+    if (mixed) hipLaunchKernelGGL((k_force<true, PosX>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
+    else hipLaunchKernelGGL((k_force<false, PosX>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
     if (hipGetLastError() != hipSuccess) return MDY_ERR_HIP;
     // quadrature fallbacks: rare; the count is read back to drive them
     int h_q[4];
@@ -184,8 +191,12 @@ int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hi
             for (int comp = 0; comp < 6; comp++)
                 hipLaunchKernelGGL(k_force_simpson_part, dim3(sblk), dim3(kBlock), 0, st, D, d_x, q, moved, comp,
                                    c.simpson_part.p);
-            hipLaunchKernelGGL(k_force_simpson_final, dim3(1), dim3(kBlock), 0, st, D, d_x, q, moved,
-                               c.simpson_part.p, sblk, c.contrib.p, c.qn.p + 2);
+            if (mixed)
+                hipLaunchKernelGGL(k_force_simpson_final<true>, dim3(1), dim3(kBlock), 0, st, D, d_x, q, moved,
+                                   c.simpson_part.p, sblk, c.contrib.p, c.qn.p + 2);
+            else
+                hipLaunchKernelGGL(k_force_simpson_final<false>, dim3(1), dim3(kBlock), 0, st, D, d_x, q, moved,
+                                   c.simpson_part.p, sblk, c.contrib.p, c.qn.p + 2);
             int fl[2];
             if (hipMemcpyAsync(fl, c.qn.p + 2, sizeof(fl), hipMemcpyDeviceToHost, st) != hipSuccess) return MDY_ERR_HIP;
             if (hipStreamSynchronize(st) != hipSuccess) return MDY_ERR_HIP;
@@ -193,7 +204,12 @@ int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hi
             if (moved || !fl[1]) return MDY_ERR_FORCE;  // the reference exits here
         }
     }
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_force, accumulate);
+    if (mixed)
+        hipLaunchKernelGGL(k_gather<true>, dim3(grid_for(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_force,
+                           accumulate);
+    else
+        hipLaunchKernelGGL(k_gather<false>, dim3(grid_for(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_force,
+                           accumulate);
     return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
 }
 

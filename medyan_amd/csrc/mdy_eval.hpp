@@ -68,22 +68,102 @@ __device__ __forceinline__ void enqueue(int *q, int *qn, int v, int *err) {
 __device__ __forceinline__ void put3(double *__restrict__ c, int slot, d3 v) {
     c[3 * slot] = v.x; c[3 * slot + 1] = v.y; c[3 * slot + 2] = v.z;
 }
+// mixed-precision force records: 3 floats per slot
+__device__ __forceinline__ void put3(float *__restrict__ c, int slot, d3 v) {
+    c[3 * slot] = (float)v.x; c[3 * slot + 1] = (float)v.y; c[3 * slot + 2] = (float)v.z;
+}
+__device__ __forceinline__ void put3(float *__restrict__ c, int slot, f3 v) {
+    c[3 * slot] = v.x; c[3 * slot + 1] = v.y; c[3 * slot + 2] = v.z;
+}
+// force-record element type of each form
+template <bool MIXED> struct SlotT { using T = double; };
+template <> struct SlotT<true> { using T = float; };
 
-// ---- one interaction: energy (into the exact total and, if PER_FF, into the
-// per-force-field double sums) and, if FORCES, its per-bead force slots. The
-// positions are loaded once for both. Written with one accumulation site and
-// one copy of the excluded-volume invariants and force, so the inlined code
-// stays small (it runs from the instruction cache of every CU).
-template <class P>
-__device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double acc[kNFF], mdy_xsum &xs,
-                                     double *__restrict__ contrib, const bool PER_FF, const bool FORCES) {
+// ---- the interaction sweep -------------------------------------------------
+// Every interaction is an item of the index space [EV | FS | FB | LM | BS | BB |
+// BD]. An item's record (its bead offsets and parameters) and its beads'
+// coordinates are fetched one item ahead of the one being computed, so each
+// thread's chain of dependent memory accesses overlaps with its arithmetic.
+enum { IT_EV = 0, IT_FS, IT_FB, IT_LM, IT_BS, IT_BB, IT_BD, IT_NONE };
+struct ItemRec {
+    int type, j;      // kind and index within its table
+    int4 b;           // bead offsets (x3); unused entries repeat the first
+    double p0, p1, p2, p3;
+};
+__device__ __forceinline__ ItemRec fetch_rec(const Dev &D, int i) {
+    ItemRec r;
+    r.p0 = r.p1 = r.p2 = r.p3 = 0.0;
+    if (i < D.n_ev) {
+        r.type = IT_EV; r.j = i;
+        r.b = D.ev_bs[i];
+        r.p0 = D.ev_kr[i];
+    } else if ((i -= D.n_ev) < D.n_fs) {
+        r.type = IT_FS; r.j = i;
+        const int2 b = D.fs_bs[i];
+        r.b = make_int4(b.x, b.y, b.x, b.x);
+        r.p0 = D.fs_k[i]; r.p1 = D.fs_l0[i];
+    } else if ((i -= D.n_fs) < D.n_fb) {
+        r.type = IT_FB; r.j = i;
+        const int *b = D.fb_bs + 3 * i;
+        r.b = make_int4(b[0], b[1], b[2], b[0]);
+        r.p0 = D.fb_k[i]; r.p1 = D.fb_th[i]; r.p2 = D.fb_ct[i]; r.p3 = D.fb_st[i];
+    } else if ((i -= D.n_fb) < D.n_lm) {
+        r.type = IT_LM; r.j = i;
+        r.b = D.lm_bs[i];
+        r.p0 = D.lm_k[i]; r.p1 = D.lm_l0[i]; r.p2 = D.lm_p1[i]; r.p3 = D.lm_p2[i];
+    } else if ((i -= D.n_lm) < D.n_bs) {
+        r.type = IT_BS; r.j = i;
+        const int *b = D.bs_bs + 3 * i;
+        r.b = make_int4(b[0], b[1], b[2], b[0]);
+        r.p0 = D.bs_k[i]; r.p1 = D.bs_l0[i]; r.p2 = D.bs_p[i];
+    } else if ((i -= D.n_bs) < D.n_bb) {
+        r.type = IT_BB; r.j = i;
+        r.b = D.bb_bs[i];
+        r.p0 = D.bb_k[i]; r.p1 = D.bb_th[i]; r.p2 = D.bb_ct[i]; r.p3 = D.bb_st[i];
+    } else if ((i -= D.n_bb) < D.n_bd) {
+        r.type = IT_BD; r.j = i;
+        const int bead = D.bd_bead[i];
+        r.b = make_int4(bead, D.bd_plane[i], bead, bead);
+        r.p0 = D.bd_k[i]; r.p1 = D.bd_sl[i];
+    } else {
+        r.type = IT_NONE; r.j = 0;
+        r.b = make_int4(0, 0, 0, 0);
+    }
+    return r;
+}
+__device__ __forceinline__ int rec_nbeads(int type) {
+    return type == IT_EV || type == IT_LM || type == IT_BB ? 4 : type == IT_FB || type == IT_BS ? 3
+         : type == IT_FS ? 2 : type == IT_BD ? 1 : 0;
+}
+// bead k's position (the unused entries of a record repeat its first bead)
+__device__ __forceinline__ d3 rec_pos(const PosAny &P, const ItemRec &r, int k) {
+    const int o = k == 0 ? r.b.x : k == 1 ? (r.type == IT_BD ? r.b.x : r.b.y) : k == 2 ? r.b.z : r.b.w;
+    return P(o);
+}
+
+// ---- one interaction from its record and positions: energy (into the exact
+// total and, if PER_FF, into the per-force-field double sums) and, if FORCES,
+// its per-bead force slots. One accumulation site and one copy of the
+// excluded-volume invariants and force keep the inlined code small (it runs
+// from the instruction cache of every CU).
+// Mixed-precision form: a pair's force runs in FP32 only when the squared
+// normalized triple product q = -JJ / (a b c) of the two cylinders is at
+// least this; nearer-coplanar pairs amplify FP32 rounding as ~6e-8 / q (at
+// 0.03 the FP32 pair force stays within ~6e-5 of its FP64 value, about 3/4 of
+// the pairs of a random network qualify).
+constexpr double kMixedMinQ = 0.03;
+
+template <bool MIXED = false>
+__device__ __forceinline__ void item_compute(const Dev &D, const ItemRec &r, const d3 c[4], double acc[kNFF],
+                                             mdy_xsum &xs, typename SlotT<MIXED>::T *__restrict__ contrib,
+                                             const bool PER_FF, const bool FORCES) {
     double U = 0.0;
     int ff = 0;
     bool add = true;
-    if (i < D.n_ev) {
-        int4 b = D.ev_bs[i];
-        const double kr = D.ev_kr[i];
-        const d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
+    const int j = r.j;
+    if (r.type == IT_EV) {
+        const double kr = r.p0;
+        const d3 c1 = c[0], c2 = c[1], c3 = c[2], c4 = c[3];
         ff = MDY_FF_EV;
         EvInv v;
         d3 c2e = c2;
@@ -100,84 +180,85 @@ __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double a
             }
         }
         if (ev_energy_bad(U)) {
-            enqueue(D.qe, D.qe_n, i, D.err);
+            enqueue(D.qe, D.qe_n, j, D.err);
             add = false;
         }
         if (FORCES) {
-            d3 f[4];
-            ev_force_inv(v, c1, c2e, c3, c4, kr, f);
-            if (!(finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]))) {
-                enqueue(D.qf, D.qf_n, i, D.err);
-            } else {
-                int s = D.slot_ev + 4 * i;
-                put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
+            bool done = false;
+            if constexpr (MIXED) {
+                if (-v.JJ / (v.a * v.b * v.c) >= kMixedMinQ) {
+                    // well-conditioned pair: derivative blocks in FP32
+                    f3 f[4];
+                    ev_force_inv_t<float>(v, c1, c2e, c3, c4, kr, f);
+                    if (finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3])) {
+                        int s = D.slot_ev + 4 * j;
+                        put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]);
+                        put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
+                        done = true;
+                    }
+                }
+            }
+            if (!done) {
+                d3 f[4];
+                ev_force_inv(v, c1, c2e, c3, c4, kr, f);
+                if (!(finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]))) {
+                    enqueue(D.qf, D.qf_n, j, D.err);
+                } else {
+                    int s = D.slot_ev + 4 * j;
+                    put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
+                }
             }
         }
-    } else if ((i -= D.n_ev) < D.n_fs) {
-        int2 b = D.fs_bs[i];
-        d3 c1 = pos(b.x), c2 = pos(b.y);
+    } else if (r.type == IT_FS) {
         ff = MDY_FF_FS;
-        U = fs_energy(c1, c2, D.fs_k[i], D.fs_l0[i]);
+        U = fs_energy(c[0], c[1], r.p0, r.p1);
         if (FORCES) {
             d3 f1, f2;
-            fs_force(c1, c2, D.fs_k[i], D.fs_l0[i], f1, f2);
-            int s = D.slot_fs + 2 * i;
+            fs_force(c[0], c[1], r.p0, r.p1, f1, f2);
+            int s = D.slot_fs + 2 * j;
             put3(contrib, s, f1); put3(contrib, s + 1, f2);
         }
-    } else if ((i -= D.n_fs) < D.n_fb) {
-        const int *b = D.fb_bs + 3 * i;
-        d3 c1 = pos(b[0]), c2 = pos(b[1]), c3 = pos(b[2]);
-        const double k = D.fb_k[i], th = D.fb_th[i], ct = D.fb_ct[i], st = D.fb_st[i];
+    } else if (r.type == IT_FB) {
         ff = MDY_FF_FB;
-        U = fb_energy(c1, c2, c3, k, th, ct, st);
+        U = fb_energy(c[0], c[1], c[2], r.p0, r.p1, r.p2, r.p3);
         if (FORCES) {
             d3 f1, f2, f3;
-            fb_force(c1, c2, c3, k, th, ct, st, f1, f2, f3);
-            int s = D.slot_fb + 3 * i;
+            fb_force(c[0], c[1], c[2], r.p0, r.p1, r.p2, r.p3, f1, f2, f3);
+            int s = D.slot_fb + 3 * j;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
         }
-    } else if ((i -= D.n_fb) < D.n_lm) {
-        int4 b = D.lm_bs[i];
-        d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
-        const double k = D.lm_k[i], l0 = D.lm_l0[i], p1 = D.lm_p1[i], p2 = D.lm_p2[i];
-        ff = (i < D.n_ls) ? MDY_FF_LS : MDY_FF_MS;
-        U = ls_energy(c1, c2, c3, c4, k, l0, p1, p2);
+    } else if (r.type == IT_LM) {
+        ff = (j < D.n_ls) ? MDY_FF_LS : MDY_FF_MS;
+        U = ls_energy(c[0], c[1], c[2], c[3], r.p0, r.p1, r.p2, r.p3);
         if (FORCES) {
             d3 f1, f2, f3, f4;
-            ls_force(c1, c2, c3, c4, k, l0, p1, p2, f1, f2, f3, f4);
-            int s = D.slot_lm + 4 * i;
+            ls_force(c[0], c[1], c[2], c[3], r.p0, r.p1, r.p2, r.p3, f1, f2, f3, f4);
+            int s = D.slot_lm + 4 * j;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
         }
-    } else if ((i -= D.n_lm) < D.n_bs) {
-        const int *b = D.bs_bs + 3 * i;
-        d3 c1 = pos(b[0]), c2 = pos(b[1]), c3 = pos(b[2]);
-        const double k = D.bs_k[i], l0 = D.bs_l0[i], p = D.bs_p[i];
+    } else if (r.type == IT_BS) {
         ff = MDY_FF_BS;
-        U = bs_energy(c1, c2, c3, k, l0, p);
+        U = bs_energy(c[0], c[1], c[2], r.p0, r.p1, r.p2);
         if (FORCES) {
             d3 f1, f2, f3;
-            bs_force(c1, c2, c3, k, l0, p, f1, f2, f3);
-            int s = D.slot_bs + 3 * i;
+            bs_force(c[0], c[1], c[2], r.p0, r.p1, r.p2, f1, f2, f3);
+            int s = D.slot_bs + 3 * j;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
         }
-    } else if ((i -= D.n_bs) < D.n_bb) {
-        int4 b = D.bb_bs[i];
-        d3 c1 = pos(b.x), c2 = pos(b.y), c3 = pos(b.z), c4 = pos(b.w);
-        const double k = D.bb_k[i], th = D.bb_th[i], ct = D.bb_ct[i], st = D.bb_st[i];
+    } else if (r.type == IT_BB) {
         ff = MDY_FF_BB;
-        U = bb_energy(c1, c2, c3, c4, k, th, ct, st);
+        U = bb_energy(c[0], c[1], c[2], c[3], r.p0, r.p1, r.p2, r.p3);
         if (FORCES) {
             d3 f1, f2, f3, f4;
-            bb_force(c1, c2, c3, c4, k, th, ct, st, f1, f2, f3, f4);
-            int s = D.slot_bb + 4 * i;
+            bb_force(c[0], c[1], c[2], c[3], r.p0, r.p1, r.p2, r.p3, f1, f2, f3, f4);
+            int s = D.slot_bb + 4 * j;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
         }
-    } else if ((i -= D.n_bb) < D.n_bd) {
-        const double *pl = D.planes + 4 * D.bd_plane[i];
-        d3 c = pos(D.bd_bead[i]);
+    } else if (r.type == IT_BD) {
+        const double *pl = D.planes + 4 * r.b.y;
         ff = MDY_FF_BD;
-        U = bd_energy(pl, c, D.bd_k[i], D.bd_sl[i]);
-        if (FORCES) put3(contrib, D.slot_bd + i, bd_force(pl, c, D.bd_k[i], D.bd_sl[i]));
+        U = bd_energy(pl, c[0], r.p0, r.p1);
+        if (FORCES) put3(contrib, D.slot_bd + j, bd_force(pl, c[0], r.p0, r.p1));
     } else {
         add = false;
     }
@@ -190,20 +271,56 @@ __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double a
     }
 }
 
+// All items i = start, start + stride, ... below D.n_items() at positions P.
+// (Fetching the next record ahead of the arithmetic was measured slower: the
+// extra live registers cost more than the memory round trip they hide.)
+template <bool MIXED = false>
+__device__ __forceinline__ void sweep(const Dev &D, const PosAny &P, int start, int stride, double acc[kNFF],
+                                      mdy_xsum &xs, typename SlotT<MIXED>::T *__restrict__ contrib,
+                                      const bool PER_FF, const bool FORCES) {
+    const int total = D.n_items();
+    for (int i = start; i < total; i += stride) {
+        const ItemRec r = fetch_rec(D, i);
+        const int nb = rec_nbeads(r.type);
+        d3 c[4];
+        c[0] = rec_pos(P, r, 0);
+        c[1] = nb > 1 ? rec_pos(P, r, 1) : c[0];
+        c[2] = nb > 2 ? rec_pos(P, r, 2) : c[0];
+        c[3] = nb > 3 ? rec_pos(P, r, 3) : c[0];
+        item_compute<MIXED>(D, r, c, acc, xs, contrib, PER_FF, FORCES);
+    }
+}
+
+// single item (one-shot kernels)
+template <bool MIXED = false, class P>
+__device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double acc[kNFF], mdy_xsum &xs,
+                                     typename SlotT<MIXED>::T *__restrict__ contrib, const bool PER_FF,
+                                     const bool FORCES) {
+    ItemRec r = fetch_rec(D, i);
+    const int nb = rec_nbeads(r.type);
+    d3 c[4];
+    c[0] = pos(r.b.x);
+    c[1] = nb > 1 ? pos(r.b.y) : c[0];
+    c[2] = nb > 2 ? pos(r.b.z) : c[0];
+    c[3] = nb > 3 ? pos(r.b.w) : c[0];
+    item_compute<MIXED>(D, r, c, acc, xs, contrib, PER_FF, FORCES);
+}
+
 // Per-bead force: the slots of bead b in reference order, starting from 0.0
 // (ForceFieldManager::computeForces zero-fills, then each force field adds).
 // The first kGS static and kGD dynamic slot indices and values are loaded
 // together before any addition, so a bead costs three dependent memory round
 // trips (offsets, indices, values) instead of two per slot.
 constexpr int kGS = 8, kGD = 8;
-__device__ __forceinline__ d3 gather_bead(const Dev &D, const double *__restrict__ contrib, int b) {
+template <class S>
+__device__ __forceinline__ d3 gather_bead(const Dev &D, const S *__restrict__ contrib, int b) {
     const int s0 = D.st_off[b], s1 = D.st_off[b + 1], d0 = D.dy_off[b], d1 = D.dy_off[b + 1];
     int is[kGS], id[kGD];
 #pragma unroll
     for (int j = 0; j < kGS; j++) is[j] = (s0 + j < s1) ? D.st_idx[s0 + j] : 0;
 #pragma unroll
     for (int j = 0; j < kGD; j++) id[j] = (d0 + j < d1) ? D.dy_idx[d0 + j] : 0;
-    double vs[kGS][3], vd[kGD][3];
+    S vs[kGS][3], vd[kGD][3];
 #pragma unroll
     for (int j = 0; j < kGS; j++) {
         if (s0 + j < s1) { vs[j][0] = contrib[3 * is[j]]; vs[j][1] = contrib[3 * is[j] + 1]; vs[j][2] = contrib[3 * is[j] + 2]; }
