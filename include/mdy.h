@@ -212,6 +212,12 @@ int mdy_forces_dev(mdy_ctx *ctx, const double *d_coord, double *d_force, int acc
 /* Hessian-vector product H v = d/dt grad E(x + t v) at t = 0 of the
  * exact-gradient force field (no coplanar perturbation); writes H v. */
 int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *d_hv, void *stream);
+/* Directional derivatives of the individual energies of the exact-gradient
+ * force field: d_out[k] = grad E_k(x) . w for the MDY_NFF force fields, then
+ * their sum. No reference counterpart (derivative of
+ * ForceFieldManager::computeIndividualEnergies, ForceFieldManager.cpp:209-227);
+ * used for parameter sensitivities of minimized structures. */
+int mdy_energy_jvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_w, double *d_out, void *stream);
 
 /* CGMethod::minimize<method> with backtracking line search
  * (src/Mechanics/Minimizer/CGMethod.hpp:826-1743), run on the device without

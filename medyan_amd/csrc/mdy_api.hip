@@ -419,6 +419,17 @@ int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *
     return MDY_OK;
 }
 
+int mdy_energy_jvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_w, double *d_out, void *stream) {
+    if (!ctx || !d_coord || !d_w || !d_out) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.have_sys || !c.have_nl) return fail(ctx, MDY_ERR_STATE, "network / neighbor list missing");
+    (void)hipSetDevice(c.device);
+    hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+    int rc = launch_energy_jvp(c, d_coord, d_w, d_out, st);
+    if (rc != MDY_OK) return fail(ctx, rc, "energy jvp failed");
+    return MDY_OK;
+}
+
 int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *p, double *d_coord, double *d_force_out,
                      mdy_min_result *result) {
     if (!ctx || !p || !d_coord || !result) return MDY_ERR_ARG;

@@ -7,6 +7,11 @@
 // are built with -ffp-contract=off; arctangent and exponential are the shared
 // mdy_atan / mdy_exp (mdy_xmath.h) and cos/sin of equilibrium angles are
 // precomputed by the host, so the FP64 results equal the CPU path's bits.
+//
+// The math is written once over a scalar type T: double (the FP64 path),
+// float (excluded-volume derivative blocks of the mixed-precision form) and
+// Dual, a forward-mode dual number whose tangent part turns a force
+// evaluation into a Hessian-vector product.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -16,55 +21,112 @@
 
 namespace mdy {
 
-struct d3 { double x, y, z; };
-struct f3 { float x, y, z; };   // force records of the mixed-precision form
-template <class R> struct vec3_of { using T = d3; };
-template <> struct vec3_of<float> { using T = f3; };
+template <class T>
+struct V3 { T x, y, z; };
+using d3 = V3<double>;
+using f3 = V3<float>;   // force records of the mixed-precision form
+template <class R> struct vec3_of { using T = V3<R>; };
+
+// ---- forward-mode dual number: value v and tangent d (d/dt at t = 0) -------
+struct Dual {
+    double v, d;
+    Dual() = default;
+    __device__ explicit Dual(double x) : v(x), d(0.0) {}
+    __device__ Dual(double x, double t) : v(x), d(t) {}
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+    const double q = a.v / b.v;
+    return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator+(Dual a, double b) { return Dual(a.v + b, a.d); }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return Dual(a + b.v, b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return Dual(a.v - b, a.d); }
+__device__ __forceinline__ Dual operator-(double a, Dual b) { return Dual(a - b.v, -b.d); }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+__device__ __forceinline__ Dual operator/(Dual a, double b) { return Dual(a.v / b, a.d / b); }
+__device__ __forceinline__ Dual operator/(double a, Dual b) {
+    const double q = a / b.v;
+    return Dual(q, -q * b.d / b.v);
+}
+__device__ __forceinline__ bool operator<(Dual a, Dual b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator>(Dual a, Dual b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator<(Dual a, double b) { return a.v < b; }
+__device__ __forceinline__ bool operator>(Dual a, double b) { return a.v > b; }
+__device__ __forceinline__ bool operator<(double a, Dual b) { return a < b.v; }
+
+// scalar functions for every T (distinct names: the C library's and the
+// shared ones stay the only candidates for double and float)
+__device__ __forceinline__ double xsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float xsqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ Dual xsqrt(Dual a) {
+    const double s = sqrt(a.v);
+    return Dual(s, a.d / (2 * s));
+}
+__device__ __forceinline__ double xfabs(double a) { return fabs(a); }
+__device__ __forceinline__ Dual xfabs(Dual a) { return a.v < 0 ? -a : a; }
+__device__ __forceinline__ double xatan(double a) { return mdy_atan(a); }
+__device__ __forceinline__ Dual xatan(Dual a) { return Dual(mdy_atan(a.v), a.d / (1 + a.v * a.v)); }
+__device__ __forceinline__ double xexp(double a) { return mdy_exp(a); }
+__device__ __forceinline__ Dual xexp(Dual a) {
+    const double e = mdy_exp(a.v);
+    return Dual(e, a.d * e);
+}
+__device__ __forceinline__ bool xfinite(double a) { return isfinite(a); }
+__device__ __forceinline__ bool xfinite(float a) { return isfinite(a); }
+__device__ __forceinline__ bool xfinite(Dual a) { return isfinite(a.v) && isfinite(a.d); }
+template <class R, class T> __device__ __forceinline__ R cvt(T a) { return (R)a; }
+template <> __device__ __forceinline__ Dual cvt<Dual, Dual>(Dual a) { return a; }
 
 __device__ __forceinline__ d3 ld3(const double *__restrict__ p) { return d3{p[0], p[1], p[2]}; }
-__device__ __forceinline__ d3 sub(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> __device__ __forceinline__ V3<T> sub(V3<T> a, V3<T> b) { return V3<T>{a.x - b.x, a.y - b.y, a.z - b.z}; }
 
 // std::max / std::min argument order (NaN handling identical to the host code)
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+template <class T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
 
 // (v2-v1).(v4-v3), MathFunctions.h:469-476
-__device__ __forceinline__ double sprod(d3 v1, d3 v2, d3 v3, d3 v4) {
+template <class T> __device__ __forceinline__ T sprod(V3<T> v1, V3<T> v2, V3<T> v3, V3<T> v4) {
     return ((v2.x - v1.x) * (v4.x - v3.x) + (v2.y - v1.y) * (v4.y - v3.y) + (v2.z - v1.z) * (v4.z - v3.z));
 }
-__device__ __forceinline__ double tpdist(d3 v1, d3 v2) {
-    return sqrt((v2.x - v1.x) * (v2.x - v1.x) + (v2.y - v1.y) * (v2.y - v1.y) + (v2.z - v1.z) * (v2.z - v1.z));
+template <class T> __device__ __forceinline__ T tpdist(V3<T> v1, V3<T> v2) {
+    return xsqrt((v2.x - v1.x) * (v2.x - v1.x) + (v2.y - v1.y) * (v2.y - v1.y) + (v2.z - v1.z) * (v2.z - v1.z));
 }
-__device__ __forceinline__ d3 midpt(d3 v1, d3 v2, double alpha) {
+template <class T> __device__ __forceinline__ V3<T> midpt(V3<T> v1, V3<T> v2, double alpha) {
     double beta = 1 - alpha;
-    return d3{(v1.x * beta + alpha * v2.x), (v1.y * beta + alpha * v2.y), (v1.z * beta + alpha * v2.z)};
+    return V3<T>{(v1.x * beta + alpha * v2.x), (v1.y * beta + alpha * v2.y), (v1.z * beta + alpha * v2.z)};
 }
-__device__ __forceinline__ d3 cross3(d3 v1, d3 v2) {
-    return d3{v1.y * v2.z - v1.z * v2.y, v1.z * v2.x - v1.x * v2.z, v1.x * v2.y - v1.y * v2.x};
+template <class T> __device__ __forceinline__ V3<T> cross3(V3<T> v1, V3<T> v2) {
+    return V3<T>{v1.y * v2.z - v1.z * v2.y, v1.z * v2.x - v1.x * v2.z, v1.x * v2.y - v1.y * v2.x};
 }
-__device__ __forceinline__ d3 normalize3(d3 v) {
-    double x2 = v.x * v.x, y2 = v.y * v.y, z2 = v.z * v.z;
-    double n = sqrt(x2 + y2 + z2);
-    return d3{v.x / n, v.y / n, v.z / n};
+template <class T> __device__ __forceinline__ V3<T> normalize3(V3<T> v) {
+    T x2 = v.x * v.x, y2 = v.y * v.y, z2 = v.z * v.z;
+    T n = xsqrt(x2 + y2 + z2);
+    return V3<T>{v.x / n, v.y / n, v.z / n};
 }
-__device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> __device__ __forceinline__ T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ bool are_equal0(double a) { return fabs(a - 0.0) < 1e-6; }
 
 // ---------------------------------------------------------------------------
 // a3 FilamentStretchingHarmonic (FilamentStretchingHarmonic.cpp:185-275)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double fs_energy(d3 c1, d3 c2, double k, double l0) {
-    double dist = tpdist(c1, c2) - l0;
+template <class T> __device__ __forceinline__ T fs_energy(V3<T> c1, V3<T> c2, double k, double l0) {
+    T dist = tpdist(c1, c2) - l0;
     return 0.5 * k * dist * dist;
 }
 // forces on c1, c2
-__device__ __forceinline__ void fs_force(d3 c1, d3 c2, double k, double l0, d3 &f1, d3 &f2) {
-    double dist = tpdist(c1, c2);
-    double invL = 1 / dist;
-    double f0 = k * (dist - l0) * invL;
-    double Fx = -f0 * (c2.x - c1.x), Fy = -f0 * (c2.y - c1.y), Fz = -f0 * (c2.z - c1.z);
-    f2 = d3{Fx, Fy, Fz};
-    f1 = d3{-Fx, -Fy, -Fz};
+template <class T>
+__device__ __forceinline__ void fs_force(V3<T> c1, V3<T> c2, double k, double l0, V3<T> &f1, V3<T> &f2) {
+    T dist = tpdist(c1, c2);
+    T invL = 1 / dist;
+    T f0 = k * (dist - l0) * invL;
+    T Fx = -f0 * (c2.x - c1.x), Fy = -f0 * (c2.y - c1.y), Fz = -f0 * (c2.z - c1.z);
+    f2 = V3<T>{Fx, Fy, Fz};
+    f1 = V3<T>{-Fx, -Fy, -Fz};
 }
 
 // ---------------------------------------------------------------------------
@@ -72,129 +134,143 @@ __device__ __forceinline__ void fs_force(d3 c1, d3 c2, double k, double l0, d3 &
 // ---------------------------------------------------------------------------
 // cos(theta0) and sin(theta0) arrive precomputed by the host C library (the
 // same values the CPU path uses): ct = cos(th0), st = sin(th0).
-__device__ __forceinline__ double cosbend_energy(double l1l2, double L1, double L2, double k, double th0, double ct,
-                                                 double st) {
-    double L1L2 = L1 * L2;
-    double x = l1l2 / L1L2;
-    if (x < -1.0) x = -1.0;
-    else if (x > 1.0) x = 1.0;
+template <class T>
+__device__ __forceinline__ T cosbend_energy(T l1l2, T L1, T L2, double k, double th0, double ct, double st) {
+    T L1L2 = L1 * L2;
+    T x = l1l2 / L1L2;
+    if (x < -1.0) x = T(-1.0);
+    else if (x > 1.0) x = T(1.0);
     if (are_equal0(th0)) return k * (1 - x);
-    double cosA = x;
-    double sinA = smax(sqrt(1 - cosA * cosA), 0.0);
-    double cosAmB = cosA * ct + sinA * st;
+    T cosA = x;
+    T sinA = smax(xsqrt(1 - cosA * cosA), T(0.0));
+    T cosAmB = cosA * ct + sinA * st;
     return k * (1 - cosAmB);
 }
-__device__ __forceinline__ void cosbend_coef(double L1sq, double L2sq, double l1l2, double kb, double th0, double ct,
-                                             double st, double &A, double &B, double &C, double &ke) {
-    A = 1 / sqrt(L1sq * L2sq);
-    double x = l1l2 * A;
+template <class T>
+__device__ __forceinline__ void cosbend_coef(T L1sq, T L2sq, T l1l2, double kb, double th0, double ct, double st,
+                                             T &A, T &B, T &C, T &ke) {
+    A = 1 / xsqrt(L1sq * L2sq);
+    T x = l1l2 * A;
     B = x / L1sq;
     C = x / L2sq;
-    if (are_equal0(th0)) { ke = kb; return; }
-    if (fabs(fabs(x) - 1.0) < 0.001) x = 0.999 * x;
-    if (x < -1.0) x = -1.0;
-    else if (x > 1.0) x = 1.0;
-    double cosp = x;
-    double sinp = sqrt(smax((1 - cosp * cosp), 0.0));
-    double sinpmq = sinp * ct - cosp * st;
+    if (are_equal0(th0)) { ke = T(kb); return; }
+    if (xfabs(xfabs(x) - 1.0) < 0.001) x = 0.999 * x;
+    if (x < -1.0) x = T(-1.0);
+    else if (x > 1.0) x = T(1.0);
+    T cosp = x;
+    T sinp = xsqrt(smax((1 - cosp * cosp), T(0.0)));
+    T sinpmq = sinp * ct - cosp * st;
     ke = kb * sinpmq / sinp;
 }
 // filament: beads c1, c2, c3; r1 = c2-c1, r2 = c3-c2
-__device__ __forceinline__ double fb_energy(d3 c1, d3 c2, d3 c3, double k, double th0, double ct, double st) {
-    double L1 = sqrt(sprod(c1, c2, c1, c2));
-    double L2 = sqrt(sprod(c2, c3, c2, c3));
-    double l1l2 = sprod(c1, c2, c2, c3);
+template <class T>
+__device__ __forceinline__ T fb_energy(V3<T> c1, V3<T> c2, V3<T> c3, double k, double th0, double ct, double st) {
+    T L1 = xsqrt(sprod(c1, c2, c1, c2));
+    T L2 = xsqrt(sprod(c2, c3, c2, c3));
+    T l1l2 = sprod(c1, c2, c2, c3);
     return cosbend_energy(l1l2, L1, L2, k, th0, ct, st);
 }
-__device__ __forceinline__ void fb_force(d3 c1, d3 c2, d3 c3, double k, double th0, double ct, double st, d3 &f1,
-                                         d3 &f2, d3 &f3) {
-    double A, B, C, ke;
+template <class T>
+__device__ __forceinline__ void fb_force(V3<T> c1, V3<T> c2, V3<T> c3, double k, double th0, double ct, double st,
+                                         V3<T> &f1, V3<T> &f2, V3<T> &f3) {
+    T A, B, C, ke;
     cosbend_coef(sprod(c1, c2, c1, c2), sprod(c2, c3, c2, c3), sprod(c1, c2, c2, c3), k, th0, ct, st, A, B, C, ke);
-    d3 r1 = sub(c2, c1), r2 = sub(c3, c2);
-    d3 F1 = d3{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
-    d3 F2 = d3{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
-    f1 = d3{-F1.x, -F1.y, -F1.z};
-    f2 = d3{F1.x - F2.x, F1.y - F2.y, F1.z - F2.z};
+    V3<T> r1 = sub(c2, c1), r2 = sub(c3, c2);
+    V3<T> F1 = V3<T>{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
+    V3<T> F2 = V3<T>{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
+    f1 = V3<T>{-F1.x, -F1.y, -F1.z};
+    f2 = V3<T>{F1.x - F2.x, F1.y - F2.y, F1.z - F2.z};
     f3 = F2;
 }
 // branch: mother (c1, c2), daughter (c3, c4)
-__device__ __forceinline__ double bb_energy(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0, double ct, double st) {
-    double L1 = sqrt(sprod(c1, c2, c1, c2));
-    double L2 = sqrt(sprod(c3, c4, c3, c4));
-    double l1l2 = sprod(c1, c2, c3, c4);
+template <class T>
+__device__ __forceinline__ T bb_energy(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double k, double th0, double ct,
+                                       double st) {
+    T L1 = xsqrt(sprod(c1, c2, c1, c2));
+    T L2 = xsqrt(sprod(c3, c4, c3, c4));
+    T l1l2 = sprod(c1, c2, c3, c4);
     return cosbend_energy(l1l2, L1, L2, k, th0, ct, st);
 }
-__device__ __forceinline__ void bb_force(d3 c1, d3 c2, d3 c3, d3 c4, double k, double th0, double ct, double st, d3 &f1,
-                                         d3 &f2, d3 &f3, d3 &f4) {
-    double A, B, C, ke;
+template <class T>
+__device__ __forceinline__ void bb_force(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double k, double th0, double ct,
+                                         double st, V3<T> &f1, V3<T> &f2, V3<T> &f3, V3<T> &f4) {
+    T A, B, C, ke;
     cosbend_coef(sprod(c1, c2, c1, c2), sprod(c3, c4, c3, c4), sprod(c1, c2, c3, c4), k, th0, ct, st, A, B, C, ke);
-    d3 r1 = sub(c2, c1), r2 = sub(c4, c3);
-    d3 F1 = d3{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
-    d3 F2 = d3{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
-    f1 = d3{-F1.x, -F1.y, -F1.z};
+    V3<T> r1 = sub(c2, c1), r2 = sub(c4, c3);
+    V3<T> F1 = V3<T>{ke * (r2.x * A - r1.x * B), ke * (r2.y * A - r1.y * B), ke * (r2.z * A - r1.z * B)};
+    V3<T> F2 = V3<T>{ke * (r1.x * A - r2.x * C), ke * (r1.y * A - r2.y * C), ke * (r1.z * A - r2.z * C)};
+    f1 = V3<T>{-F1.x, -F1.y, -F1.z};
     f2 = F1;
-    f3 = d3{-F2.x, -F2.y, -F2.z};
+    f3 = V3<T>{-F2.x, -F2.y, -F2.z};
     f4 = F2;
 }
 
 // ---------------------------------------------------------------------------
 // a5 / a6 linker and motor springs (LinkerStretchingHarmonic.cpp:191-336)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double ls_energy(d3 c1, d3 c2, d3 c3, d3 c4, double k, double l0, double p1, double p2) {
-    d3 v1 = midpt(c1, c2, p1), v2 = midpt(c3, c4, p2);
-    double dist = tpdist(v1, v2) - l0;
+template <class T>
+__device__ __forceinline__ T ls_energy(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double k, double l0, double p1,
+                                       double p2) {
+    V3<T> v1 = midpt(c1, c2, p1), v2 = midpt(c3, c4, p2);
+    T dist = tpdist(v1, v2) - l0;
     return 0.5 * k * dist * dist;
 }
-__device__ __forceinline__ void ls_force(d3 c1, d3 c2, d3 c3, d3 c4, double k, double l0, double p1, double p2, d3 &f1,
-                                         d3 &f2, d3 &f3, d3 &f4) {
-    d3 v1 = midpt(c1, c2, p1), v2 = midpt(c3, c4, p2);
-    double dist = tpdist(v1, v2);
-    double invL = 1 / dist;
-    double f0 = k * (dist - l0) * invL;
-    d3 Fl = d3{-f0 * (v2.x - v1.x), -f0 * (v2.y - v1.y), -f0 * (v2.z - v1.z)};
-    f1 = d3{-Fl.x * (1 - p1), -Fl.y * (1 - p1), -Fl.z * (1 - p1)};
-    f2 = d3{-Fl.x * (p1), -Fl.y * (p1), -Fl.z * (p1)};
-    f3 = d3{Fl.x * (1 - p2), Fl.y * (1 - p2), Fl.z * (1 - p2)};
-    f4 = d3{Fl.x * (p2), Fl.y * (p2), Fl.z * (p2)};
+template <class T>
+__device__ __forceinline__ void ls_force(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double k, double l0, double p1,
+                                         double p2, V3<T> &f1, V3<T> &f2, V3<T> &f3, V3<T> &f4) {
+    V3<T> v1 = midpt(c1, c2, p1), v2 = midpt(c3, c4, p2);
+    T dist = tpdist(v1, v2);
+    T invL = 1 / dist;
+    T f0 = k * (dist - l0) * invL;
+    V3<T> Fl = V3<T>{-f0 * (v2.x - v1.x), -f0 * (v2.y - v1.y), -f0 * (v2.z - v1.z)};
+    f1 = V3<T>{-Fl.x * (1 - p1), -Fl.y * (1 - p1), -Fl.z * (1 - p1)};
+    f2 = V3<T>{-Fl.x * (p1), -Fl.y * (p1), -Fl.z * (p1)};
+    f3 = V3<T>{Fl.x * (1 - p2), Fl.y * (1 - p2), Fl.z * (1 - p2)};
+    f4 = V3<T>{Fl.x * (p2), Fl.y * (p2), Fl.z * (p2)};
 }
 
 // ---------------------------------------------------------------------------
 // a7 BranchingStretchingHarmonic (BranchingStretchingHarmonic.cpp:195-318)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double bs_energy(d3 c1, d3 c2, d3 c3, double k, double l0, double p) {
-    d3 v1 = midpt(c1, c2, p);
-    double dist = tpdist(v1, c3) - l0;
+template <class T>
+__device__ __forceinline__ T bs_energy(V3<T> c1, V3<T> c2, V3<T> c3, double k, double l0, double p) {
+    V3<T> v1 = midpt(c1, c2, p);
+    T dist = tpdist(v1, c3) - l0;
     return 0.5 * k * dist * dist;
 }
-__device__ __forceinline__ void bs_force(d3 c1, d3 c2, d3 c3, double k, double l0, double p, d3 &f1, d3 &f2, d3 &f3) {
-    d3 v1 = midpt(c1, c2, p);
-    double dist = tpdist(v1, c3);
-    double invL = 1 / dist;
-    double f0 = k * (dist - l0) * invL;
-    d3 r = sub(c3, v1);
-    f1 = d3{-f0 * (r.x) * (p - 1), -f0 * (r.y) * (p - 1), -f0 * (r.z) * (p - 1)};
-    f2 = d3{f0 * (r.x) * p, f0 * (r.y) * p, f0 * (r.z) * p};
-    f3 = d3{-f0 * (r.x), -f0 * (r.y), -f0 * (r.z)};
+template <class T>
+__device__ __forceinline__ void bs_force(V3<T> c1, V3<T> c2, V3<T> c3, double k, double l0, double p, V3<T> &f1,
+                                         V3<T> &f2, V3<T> &f3) {
+    V3<T> v1 = midpt(c1, c2, p);
+    T dist = tpdist(v1, c3);
+    T invL = 1 / dist;
+    T f0 = k * (dist - l0) * invL;
+    V3<T> r = sub(c3, v1);
+    f1 = V3<T>{-f0 * (r.x) * (p - 1), -f0 * (r.y) * (p - 1), -f0 * (r.z) * (p - 1)};
+    f2 = V3<T>{f0 * (r.x) * p, f0 * (r.y) * p, f0 * (r.z) * p};
+    f3 = V3<T>{-f0 * (r.x), -f0 * (r.y), -f0 * (r.z)};
 }
 
 // ---------------------------------------------------------------------------
 // a10 BoundaryCylinderRepulsionExp (BoundaryCylinderRepulsionExp.cpp:27-110)
 // plane = (a, b, c, d); distance per BoundaryElementImpl.h:71-74
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double plane_dist(const double *__restrict__ pl, d3 p) {
+template <class T> __device__ __forceinline__ T plane_dist(const double *__restrict__ pl, V3<T> p) {
     double a = pl[0], b = pl[1], c = pl[2], d = pl[3];
     return (a * p.x + b * p.y + c * p.z + d) / sqrt(a * a + b * b + c * c);
 }
-__device__ __forceinline__ double bd_energy(const double *__restrict__ pl, d3 p, double k, double sl) {
-    double r = plane_dist(pl, p);
-    double R = -r / sl;
-    return k * mdy_exp(R);
+template <class T>
+__device__ __forceinline__ T bd_energy(const double *__restrict__ pl, V3<T> p, double k, double sl) {
+    T r = plane_dist(pl, p);
+    T R = -r / sl;
+    return k * xexp(R);
 }
-__device__ __forceinline__ d3 bd_force(const double *__restrict__ pl, d3 p, double k, double sl) {
-    double r = plane_dist(pl, p);
-    double R = -r / sl;
-    double f0 = k * mdy_exp(R) / sl;
-    return d3{f0 * pl[0], f0 * pl[1], f0 * pl[2]};
+template <class T>
+__device__ __forceinline__ V3<T> bd_force(const double *__restrict__ pl, V3<T> p, double k, double sl) {
+    T r = plane_dist(pl, p);
+    T R = -r / sl;
+    T f0 = k * xexp(R) / sl;
+    return V3<T>{f0 * pl[0], f0 * pl[1], f0 * pl[2]};
 }
 
 // ---------------------------------------------------------------------------
@@ -213,49 +289,52 @@ __device__ __forceinline__ d3 moved_off_plane(d3 p1, d3 p2, d3 p3, d3 src, doubl
     return d3{(src.x + n.x * d), (src.y + n.y * d), (src.z + n.z * d)};
 }
 
-struct EvInv {
-    double a, b, c, d, e, F;
-    double AA, BB, CC, DD, EE, FF, GG, HH, JJ;
-    double ATG1, ATG2, ATG3, ATG4;
+template <class T>
+struct EvInvT {
+    T a, b, c, d, e, F;
+    T AA, BB, CC, DD, EE, FF, GG, HH, JJ;
+    T ATG1, ATG2, ATG3, ATG4;
 };
+using EvInv = EvInvT<double>;
 
-__device__ __forceinline__ void ev_invariants(d3 c1, d3 c2, d3 c3, d3 c4, EvInv &v) {
+template <class T>
+__device__ __forceinline__ void ev_invariants(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, EvInvT<T> &v) {
     v.a = sprod(c1, c2, c1, c2);
     v.b = sprod(c3, c4, c3, c4);
     v.c = sprod(c3, c1, c3, c1);
     v.d = sprod(c1, c2, c3, c4);
     v.e = sprod(c1, c2, c3, c1);
     v.F = sprod(c3, c4, c3, c1);
-    double g = sprod(c1, c2, c1, c4);
-    double h = sprod(c3, c4, c3, c2);
-    double I = sprod(c3, c4, c1, c4);
-    double sqD, sqE;
+    T g = sprod(c1, c2, c1, c4);
+    T h = sprod(c3, c4, c3, c2);
+    T I = sprod(c3, c4, c1, c4);
+    T sqD, sqE;
     {
-        double x = c1.x - c4.x, y = c1.y - c4.y, z = c1.z - c4.z;
+        T x = c1.x - c4.x, y = c1.y - c4.y, z = c1.z - c4.z;
         sqD = (x * x + y * y + z * z);
         x = c3.x - c2.x; y = c3.y - c2.y; z = c3.z - c2.z;
         sqE = (x * x + y * y + z * z);
     }
-    const double a = v.a, b = v.b, c = v.c, d = v.d, e = v.e, F = v.F;
-    double ac = sqrt(a * c), bc = sqrt(b * c), aD = sqrt(a * sqD), bE = sqrt(b * sqE);
-    v.AA = sqrt(smax(0.0, (ac + e) * (ac - e)));
-    v.BB = sqrt(smax(0.0, (bc + F) * (bc - F)));
+    const T a = v.a, b = v.b, c = v.c, d = v.d, e = v.e, F = v.F;
+    T ac = xsqrt(a * c), bc = xsqrt(b * c), aD = xsqrt(a * sqD), bE = xsqrt(b * sqE);
+    v.AA = xsqrt(smax(T(0.0), (ac + e) * (ac - e)));
+    v.BB = xsqrt(smax(T(0.0), (bc + F) * (bc - F)));
     v.CC = d * e - a * F;
     v.DD = b * e - d * F;
-    v.EE = sqrt(smax(0.0, (aD + g) * (aD - g)));
-    v.FF = sqrt(smax(0.0, (bE + h) * (bE - h)));
+    v.EE = xsqrt(smax(T(0.0), (aD + g) * (aD - g)));
+    v.FF = xsqrt(smax(T(0.0), (bE + h) * (bE - h)));
     v.GG = d * g - a * I;
     v.HH = v.CC + v.GG - v.DD;
-    d3 cp = cross3(sub(c1, c2), sub(c3, c4));
-    double JJ = dot3(cp, sub(c3, c1));
+    V3<T> cp = cross3(sub(c1, c2), sub(c3, c4));
+    T JJ = dot3(cp, sub(c3, c1));
     v.JJ = -JJ * JJ;
-    v.ATG1 = mdy_atan((a + e) / v.AA) - mdy_atan(e / v.AA);
-    v.ATG2 = mdy_atan((a + e - d) / v.EE) - mdy_atan((e - d) / v.EE);
-    v.ATG3 = mdy_atan((F) / v.BB) - mdy_atan((F - b) / v.BB);
-    v.ATG4 = mdy_atan((d + F) / v.FF) - mdy_atan((d + F - b) / v.FF);
+    v.ATG1 = xatan((a + e) / v.AA) - xatan(e / v.AA);
+    v.ATG2 = xatan((a + e - d) / v.EE) - xatan((e - d) / v.EE);
+    v.ATG3 = xatan((F) / v.BB) - xatan((F - b) / v.BB);
+    v.ATG4 = xatan((d + F) / v.FF) - xatan((d + F - b) / v.FF);
 }
 
-__device__ __forceinline__ double ev_energy_inv(const EvInv &v, double kr) {
+template <class T> __device__ __forceinline__ T ev_energy_inv(const EvInvT<T> &v, double kr) {
     return 0.5 * kr / v.JJ * (v.CC / v.AA * v.ATG1 + v.GG / v.EE * v.ATG2 + v.DD / v.BB * v.ATG3 + v.HH / v.FF * v.ATG4);
 }
 __device__ __forceinline__ double ev_energy(d3 c1, d3 c2, d3 c3, d3 c4, double kr) {
@@ -267,17 +346,18 @@ __device__ __forceinline__ bool ev_energy_bad(double U) { return fabs(U) == INFI
 
 // Analytic pair force; f[0..3] = forces on c1..c4. Direction vectors use the
 // c2 passed in (the moved one in the coplanar case), as the reference does.
-// R = double: the FP64 path; R = float: the mixed-precision form, whose
-// derivative blocks run in FP32 from the FP64 invariants.
-template <class R>
-__device__ __forceinline__ void ev_force_inv_t(const EvInv &v, d3 c1, d3 c2, d3 c3, d3 c4, double kr_,
-                                               typename vec3_of<R>::T f[4]) {
-    const R a = (R)v.a, b = (R)v.b, c = (R)v.c, d = (R)v.d, e = (R)v.e, F = (R)v.F;
-    const R AA = (R)v.AA, BB = (R)v.BB, CC = (R)v.CC, DD = (R)v.DD, EE = (R)v.EE, FF = (R)v.FF, GG = (R)v.GG,
-            HH = (R)v.HH;
-    const R ATG1 = (R)v.ATG1, ATG2 = (R)v.ATG2, ATG3 = (R)v.ATG3, ATG4 = (R)v.ATG4;
-    const R kr = (R)kr_;
-    const R invJJ = 1 / (R)v.JJ;
+// R = T: the FP64 path (T = double) and the Hessian-vector product (T = Dual);
+// T = double, R = float: the mixed-precision form, whose derivative blocks run
+// in FP32 from the FP64 invariants.
+template <class R, class T>
+__device__ __forceinline__ void ev_force_inv_t(const EvInvT<T> &v, V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4,
+                                               double kr_, V3<R> f[4]) {
+    const R a = cvt<R>(v.a), b = cvt<R>(v.b), c = cvt<R>(v.c), d = cvt<R>(v.d), e = cvt<R>(v.e), F = cvt<R>(v.F);
+    const R AA = cvt<R>(v.AA), BB = cvt<R>(v.BB), CC = cvt<R>(v.CC), DD = cvt<R>(v.DD), EE = cvt<R>(v.EE),
+            FF = cvt<R>(v.FF), GG = cvt<R>(v.GG), HH = cvt<R>(v.HH);
+    const R ATG1 = cvt<R>(v.ATG1), ATG2 = cvt<R>(v.ATG2), ATG3 = cvt<R>(v.ATG3), ATG4 = cvt<R>(v.ATG4);
+    const R kr = R(kr_);
+    const R invJJ = 1 / cvt<R>(v.JJ);
     const R sumBlock = CC / AA * ATG1 + GG / EE * ATG2 + DD / BB * ATG3 + HH / FF * ATG4;
     const R enFactor = kr * invJJ / 2;
     const R A1 = AA * AA / (AA * AA + (a + e) * (a + e));
@@ -333,9 +413,10 @@ __device__ __forceinline__ void ev_force_inv_t(const EvInv &v, d3 c1, d3 c2, d3 
     dv[3][0] = enFactor * (sb * (J21) + (k21));
     dv[3][1] = enFactor * (sb * (J22) + (k22));
     dv[3][2] = enFactor * (sb * (J23) + (k23));
-    const d3 u1d = sub(c2, c1), u2d = sub(c4, c3), u3d = sub(c1, c3);
-    const R u1x = (R)u1d.x, u1y = (R)u1d.y, u1z = (R)u1d.z, u2x = (R)u2d.x, u2y = (R)u2d.y, u2z = (R)u2d.z;
-    const R u3x = (R)u3d.x, u3y = (R)u3d.y, u3z = (R)u3d.z;
+    const V3<T> u1d = sub(c2, c1), u2d = sub(c4, c3), u3d = sub(c1, c3);
+    const R u1x = cvt<R>(u1d.x), u1y = cvt<R>(u1d.y), u1z = cvt<R>(u1d.z);
+    const R u2x = cvt<R>(u2d.x), u2y = cvt<R>(u2d.y), u2z = cvt<R>(u2d.z);
+    const R u3x = cvt<R>(u3d.x), u3y = cvt<R>(u3d.y), u3z = cvt<R>(u3d.z);
 #pragma unroll
     for (int bi = 0; bi < 4; bi++) {
         f[bi].x = -(dv[bi][0] * u1x + dv[bi][1] * u2x + dv[bi][2] * u3x);
@@ -351,8 +432,9 @@ __device__ __forceinline__ void ev_force(d3 c1, d3 c2, d3 c3, d3 c4, double kr, 
     ev_invariants(c1, c2, c3, c4, v);
     ev_force_inv(v, c1, c2, c3, c4, kr, f);
 }
-__device__ __forceinline__ bool finite_d3(d3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
-__device__ __forceinline__ bool finite_d3(f3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+template <class T> __device__ __forceinline__ bool finite_d3(V3<T> v) {
+    return xfinite(v.x) && xfinite(v.y) && xfinite(v.z);
+}
 
 // Simpson integrands (CylinderExclVolRepulsion.h:53-72)
 __device__ __forceinline__ double ev_r2(double a, double b, double c, double d, double e, double F, double s, double t) {

@@ -14,6 +14,7 @@ int exclusive_scan(Ctx &c, const int *in, int *out, int n, hipStream_t st);  // 
 int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res);
 // mdy_hvp.hip
 int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st);
+int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_out, hipStream_t st);
 
 template <class T>
 int ensure(DBuf<T> &b, size_t n) {

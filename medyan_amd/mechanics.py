@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
         "mdy_energy_dev": (C.c_int, [vp, vp, vp, vp]),
         "mdy_forces_dev": (C.c_int, [vp, vp, vp, C.c_int, vp]),
         "mdy_hvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mdy_energy_jvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "mdy_minimize": (C.c_int, [vp, C.POINTER(_abi.CGParams), f64p, f64p, C.POINTER(_abi.MinResult)]),
         "mdy_minimize_dev": (C.c_int, [vp, C.POINTER(_abi.CGParams), vp, vp, C.POINTER(_abi.MinResult)]),
         "mdy_get_trace": (C.c_int, [vp, f64p, f64p, f64p, f64p, i32p, C.c_int64, C.POINTER(C.c_int64)]),
@@ -257,6 +258,16 @@ class Context:
     def forces_dev(self, d_coord: int, d_force: int, accumulate=True, stream: int = 0):
         self._check(self.lib.mdy_forces_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_force), 1 if accumulate else 0,
                                             C.c_void_p(stream) if stream else None))
+
+    def hvp_dev(self, d_coord: int, d_v: int, d_hv: int, stream: int = 0):
+        """H v of the exact-gradient force field at the device coordinates."""
+        self._check(self.lib.mdy_hvp_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_v), C.c_void_p(d_hv),
+                                         C.c_void_p(stream) if stream else None))
+
+    def energy_jvp_dev(self, d_coord: int, d_w: int, d_out: int, stream: int = 0):
+        """d_out[k] = grad E_k . w per force field, then the sum (MDY_NFF + 1 doubles)."""
+        self._check(self.lib.mdy_energy_jvp_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_w), C.c_void_p(d_out),
+                                                C.c_void_p(stream) if stream else None))
 
     def trace(self):
         n = C.c_int64()
