@@ -454,6 +454,49 @@ int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *p, double *d_coord, doub
     return MDY_OK;
 }
 
+int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params, double *const *d_coord,
+                       double *const *d_force_out, mdy_min_result *results, double *net_ms,
+                       int threads_per_block, void *stream, double *total_ms) {
+    if (!ctxs || n <= 0 || !params || !d_coord || !results) return MDY_ERR_ARG;
+    if (threads_per_block == 0) threads_per_block = 64;
+    if (threads_per_block != 64 && threads_per_block != 128 && threads_per_block != 256) {
+        if (ctxs[0]) ctxs[0]->c.msg = "threads_per_block must be 64, 128 or 256";
+        return MDY_ERR_ARG;
+    }
+    std::vector<Ctx *> cs(n);
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || !d_coord[i]) return MDY_ERR_ARG;
+        Ctx &c = ctxs[i]->c;
+        if (!c.have_sys || !c.have_nl) return fail(ctxs[0], MDY_ERR_STATE, "network / neighbor list missing");
+        if (c.device != ctxs[0]->c.device || c.opts.precision != ctxs[0]->c.opts.precision)
+            return fail(ctxs[0], MDY_ERR_ARG, "batched contexts must share device and precision");
+        if (params[i].method < MDY_CG_STEEPEST || params[i].method > MDY_CG_POLAK_RIBIERE)
+            return fail(ctxs[0], MDY_ERR_ARG, "method");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return fail(ctxs[0], MDY_ERR_ARG, "a context appears twice in the batch");
+        cs[i] = &c;
+    }
+    Ctx &c0 = ctxs[0]->c;
+    (void)hipSetDevice(c0.device);
+    hipStream_t st = stream ? (hipStream_t)stream : c0.stream;
+    int rc = run_minimize_batch(cs.data(), n, params, d_coord, d_force_out, results, net_ms, threads_per_block, st,
+                                total_ms);
+    if (rc == MDY_ERR_LAUNCH) return fail(ctxs[0], rc, "cooperative launch failed");
+    if (rc != MDY_OK) return hipfail(ctxs[0], "minimize batch");
+    int worst = MDY_OK;
+    for (int i = 0; i < n; i++) {
+        int e = 0;
+        (void)hipMemcpy(&e, cs[i]->err.p, sizeof(int), hipMemcpyDeviceToHost);
+        if (e && results[i].status == MDY_OK) results[i].status = e;
+        if (results[i].status != MDY_OK) {
+            ctxs[i]->c.msg = "minimization error in batch";
+            if (worst == MDY_OK) worst = results[i].status;
+        }
+    }
+    if (worst != MDY_OK) return fail(ctxs[0], worst, "a minimization of the batch failed (see results[].status)");
+    return MDY_OK;
+}
+
 int mdy_minimize(mdy_ctx *ctx, const mdy_cg_params *p, double *coord_inout, double *force_out,
                  mdy_min_result *result) {
     if (!ctx || !p || !coord_inout || !result) return MDY_ERR_ARG;

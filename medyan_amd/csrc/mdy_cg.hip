@@ -18,7 +18,10 @@
 #include "mdy_eval.hpp"
 #include "mdy_launch.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 namespace mdy {
 
@@ -44,6 +47,7 @@ struct CgArgs {
     int method, try_recover, energy_call_limit, force_call_limit;
     double max_dist, lambda_max, grad_tol, energy_rel_tol;
     long long spin_limit;
+    int grid;             // blocks of this minimization's cluster (batched launch)
 };
 
 // result record layout (doubles)
@@ -52,6 +56,7 @@ enum {
     R_EA = R_EB + kNFF + 1,  // energies after
     R_FLAGS = R_EA + kNFF + 1,
     R_NEC, R_NFC, R_NIT, R_GIN, R_ERC, R_NEE, R_NSM, R_NREC, R_STATUS, R_TRN, R_LAST, R_FOUT, R_NPH,
+    R_T0, R_T1,            // 100 MHz wall clock at the leader's start and end
     R_SIZE
 };
 
@@ -83,8 +88,10 @@ struct Cg {
     bool aborted = false;
     int status = MDY_OK;
 
-    __device__ Cg(const CgArgs &a, double *l) : A(a), lds(l) {
-        G = gridDim.x; blk = blockIdx.x; tid = threadIdx.x;
+    // blk: this block's index within the cluster of G blocks that runs the
+    // minimization (the whole grid, or one network's share of a batch)
+    __device__ Cg(const CgArgs &a, double *l, int blk_, int G_) : A(a), lds(l) {
+        G = G_; blk = blk_; tid = threadIdx.x;
         gtid = blk * blockDim.x + tid; gs = G * blockDim.x;
     }
     __device__ long long *ldsl() const { return reinterpret_cast<long long *>(lds); }
@@ -433,15 +440,13 @@ enum Stage {
     ST_REC_GATHER, ST_FINAL, ST_FINAL_DONE, ST_FINISH
 };
 
-// The arguments come by value: pointers read from the kernel-argument segment
-// are known to be global, so every access is a global (not flat) access.
+// One minimization run by a cluster of G blocks (blk = index within it).
 template <bool MIXED>
-__global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
-    __shared__ double lds[17 * 16];
-    const CgArgs &A = Av;
-    Cg<MIXED> K(A, lds);
+__device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *lds, int blk, int G) {
+    Cg<MIXED> K(A, lds, blk, G);
     double *R = A.out;
     const bool leader = (K.blk == 0 && K.tid == 0);
+    if (leader) R[R_T0] = (double)wall_clock64();
 #define FB(i) ((i) ? A.F1 : A.F0)
 #define SB(i) ((i) ? A.S1 : A.S0)
     unsigned flags = 0;
@@ -688,6 +693,7 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
         R[R_FLAGS] = flags; R[R_NEC] = nEC; R[R_NFC] = nFC; R[R_NIT] = nIter; R[R_GIN] = gradInf; R[R_ERC] = relE;
         R[R_NEE] = nEE; R[R_NSM] = nSM; R[R_NREC] = nRec; R[R_STATUS] = K.status; R[R_TRN] = (double)trn;
         R[R_LAST] = after ? 1.0 : 0.0; R[R_FOUT] = (double)cur; R[R_NPH] = (double)K.ph;
+        R[R_T1] = (double)wall_clock64();
     }
     return;
 aborted:
@@ -696,9 +702,39 @@ aborted:
 #undef SB
 }
 
+// The arguments come by value: pointers read from the kernel-argument segment
+// are known to be global, so every access is a global (not flat) access.
+template <bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
+    __shared__ double lds[17 * 16];
+    cg_run<MIXED>(Av, lds, blockIdx.x, gridDim.x);
+}
+
+// Batched form: independent minimizations in one cooperative launch. Block b
+// belongs to network map[b].x as its block map[b].y; each network's cluster
+// runs its own state machine with its own barrier, reductions and buffers.
+// Pointers loaded from generic memory are flat to the compiler; loaded from
+// the constant address space (as kernel arguments are) they are known to be
+// global, so the arguments are read through a constant-space view and every
+// access through them uses global instructions, as in the single-network
+// kernel.
+template <bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ all, const int2 *__restrict__ map) {
+    __shared__ double lds[17 * 16];
+    const int2 m = map[blockIdx.x];
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const CgArgs ConstArgs;
+    CgArgs A = ((ConstArgs *)all)[m.x];
+#else
+    CgArgs A = all[m.x];
+#endif
+    cg_run<MIXED>(A, lds, m.y, A.grid);
+}
+
 // ---------------------------------------------------------------------------
-int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res) {
-    hipStream_t st = c.stream;
+// Buffers and arguments of one minimization run by a cluster of G blocks on
+// stream st (the counters and flags it relies on are cleared there).
+static int prepare_cg(Ctx &c, const mdy_cg_params *p, double *d_x, int G, hipStream_t st, CgArgs &A) {
     const int ndof = 3 * c.nb;
     MDY_TRY(ensure(c.F0, ndof)); MDY_TRY(ensure(c.F1, ndof));
     MDY_TRY(ensure(c.S0, ndof)); MDY_TRY(ensure(c.S1, ndof));
@@ -712,15 +748,6 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     MDY_TRY(ensure(c.tr_lam, c.trace_cap + 1)); MDY_TRY(ensure(c.tr_e, c.trace_cap + 1));
     MDY_TRY(ensure(c.tr_mf, c.trace_cap + 1)); MDY_TRY(ensure(c.tr_beta, c.trace_cap + 1));
     MDY_TRY(ensure(c.tr_safe, c.trace_cap + 1));
-    // persistent grid: all blocks resident
-    int per_cu = 0;
-    const bool mixed = c.opts.precision == MDY_PREC_MIXED;
-    const void *kern = mixed ? reinterpret_cast<const void *>(k_cg<true>) : reinterpret_cast<const void *>(k_cg<false>);
-    MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
-    if (per_cu < 1) return MDY_ERR_LAUNCH;
-    if (c.opts.blocks_per_cu > 0 && c.opts.blocks_per_cu < per_cu) per_cu = c.opts.blocks_per_cu;
-    const int G = per_cu * c.num_cu;
-    c.coop_blocks = G;
     MDY_TRY(ensure(c.bar, 64 + 16 * (size_t)G));
     MDY_TRY(ensure(c.acc, 3 * (size_t)kAccWords));
     MDY_TRY(ensure(c.red, (size_t)3 * G * kRed));
@@ -730,7 +757,6 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     MDY_HIP(hipMemsetAsync(c.acc.p, 0, 3 * kAccWords * sizeof(unsigned long long), st));
     MDY_HIP(hipMemsetAsync(c.err.p, 0, sizeof(int), st));
     if (c.prof_cap > 0) MDY_HIP(hipMemsetAsync(c.prof.p, 0, (5 * (size_t)c.prof_cap + 5) * sizeof(long long), st));
-    CgArgs A;
     A.D = c.dev();
     A.x = d_x; A.F0 = c.F0.p; A.F1 = c.F1.p; A.S0 = c.S0.p; A.S1 = c.S1.p; A.xmin = c.xmin.p;
     A.contrib = c.contrib.p; A.red = c.red.p; A.qe_u = c.qe_u.p; A.spart = c.simpson_part.p;
@@ -743,6 +769,53 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     A.max_dist = p->max_dist; A.lambda_max = p->lambda_max; A.grad_tol = p->grad_tol;
     A.energy_rel_tol = p->energy_rel_tol;
     A.spin_limit = 20000000LL;   // polls (about 10 s) before the watchdog fires
+    A.grid = G;
+    return MDY_OK;
+}
+
+// MinimizationResult from a result record
+static void collect(Ctx &c, const double *R, const mdy_cg_params *p, const CgArgs &A, mdy_min_result *res) {
+    std::memset(res, 0, sizeof(*res));
+    for (int k = 0; k <= kNFF; k++) { res->energies_before[k] = R[R_EB + k]; res->energies_after[k] = R[R_EA + k]; }
+    res->error_flags = (uint32_t)R[R_FLAGS];
+    res->num_energy_call = (int32_t)R[R_NEC];
+    res->energy_call_limit = p->energy_call_limit;
+    res->num_force_call = (int32_t)R[R_NFC];
+    res->force_call_limit = A.force_call_limit;
+    res->num_iter = (int32_t)R[R_NIT];
+    res->grad_inf_norm = R[R_GIN];
+    res->grad_tol = p->grad_tol;
+    res->energy_rel_change = R[R_ERC];
+    res->energy_rel_tol = p->energy_rel_tol;
+    res->num_energy_eval = (int32_t)R[R_NEE];
+    res->num_safe_mode = (int32_t)R[R_NSM];
+    res->num_recovery = (int32_t)R[R_NREC];
+    res->status = (int32_t)R[R_STATUS];
+    c.trace_n = (long long)R[R_TRN];
+    c.prof_n = (long long)R[R_NPH];
+}
+
+static int copy_force_out(Ctx &c, const double *R, double *d_force_out, hipStream_t st) {
+    if (d_force_out && R[R_LAST] != 0.0) {
+        const double *src = ((int)R[R_FOUT] == 0) ? c.F0.p : c.F1.p;
+        MDY_HIP(hipMemcpyAsync(d_force_out, src, sizeof(double) * 3 * (size_t)c.nb, hipMemcpyDeviceToDevice, st));
+    }
+    return MDY_OK;
+}
+
+int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res) {
+    hipStream_t st = c.stream;
+    // persistent grid: all blocks resident
+    int per_cu = 0;
+    const bool mixed = c.opts.precision == MDY_PREC_MIXED;
+    const void *kern = mixed ? reinterpret_cast<const void *>(k_cg<true>) : reinterpret_cast<const void *>(k_cg<false>);
+    MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+    if (per_cu < 1) return MDY_ERR_LAUNCH;
+    if (c.opts.blocks_per_cu > 0 && c.opts.blocks_per_cu < per_cu) per_cu = c.opts.blocks_per_cu;
+    const int G = per_cu * c.num_cu;
+    c.coop_blocks = G;
+    CgArgs A;
+    MDY_TRY(prepare_cg(c, p, d_x, G, st, A));
     hipEvent_t e0, e1;
     MDY_HIP(hipEventCreate(&e0));
     MDY_HIP(hipEventCreate(&e1));
@@ -762,30 +835,107 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     c.last_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    std::memset(res, 0, sizeof(*res));
-    for (int k = 0; k <= kNFF; k++) { res->energies_before[k] = R[R_EB + k]; res->energies_after[k] = R[R_EA + k]; }
-    res->error_flags = (uint32_t)R[R_FLAGS];
-    res->num_energy_call = (int32_t)R[R_NEC];
-    res->energy_call_limit = p->energy_call_limit;
-    res->num_force_call = (int32_t)R[R_NFC];
-    res->force_call_limit = A.force_call_limit;
-    res->num_iter = (int32_t)R[R_NIT];
-    res->grad_inf_norm = R[R_GIN];
-    res->grad_tol = p->grad_tol;
-    res->energy_rel_change = R[R_ERC];
-    res->energy_rel_tol = p->energy_rel_tol;
-    res->num_energy_eval = (int32_t)R[R_NEE];
-    res->num_safe_mode = (int32_t)R[R_NSM];
-    res->num_recovery = (int32_t)R[R_NREC];
-    res->status = (int32_t)R[R_STATUS];
-    c.trace_n = (long long)R[R_TRN];
-    c.prof_n = (long long)R[R_NPH];
-    if (d_force_out && R[R_LAST] != 0.0) {
-        const double *src = ((int)R[R_FOUT] == 0) ? c.F0.p : c.F1.p;
-        MDY_HIP(hipMemcpyAsync(d_force_out, src, sizeof(double) * ndof, hipMemcpyDeviceToDevice, st));
-        MDY_HIP(hipStreamSynchronize(st));
-    }
+    collect(c, R, p, A, res);
+    MDY_TRY(copy_force_out(c, R, d_force_out, st));
+    MDY_HIP(hipStreamSynchronize(st));
     return res->status == MDY_ERR_TIMEOUT ? MDY_ERR_TIMEOUT : MDY_OK;
+}
+
+// Batched minimizations: the resident blocks (threads_per_block threads each)
+// are split into one cluster per network, in proportion to its per-evaluation
+// work (interactions + beads), at least one block each; networks beyond the
+// resident block count run in further launches. cs[i] owns network i's
+// tables, neighbor lists and CG buffers; c0 (= cs[0]) the launch arrays.
+int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d_x, double *const *d_force_out,
+                       mdy_min_result *res, double *net_ms, int threads_per_block, hipStream_t st, double *total_ms) {
+    Ctx &c0 = *cs[0];
+    const bool mixed = c0.opts.precision == MDY_PREC_MIXED;
+    const void *kern = mixed ? reinterpret_cast<const void *>(k_cg_batch<true>)
+                             : reinterpret_cast<const void *>(k_cg_batch<false>);
+    const int tpb = threads_per_block;
+    int per_cu = 0;
+    MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, tpb, 0));
+    if (per_cu < 1) return MDY_ERR_LAUNCH;
+    const int Gmax = per_cu * c0.num_cu;
+    hipEvent_t e0, e1;
+    MDY_HIP(hipEventCreate(&e0));
+    MDY_HIP(hipEventCreate(&e1));
+    MDY_HIP(hipEventRecord(e0, st));
+    std::vector<CgArgs> args;
+    std::vector<int2> map;
+    std::vector<int> gsz;
+    int rc = MDY_OK;
+    for (int lo = 0; lo < n && rc == MDY_OK; lo += Gmax) {
+        const int hi = (n - lo < Gmax) ? n : lo + Gmax;
+        const int m = hi - lo;
+        // cluster sizes: proportional shares of the blocks left after one each
+        double tot = 0;
+        std::vector<double> w(m);
+        for (int i = 0; i < m; i++) {
+            const Ctx &ci = *cs[lo + i];
+            w[i] = (double)ci.dev().n_items() + 2.0 * ci.nb;
+            tot += w[i];
+        }
+        gsz.assign(m, 1);
+        const int spare = Gmax - m;
+        int used = 0;
+        std::vector<std::pair<double, int>> frac;
+        for (int i = 0; i < m; i++) {
+            const double share = tot > 0 ? spare * w[i] / tot : 0.0;
+            const int whole = (int)share;
+            gsz[i] += whole;
+            used += whole;
+            frac.push_back({share - whole, i});
+        }
+        std::sort(frac.begin(), frac.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+            return a.first > b.first || (a.first == b.first && a.second < b.second);
+        });
+        for (int k = 0; k < spare - used && k < m; k++) gsz[frac[k].second]++;
+        args.assign(m, CgArgs{});
+        map.clear();
+        for (int i = 0; i < m && rc == MDY_OK; i++) {
+            rc = prepare_cg(*cs[lo + i], &p[lo + i], d_x[lo + i], gsz[i], st, args[i]);
+            for (int b = 0; b < gsz[i]; b++) map.push_back(make_int2(i, b));
+        }
+        if (rc != MDY_OK) break;
+        const int G = (int)map.size();
+        rc = ensure(c0.batch_args, (size_t)m * sizeof(CgArgs));
+        if (rc == MDY_OK) rc = ensure(c0.batch_map, (size_t)G);
+        if (rc != MDY_OK) break;
+        if (hipMemcpyAsync(c0.batch_args.p, args.data(), (size_t)m * sizeof(CgArgs), hipMemcpyHostToDevice, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(c0.batch_map.p, map.data(), (size_t)G * sizeof(int2), hipMemcpyHostToDevice, st) !=
+                hipSuccess) {
+            rc = MDY_ERR_HIP;
+            break;
+        }
+        const CgArgs *pa = reinterpret_cast<const CgArgs *>(c0.batch_args.p);
+        const int2 *pm = c0.batch_map.p;
+        void *kargs[] = {&pa, &pm};
+        if (hipLaunchCooperativeKernel(kern, dim3(G), dim3(tpb), kargs, 0, st) != hipSuccess) {
+            rc = MDY_ERR_LAUNCH;
+            break;
+        }
+        // the host arrays are reused by the next chunk: wait for the copies
+        if (hipStreamSynchronize(st) != hipSuccess) { rc = MDY_ERR_HIP; break; }
+        for (int i = 0; i < m && rc == MDY_OK; i++) {
+            Ctx &ci = *cs[lo + i];
+            double R[R_SIZE];
+            if (hipMemcpy(R, ci.cg_out.p, sizeof(R), hipMemcpyDeviceToHost) != hipSuccess) { rc = MDY_ERR_HIP; break; }
+            collect(ci, R, &p[lo + i], args[i], &res[lo + i]);
+            if (net_ms) net_ms[lo + i] = (R[R_T1] - R[R_T0]) * 1e-5;   // 100 MHz ticks
+            ci.coop_blocks = gsz[i];
+            rc = copy_force_out(ci, R, d_force_out ? d_force_out[lo + i] : nullptr, st);
+        }
+    }
+    (void)hipEventRecord(e1, st);
+    (void)hipStreamSynchronize(st);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (total_ms) *total_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 }  // namespace mdy

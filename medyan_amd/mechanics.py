@@ -67,6 +67,8 @@ def load_library(path: str = LIB_PATH):
         "mdy_energy_jvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "mdy_minimize": (C.c_int, [vp, C.POINTER(_abi.CGParams), f64p, f64p, C.POINTER(_abi.MinResult)]),
         "mdy_minimize_dev": (C.c_int, [vp, C.POINTER(_abi.CGParams), vp, vp, C.POINTER(_abi.MinResult)]),
+        "mdy_minimize_batch": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(_abi.CGParams), C.POINTER(vp),
+                                         C.POINTER(vp), C.POINTER(_abi.MinResult), f64p, C.c_int, vp, f64p]),
         "mdy_get_trace": (C.c_int, [vp, f64p, f64p, f64p, f64p, i32p, C.c_int64, C.POINTER(C.c_int64)]),
         "mdy_set_trace": (C.c_int, [vp, C.c_int64]),
         "mdy_last_timing": (C.c_int, [vp, f64p]),
@@ -310,6 +312,31 @@ class Context:
         t = C.c_double()
         self._check(self.lib.mdy_last_timing(self.h, C.byref(t)))
         return t.value
+
+
+def minimize_batch(ctxs, params, d_coords, d_forces=None, threads_per_block=64, stream=0):
+    """Batched minimization (mdy_minimize_batch): ctxs[i] holds network i with
+    its neighbor lists; d_coords[i] its device coordinates (updated in place).
+    params: one CGParams / ConjugateGradientParams for all, or one per
+    network. Returns (results, per-network ms, total ms)."""
+    n = len(ctxs)
+    lib = load_library()
+    if not isinstance(params, (list, tuple)):
+        params = [params] * n
+    ps = (_abi.CGParams * n)()
+    for i, p in enumerate(params):
+        ps[i] = p.to_c(ctxs[i].net.n_beads) if isinstance(p, ConjugateGradientParams) else p
+    hs = (C.c_void_p * n)(*[c.h.value if isinstance(c.h, C.c_void_p) else c.h for c in ctxs])
+    xs = (C.c_void_p * n)(*[int(d) for d in d_coords])
+    fs = (C.c_void_p * n)(*[int(d) if d else None for d in d_forces]) if d_forces is not None else None
+    res = (_abi.MinResult * n)()
+    ms = np.zeros(n)
+    tot = C.c_double()
+    rc = lib.mdy_minimize_batch(hs, n, ps, xs, fs, res, _f64(ms), int(threads_per_block),
+                                C.c_void_p(stream) if stream else None, C.byref(tot))
+    if rc != 0:
+        raise MdyError(rc, lib.mdy_last_error(ctxs[0].h).decode())
+    return list(res), ms, tot.value
 
 
 class ForceFieldManager:

@@ -1,6 +1,11 @@
 // Drives the C++ mirror of MEDYAN's mechanics interface (include/mdy_medyan.hpp)
 // the way MController::run drives the reference (Controller/MController.h:109-211):
-// vectorize, energies, forces, minimize. Needs a HIP device. Prints "ALL OK".
+// vectorize, energies, forces, minimize. Needs a HIP device.
+// Built as a shared object whose mdy_shim_run() returns 0 when every check
+// holds (a failed check returns its line number; an escaped exception -1),
+// so it runs inside the test process: a second process on the same GPU
+// could not make the cooperative launches the minimizer uses while the test
+// process holds the device's cooperative-launch resource.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -9,17 +14,17 @@
 
 #include "mdy_medyan.hpp"
 
-#define REQUIRE(c)                                                         \
-    do {                                                                   \
-        if (!(c)) {                                                        \
+#define REQUIRE(c)                                                              \
+    do {                                                                        \
+        if (!(c)) {                                                             \
             std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #c); \
-            std::exit(1);                                                  \
-        }                                                                  \
+            return __LINE__;                                                    \
+        }                                                                       \
     } while (0)
 
 using namespace mdy_medyan;
 
-int main() {
+static int run() {
     mdy_gen_params gp;
     mdy_gen_defaults(&gp);
     gp.n_filaments = 20;
@@ -75,4 +80,13 @@ int main() {
     mdy_network_free(net);
     std::printf("ALL OK: %d CG iterations, max|F| %.4f pN\n", r.numIter, r.gradInfNorm);
     return 0;
+}
+
+extern "C" int mdy_shim_run() {
+    try {
+        return run();
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "FAILED: exception %s\n", e.what());
+        return -1;
+    }
 }

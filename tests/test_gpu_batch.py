@@ -1,0 +1,113 @@
+"""Batched minimization (mdy_minimize_batch): independent networks in one
+cooperative launch, one cluster of blocks per network.
+
+Every decision of the minimizer comes from exact sums and a fixed-order force
+gather, so a network's trajectory does not depend on how many blocks run it:
+each batched result must equal the single-network minimization bit for bit
+(coordinates, forces, total energy, call counts), in FP64 and in the mixed
+form. The per-force-field energy reports are plain double sums whose
+association follows the block count; they agree to 1e-12.
+The single-network path is itself pinned to the oracle bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from medyan_amd import _abi, ensemble, mechanics, network
+
+pytestmark = pytest.mark.gpu
+NFF = _abi.MDY_NFF
+
+
+def nets_small(lib):
+    return [network.generate(lib, "net500"),
+            network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=2),
+            network.generate(lib, "net500", n_filaments=60, cyl_per_filament=3, box=500.0, n_linkers=40,
+                             n_motors=4, seed=3),
+            ensemble.ensemble_network(lib, 1200, 0),
+            ensemble.ensemble_network(lib, 3000, 1),
+            network.generate(lib, None, seed=11, n_filaments=3, cyl_per_filament=4, box=800.0, n_linkers=2,
+                             n_motors=0, n_branches=0)]
+
+
+def ctx_for(net, precision):
+    c = mechanics.Context(precision=precision)
+    c.set_network(net)
+    c.build_neighbors()
+    return c
+
+
+def same(a, b):
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@pytest.mark.parametrize("precision,tpb", [(_abi.MDY_PREC_FP64, 64), (_abi.MDY_PREC_FP64, 256),
+                                           (_abi.MDY_PREC_MIXED, 128)])
+def test_batch_equals_single_minimizations(lib, precision, tpb):
+    nets = nets_small(lib)
+    print(f"[gpu] batch of {len(nets)}, precision {precision}, {tpb} threads per block", flush=True)
+    singles = []
+    for net in nets:
+        c = ctx_for(net, precision)
+        x, f, r = c.minimize(_abi.default_cg_params(net.n_beads))
+        singles.append((x, f, r))
+    ctxs = [ctx_for(net, precision) for net in nets]
+    xs = [torch.tensor(net.coord, device="cuda") for net in nets]
+    fs = [torch.zeros_like(x) for x in xs]
+    params = [_abi.default_cg_params(net.n_beads) for net in nets]
+    res, ms, tot = mechanics.minimize_batch(ctxs, params, [x.data_ptr() for x in xs], [f.data_ptr() for f in fs],
+                                            threads_per_block=tpb)
+    torch.cuda.synchronize()
+    for i, (x1, f1, r1) in enumerate(singles):
+        r = res[i]
+        assert r.status == 0 and r.error_flags == r1.error_flags
+        assert (r.num_iter, r.num_energy_call, r.num_force_call) == (r1.num_iter, r1.num_energy_call,
+                                                                     r1.num_force_call)
+        # the total is an exact sum; the per-force-field reports are plain
+        # double sums whose order follows the cluster's block count
+        assert r.energies_after[NFF] == r1.energies_after[NFF]
+        np.testing.assert_allclose(np.array(r.energies_after[:NFF]), np.array(r1.energies_after[:NFF]),
+                                   rtol=1e-12, atol=1e-12 * abs(r1.energies_after[NFF]))
+        assert r.grad_inf_norm == r1.grad_inf_norm
+        assert same(xs[i].cpu().numpy(), x1), f"network {i}: coordinates differ"
+        assert same(fs[i].cpu().numpy(), f1), f"network {i}: forces differ"
+        assert 0 < ms[i] <= tot * 1.01
+    print(f"[gpu] batch {tot:.1f} ms; per network {np.round(ms, 2)}")
+
+
+def test_batch_larger_than_resident_blocks(lib):
+    """More networks than resident blocks: the call runs several launches."""
+    base = [network.generate(lib, None, seed=100 + i, n_filaments=3, cyl_per_filament=3, box=700.0, n_linkers=2,
+                             n_motors=0, n_branches=0) for i in range(4)]
+    n = 256 + 8   # above one 256-thread block per CU on 256 CUs
+    nets = [base[i % 4] for i in range(n)]
+    ctxs = [ctx_for(net, _abi.MDY_PREC_FP64) for net in nets]
+    xs = [torch.tensor(net.coord, device="cuda") for net in nets]
+    params = [_abi.default_cg_params(net.n_beads) for net in nets]
+    res, ms, tot = mechanics.minimize_batch(ctxs, params, [x.data_ptr() for x in xs], threads_per_block=256)
+    torch.cuda.synchronize()
+    ref = [ctx_for(b, _abi.MDY_PREC_FP64).minimize(_abi.default_cg_params(b.n_beads)) for b in base]
+    for i in range(n):
+        assert res[i].status == 0
+        assert same(xs[i].cpu().numpy(), ref[i % 4][0])
+    print(f"[gpu] {n} networks in {tot:.1f} ms")
+
+
+def test_batch_argument_errors(lib):
+    net = network.generate(lib, None, seed=5, n_filaments=3, cyl_per_filament=3, box=700.0, n_linkers=2,
+                           n_motors=0, n_branches=0)
+    c = ctx_for(net, _abi.MDY_PREC_FP64)
+    x = torch.tensor(net.coord, device="cuda")
+    p = _abi.default_cg_params(net.n_beads)
+    with pytest.raises(mechanics.MdyError) as e:
+        mechanics.minimize_batch([c, c], [p, p], [x.data_ptr(), x.data_ptr()])
+    assert e.value.code == _abi.MDY_ERR_ARG
+    with pytest.raises(mechanics.MdyError) as e:
+        mechanics.minimize_batch([c], [p], [x.data_ptr()], threads_per_block=96)
+    assert e.value.code == _abi.MDY_ERR_ARG
+    m = mechanics.Context(precision=_abi.MDY_PREC_MIXED)
+    m.set_network(net)
+    m.build_neighbors()
+    with pytest.raises(mechanics.MdyError) as e:
+        mechanics.minimize_batch([c, m], [p, p], [x.data_ptr(), x.data_ptr()])
+    assert e.value.code == _abi.MDY_ERR_ARG
