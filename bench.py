@@ -16,6 +16,11 @@ events on the stream it runs on. The CPU baseline is the oracle (a C
 restatement of the reference minimizer, built with the reference's release
 flags) timed single-threaded on a bounded number of CG iterations and scaled
 to minimizations/s with the GPU run's iteration count per minimization.
+
+Other workloads (--workload): `ensemble256`, 256 independent networks
+minimized by one batched launch (networks/s, per-network p50/p99); `net200k`,
+successive minimizations of a 200k-cylinder network with the neighbor list
+rebuilt before each (minimizations/s mixed vs FP64, GB/s per kernel).
 """
 import argparse
 import glob
@@ -73,7 +78,7 @@ def traffic_from_profiles(workload):
     return None
 
 
-def cpu_baseline(net, gpu_iters, budget_s):
+def cpu_baseline(net, gpu_iters, budget_s, name=WORKLOAD):
     """The oracle minimizer (reference release flags), one thread, on the same
     network for a bounded number of iterations."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
@@ -94,10 +99,46 @@ def cpu_baseline(net, gpu_iters, budget_s):
     dt = time.perf_counter() - t0
     per_iter = dt / max(r.num_iter, 1)
     return dict(value=1.0 / (per_iter * gpu_iters), unit="minimizations/s", cores=1, kind="port",
-                sample=(f"{WORKLOAD}: first {r.num_iter} CG iterations of the oracle minimizer "
+                sample=(f"{name}: first {r.num_iter} CG iterations of the oracle minimizer "
                         f"(-O2 -ffast-math, 1 thread) in {dt:.1f} s = {per_iter * 1e3:.2f} ms/iteration, "
                         f"scaled by the GPU's {gpu_iters} iterations per minimization; "
                         f"host {_cpu_model()}, 1 of {os.cpu_count()} cores"))
+
+
+def cpu_baseline_ensemble(nets, iters, budget_s):
+    """The oracle minimizer, one thread, on three networks of the ensemble
+    (smallest, median, largest) for a bounded number of iterations each; the
+    per-iteration cost per interaction is fitted and scaled to the whole
+    ensemble with the GPU's iteration count of every network."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import numpy as np
+    import pyoracle
+    from medyan_amd import _abi
+
+    o = pyoracle.Oracle(fast=True)
+    o.set_exact(0)
+    order = np.argsort([n.n_cyl for n in nets])
+    pick = [order[0], order[len(order) // 2], order[-1]]
+    rates, done = [], []
+    for i in pick:
+        s = o.system(nets[i])
+        p = _abi.default_cg_params(nets[i].n_beads, energy_rel_tol=0.0)
+        t0 = time.perf_counter()
+        _, _, r, _ = s.minimize(p, iter_cap=5)
+        per = (time.perf_counter() - t0) / max(r.num_iter, 1)
+        n = max(5, int(budget_s / 3 / per))
+        t0 = time.perf_counter()
+        _, _, r, _ = s.minimize(p, iter_cap=n)
+        dt = time.perf_counter() - t0
+        rates.append(dt / max(r.num_iter, 1) / nets[i].n_cyl)
+        done.append((nets[i].n_cyl, int(r.num_iter), dt))
+    per_cyl_iter = float(np.mean(rates))
+    total = sum(per_cyl_iter * n.n_cyl * it for n, it in zip(nets, iters))
+    return dict(value=len(nets) / total, unit="networks/s", cores=1, kind="port",
+                sample=("oracle minimizer (-O2 -ffast-math, 1 thread) on the smallest, median and largest "
+                        "networks: " + "; ".join(f"{c} cylinders, {k} iterations in {t:.1f} s" for c, k, t in done)
+                        + f"; {per_cyl_iter * 1e9:.0f} ns per cylinder-iteration, scaled by every network's GPU "
+                        f"iteration count; host {_cpu_model()}, 1 of {os.cpu_count()} cores"))
 
 
 def _cpu_model():
@@ -137,6 +178,10 @@ def main():
         return
 
     lib = mechanics.load_library()
+    if a.workload == "ensemble256":
+        return bench_ensemble(a, lib)
+    if a.workload == "net200k":
+        return bench_successive(a, lib)
     net = network.generate(lib, a.workload)
     x0 = torch.from_numpy(np.ascontiguousarray(net.coord)).to(dev)
     params = _abi.default_cg_params(net.n_beads, grad_tol=1.0, energy_rel_tol=0.0)
@@ -235,6 +280,226 @@ def main():
     }
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(net, int(r.num_iter), a.cpu_seconds)
+    print(json.dumps(out))
+
+
+def bench_ensemble(a, lib):
+    """256 independent networks (cylinder counts log-uniform in [500, 20000],
+    60 cylinders per um^3, 0.5 linkers and 0.05 motors per cylinder) minimized
+    to 1 pN together by one batched launch in the mixed form. One step = one
+    batched minimization of the whole ensemble from its initial coordinates
+    (tables and neighbor lists resident), in passes whose force-call caps
+    move the blocks of finished networks to the slow ones. Threads per block
+    and the first pass's cap come from an autotuning pass over the
+    candidates before the warmup."""
+    import numpy as np
+    import torch
+    from medyan_amd import _abi, ensemble, mechanics
+
+    nets = ensemble.make_ensemble(lib, 256)
+    ctxs = []
+    t0 = time.perf_counter()
+    for net in nets:
+        c = mechanics.Context(precision=_abi.MDY_PREC_MIXED)
+        c.set_network(net)
+        c.build_neighbors()
+        ctxs.append(c)
+    setup_s = time.perf_counter() - t0
+    x0 = [torch.from_numpy(np.ascontiguousarray(n.coord)).cuda() for n in nets]
+    xs = [x.clone() for x in x0]
+    ptrs = [x.data_ptr() for x in xs]
+    params = [_abi.default_cg_params(n.n_beads, grad_tol=1.0, energy_rel_tol=0.0) for n in nets]
+    ncyl = np.array([n.n_cyl for n in nets], dtype=np.float64)
+    print(f"[ensemble256] {len(nets)} networks ready", file=sys.stderr, flush=True)
+
+    def run(tpb, first, prm=params):
+        for x, x_0 in zip(xs, x0):
+            x.copy_(x_0)
+        torch.cuda.synchronize()
+        return mechanics.minimize_batch(ctxs, prm, ptrs, threads_per_block=tpb, first_pass_calls=first)
+
+    tune = {}
+    for tpb in (64, 128):
+        for first in (500, 1000):
+            tune[(tpb, first)] = run(tpb, first)[2]
+            print(f"[ensemble256] autotune {tpb} threads, first pass {first} force calls: "
+                  f"{tune[(tpb, first)]:.1f} ms", file=sys.stderr, flush=True)
+    tpb, first = min(tune, key=tune.get)
+    for _ in range(a.warmup):
+        run(tpb, first)
+    ms_steps, per_net = [], []
+    for _ in range(a.steps):
+        res, ms, tot = run(tpb, first)
+        ms_steps.append(tot)
+        per_net.append(ms)
+        print(f"[ensemble256] step: {tot:.1f} ms", file=sys.stderr, flush=True)
+    bad = [i for i, r in enumerate(res) if r.status != 0 or r.grad_inf_norm >= 1.0]
+    if bad:
+        print(f"networks {bad[:8]} did not reach 1 pN", file=sys.stderr)
+        sys.exit(1)
+    per_net = np.concatenate(per_net)
+    batch_ms = float(np.mean(ms_steps))
+    abytes = 0
+    for net, c, r in zip(nets, ctxs, res):
+        abytes += algorithmic_bytes(net, len(c.exvol_table()[1]), len(c.boundary_table()[1]), r)
+    achieved = abytes / (batch_ms * 1e-3) / 1e9
+    iters = [int(r.num_iter) for r in res]
+    traffic = traffic_from_profiles("ensemble256")
+    # the reference's default stopping rule (force OR relative energy change < 1e-9)
+    dparams = [_abi.default_cg_params(n.n_beads, grad_tol=1.0) for n in nets]
+    _, _, default_ms = run(tpb, first, dparams)
+    # the same networks one after another, each on the whole GPU (single-network kernel)
+    seq_ms = 0.0
+    for c, x, x_0, p in zip(ctxs, xs, x0, params):
+        x.copy_(x_0)
+        torch.cuda.synchronize()
+        c.minimize_dev(p, x.data_ptr())
+        seq_ms += c.last_ms()
+    out = {
+        "metric": "networks/s to 1 pN, 256-network batched ensemble (mixed form), one launch",
+        "value": 256.0 / (batch_ms * 1e-3),
+        "unit": "networks/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": batch_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair forces (well-conditioned pairs) + f64 energies, reductions, directions",
+        "data": "synthetic (seeded network generator)",
+        "config": {
+            "workload": (f"ensemble256: 256 networks, {int(ncyl.sum())} cylinders in total "
+                         f"({int(ncyl.min())}..{int(ncyl.max())} per network, log-uniform), 60 cylinders per "
+                         "um^3, 0.5 linkers and 0.05 motors per cylinder; PR-CG + backtracking to 1 pN, "
+                         "energy criterion off; tables and neighbor lists resident"),
+            "threads_per_block": tpb,
+            "first_pass_force_calls": first,
+            "autotune_ms": {f"{k[0]} threads, first pass {k[1]}": round(v, 1) for k, v in tune.items()},
+            "per_network_ms_p50": float(np.percentile(per_net, 50)),
+            "per_network_ms_p99": float(np.percentile(per_net, 99)),
+            "per_network_ms_max": float(per_net.max()),
+            "cg_iterations_median": float(np.median(iters)),
+            "cg_iterations_max": int(max(iters)),
+            "sequential_single_network_ms": seq_ms,
+            "sequential_networks_per_s": 256.0 / (seq_ms * 1e-3),
+            "default_stopping_rule_batch_ms": default_ms,
+            "default_stopping_rule_networks_per_s": 256.0 / (default_ms * 1e-3),
+            "setup_s_contexts_and_neighbor_lists": setup_s,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_cg_batch (one cooperative launch, one block cluster per network)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": abytes,
+            "launch_ms": batch_ms,
+        },
+        "cpu_baseline": cpu_baseline_ensemble(nets, iters, a.cpu_seconds) if a.cpu_seconds > 0 else None,
+    }
+    print(json.dumps(out))
+
+
+def bench_successive(a, lib):
+    """200k cylinders: successive minimizations, each from the previous
+    minimum displaced by seeded N(0, 5 nm) bead noise, with the neighbor list
+    rebuilt before each. One step = rebuild + minimization. Mixed form is the
+    headline; the FP64 form runs the same chain from the same start."""
+    import numpy as np
+    import torch
+    from medyan_amd import _abi, mechanics, network
+
+    net = network.generate(lib, "net200k")
+    params = _abi.default_cg_params(net.n_beads, grad_tol=1.0, energy_rel_tol=0.0)
+    x_init = torch.from_numpy(np.ascontiguousarray(net.coord)).cuda()
+
+    def chain(precision):
+        ctx = mechanics.Context(precision=precision)
+        ctx.set_network(net)
+        x = x_init.clone()
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(0x4D454459 + 4)
+        nl_ms, cg_ms, res = [], [], []
+        for m in range(a.warmup + a.steps):
+            x.add_(torch.randn(x.shape, generator=gen, device="cuda", dtype=torch.float64), alpha=5.0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.build_neighbors_dev(x.data_ptr())
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            r = ctx.minimize_dev(params, x.data_ptr())
+            if r.status != 0 or r.grad_inf_norm >= 1.0:
+                print(f"minimization {m} did not reach 1 pN: {r.grad_inf_norm} status {r.status}", file=sys.stderr)
+                sys.exit(1)
+            print(f"[net200k] precision {precision} minimization {m}: {r.num_iter} iterations, "
+                  f"{ctx.last_ms():.0f} ms", file=sys.stderr, flush=True)
+            if m >= a.warmup:
+                nl_ms.append(1e3 * (t1 - t0))
+                cg_ms.append(ctx.last_ms())
+                res.append(r)
+        return ctx, nl_ms, cg_ms, res
+
+    mctx, m_nl, m_cg, m_res = chain(_abi.MDY_PREC_MIXED)
+    fctx, f_nl, f_cg, f_res = chain(_abi.MDY_PREC_FP64)
+    n_ev = len(mctx.exvol_table()[1])
+    n_bd = len(mctx.boundary_table()[1])
+    m_bytes = [algorithmic_bytes(net, n_ev, n_bd, r) for r in m_res]
+    f_bytes = [algorithmic_bytes(net, n_ev, n_bd, r) for r in f_res]
+    m_gbs = [b / (t * 1e-3) / 1e9 for b, t in zip(m_bytes, m_cg)]
+    f_gbs = [b / (t * 1e-3) / 1e9 for b, t in zip(f_bytes, f_cg)]
+    # neighbor-list rebuild: coordinates and cylinder tables read, pairs written
+    nl_bytes = 24 * net.n_beads + 4 * 8 * net.n_cyl + 8 * n_ev
+    m_step = np.array(m_nl) + np.array(m_cg)
+    f_step = np.array(f_nl) + np.array(f_cg)
+    achieved = float(np.mean(m_gbs))
+    out = {
+        "metric": "minimizations/s to 1 pN at 200k cylinders (successive, NL rebuilt each), mixed vs FP64",
+        "value": 1e3 / float(np.mean(m_step)),
+        "unit": "minimizations/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": float(np.mean(m_step)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair forces (well-conditioned pairs) + f64 energies, reductions, directions",
+        "data": "synthetic (seeded network generator)",
+        "config": {
+            "workload": (f"net200k: {net.n_cyl} cylinders, {net.n_beads} beads, {len(net.ls_k)} linkers, "
+                         f"{len(net.ms_k)} motors, {len(net.bs_k)} branch points, {n_ev} excluded-volume pairs "
+                         "at the last rebuild; each step: +N(0, 5 nm) bead displacement of the previous minimum, "
+                         "neighbor-list rebuild, PR-CG to 1 pN"),
+            "mixed_cg_iterations": [int(r.num_iter) for r in m_res],
+            "fp64_cg_iterations": [int(r.num_iter) for r in f_res],
+            "mixed_ms_per_step": float(np.mean(m_step)),
+            "fp64_ms_per_step": float(np.mean(f_step)),
+            "fp64_minimizations_per_s": 1e3 / float(np.mean(f_step)),
+            "mixed_vs_fp64_speedup": float(np.mean(f_step) / np.mean(m_step)),
+            "kernel_GBps": {
+                "k_cg mixed": achieved,
+                "k_cg fp64": float(np.mean(f_gbs)),
+                "neighbor-list rebuild (all its kernels, wall)": nl_bytes / (float(np.mean(m_nl)) * 1e-3) / 1e9,
+            },
+            "neighbor_list_ms": float(np.mean(m_nl)),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_cg (mixed form)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles("net200k"),
+            "algorithmic_bytes_per_launch": float(np.mean(m_bytes)),
+            "launch_ms": float(np.mean(m_cg)),
+        },
+        "cpu_baseline": (cpu_baseline(net, int(np.mean([r.num_iter for r in f_res])), a.cpu_seconds, "net200k")
+                         if a.cpu_seconds > 0 else None),
+    }
     print(json.dumps(out))
 
 

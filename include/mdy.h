@@ -232,18 +232,28 @@ int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *params, double *d_coord,
  * as above) in one cooperative launch. ctxs[i] holds network i with its
  * neighbor lists; all contexts share one device and one precision and appear
  * once. The resident blocks (threads_per_block = 64, 128 or 256; 0 = 64) are
- * split into one cluster per network in proportion to its per-evaluation work,
- * at least one block each; each cluster runs its own minimization with its own
- * barrier. Networks beyond the resident block count run in further launches.
+ * split into one cluster per network in proportion to weights[i] (nullable:
+ * default the per-evaluation work, interactions + 2 x beads), at least one
+ * block each; each cluster runs its own minimization with its own barrier.
+ * Networks beyond the resident block count run in further launches.
+ * first_pass_calls > 0 runs the batch in passes: pass k stops each
+ * minimization after first_pass_calls * 4^(k-1) force calls, and a network
+ * stopped by that cap (below its own limit, not converged) restarts from its
+ * initial coordinates in the next pass, whose blocks are shared among the
+ * networks still running (uncapped once each gets 64 blocks or more).
+ * Results equal those of one uncapped pass (a
+ * trajectory does not depend on its cluster size); 0 = one pass.
  * params, d_coord (device, updated in place), results: n entries each;
  * d_force_out (nullable array; entries nullable) receives final forces;
- * net_ms (nullable, n entries): each network's time from its cluster's start
- * to its end; total_ms (nullable): the whole call. stream NULL = the stream of
+ * net_ms (nullable, n entries): each network's completion time, from the
+ * start of the first launch; total_ms (nullable): the duration of the launches
+ * (HIP events on the stream). stream NULL = the stream of
  * ctxs[0]. No reference counterpart: MController::run minimizes one network
  * per call (MController.h:109-211). */
 int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params, double *const *d_coord,
                        double *const *d_force_out, mdy_min_result *results, double *net_ms,
-                       int threads_per_block, void *stream, double *total_ms);
+                       int threads_per_block, const double *weights, int first_pass_calls, void *stream,
+                       double *total_ms);
 
 /* Per-iteration trace of the last minimization (lambda, energy after the line
  * search, max force, beta, safe-mode flag), for parity checks. */

@@ -131,6 +131,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.pairs); release(c.ev_bs); release(c.ev_kr); release(c.bd_flag); release(c.bd_off); release(c.bd_bead);
     release(c.bd_plane); release(c.bd_k); release(c.bd_sl); release(c.dy_cnt); release(c.dy_off); release(c.dy_cur);
     release(c.dy_idx); release(c.scan_tmp); release(c.bbox); release(c.x); release(c.xt); release(c.f);
+    release(c.x0); release(c.batch_args); release(c.batch_map);
     release(c.contrib); release(c.red); release(c.out); release(c.qe); release(c.qf); release(c.qn); release(c.err);
     release(c.qe_u); release(c.simpson_part); release(c.F0); release(c.F1); release(c.S0); release(c.S1);
     release(c.xmin); release(c.bar); release(c.cg_out); release(c.tr_lam); release(c.tr_e); release(c.tr_mf);
@@ -456,7 +457,8 @@ int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *p, double *d_coord, doub
 
 int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params, double *const *d_coord,
                        double *const *d_force_out, mdy_min_result *results, double *net_ms,
-                       int threads_per_block, void *stream, double *total_ms) {
+                       int threads_per_block, const double *weights, int first_pass_calls, void *stream,
+                       double *total_ms) {
     if (!ctxs || n <= 0 || !params || !d_coord || !results) return MDY_ERR_ARG;
     if (threads_per_block == 0) threads_per_block = 64;
     if (threads_per_block != 64 && threads_per_block != 128 && threads_per_block != 256) {
@@ -479,8 +481,9 @@ int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params,
     Ctx &c0 = ctxs[0]->c;
     (void)hipSetDevice(c0.device);
     hipStream_t st = stream ? (hipStream_t)stream : c0.stream;
-    int rc = run_minimize_batch(cs.data(), n, params, d_coord, d_force_out, results, net_ms, threads_per_block, st,
-                                total_ms);
+    if (first_pass_calls < 0) return fail(ctxs[0], MDY_ERR_ARG, "first_pass_calls must be >= 0");
+    int rc = run_minimize_batch(cs.data(), n, params, d_coord, d_force_out, results, net_ms, threads_per_block,
+                                weights, first_pass_calls, st, total_ms);
     if (rc == MDY_ERR_LAUNCH) return fail(ctxs[0], rc, "cooperative launch failed");
     if (rc != MDY_OK) return hipfail(ctxs[0], "minimize batch");
     int worst = MDY_OK;

@@ -842,12 +842,23 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
 }
 
 // Batched minimizations: the resident blocks (threads_per_block threads each)
-// are split into one cluster per network, in proportion to its per-evaluation
-// work (interactions + beads), at least one block each; networks beyond the
-// resident block count run in further launches. cs[i] owns network i's
-// tables, neighbor lists and CG buffers; c0 (= cs[0]) the launch arrays.
+// are split into one cluster per network, in proportion to its weight (by
+// default its per-evaluation work, interactions + 2 x beads), at least one
+// block each; networks beyond the resident block count run in further
+// launches. With first_pass_calls > 0 the minimizations run in passes: pass
+// k stops every minimization after first_pass_calls * 4^(k-1) force calls;
+// one stopped by that cap (below its own limit, not converged) restarts from
+// its initial coordinates in the next pass, which shares all the blocks
+// among the networks still running; a pass in which every network gets at
+// least kUncappedShare blocks runs uncapped. A minimization's trajectory does not
+// depend on its cluster size, so the results equal those of one uncapped
+// pass; the passes only move the blocks of finished networks to the slow
+// ones. cs[i] owns network i's tables, neighbor lists and CG buffers; cs[0]
+// the launch arrays.
+constexpr int kUncappedShare = 64;   // blocks per network at which a pass runs uncapped
 int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d_x, double *const *d_force_out,
-                       mdy_min_result *res, double *net_ms, int threads_per_block, hipStream_t st, double *total_ms) {
+                       mdy_min_result *res, double *net_ms, int threads_per_block, const double *weights,
+                       int first_pass_calls, hipStream_t st, double *total_ms) {
     Ctx &c0 = *cs[0];
     const bool mixed = c0.opts.precision == MDY_PREC_MIXED;
     const void *kern = mixed ? reinterpret_cast<const void *>(k_cg_batch<true>)
@@ -857,82 +868,130 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
     MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, tpb, 0));
     if (per_cu < 1) return MDY_ERR_LAUNCH;
     const int Gmax = per_cu * c0.num_cu;
+    if (first_pass_calls > 0) {   // initial coordinates, for restarts
+        for (int i = 0; i < n; i++) {
+            Ctx &ci = *cs[i];
+            MDY_TRY(ensure(ci.x0, 3 * (size_t)ci.nb));
+            MDY_HIP(hipMemcpyAsync(ci.x0.p, d_x[i], sizeof(double) * 3 * (size_t)ci.nb, hipMemcpyDeviceToDevice,
+                                   st));
+        }
+    }
     hipEvent_t e0, e1;
     MDY_HIP(hipEventCreate(&e0));
     MDY_HIP(hipEventCreate(&e1));
-    MDY_HIP(hipEventRecord(e0, st));
+    double kernel_ms = 0;
+    double t_origin = -1.0;   // device clock at the first launch's start
     std::vector<CgArgs> args;
     std::vector<int2> map;
-    std::vector<int> gsz;
+    std::vector<int> gsz, todo(n), next;
+    for (int i = 0; i < n; i++) todo[i] = i;
+    long long cap = first_pass_calls > 0 ? first_pass_calls : 0;
     int rc = MDY_OK;
-    for (int lo = 0; lo < n && rc == MDY_OK; lo += Gmax) {
-        const int hi = (n - lo < Gmax) ? n : lo + Gmax;
-        const int m = hi - lo;
-        // cluster sizes: proportional shares of the blocks left after one each
-        double tot = 0;
-        std::vector<double> w(m);
-        for (int i = 0; i < m; i++) {
-            const Ctx &ci = *cs[lo + i];
-            w[i] = (double)ci.dev().n_items() + 2.0 * ci.nb;
-            tot += w[i];
+    while (!todo.empty() && rc == MDY_OK) {
+        next.clear();
+        const int nt = (int)todo.size();
+        // few networks left: each gets enough blocks that a larger cluster
+        // would not be faster, so this pass runs them to the end
+        if ((long long)nt * kUncappedShare <= Gmax) cap = 0;
+        for (int lo = 0; lo < nt && rc == MDY_OK; lo += Gmax) {
+            const int m = (nt - lo < Gmax) ? nt - lo : Gmax;
+            // cluster sizes: proportional shares of the blocks left after one each
+            double tot = 0;
+            std::vector<double> w(m);
+            for (int i = 0; i < m; i++) {
+                const int k = todo[lo + i];
+                const Ctx &ci = *cs[k];
+                w[i] = weights ? weights[k] : (double)ci.dev().n_items() + 2.0 * ci.nb;
+                if (!(w[i] > 0.0)) w[i] = 0.0;
+                tot += w[i];
+            }
+            gsz.assign(m, 1);
+            const int spare = Gmax - m;
+            int used = 0;
+            std::vector<std::pair<double, int>> frac;
+            for (int i = 0; i < m; i++) {
+                const double share = tot > 0 ? spare * w[i] / tot : 0.0;
+                const int whole = (int)share;
+                gsz[i] += whole;
+                used += whole;
+                frac.push_back({share - whole, i});
+            }
+            std::sort(frac.begin(), frac.end(),
+                      [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+                          return a.first > b.first || (a.first == b.first && a.second < b.second);
+                      });
+            for (int k = 0; k < spare - used && k < m; k++) gsz[frac[k].second]++;
+            args.assign(m, CgArgs{});
+            map.clear();
+            std::vector<int> limit(m);
+            for (int i = 0; i < m && rc == MDY_OK; i++) {
+                const int k = todo[lo + i];
+                rc = prepare_cg(*cs[k], &p[k], d_x[k], gsz[i], st, args[i]);
+                limit[i] = args[i].force_call_limit;
+                if (cap > 0 && cap < (long long)args[i].force_call_limit) args[i].force_call_limit = (int)cap;
+                for (int b = 0; b < gsz[i]; b++) map.push_back(make_int2(i, b));
+            }
+            if (rc != MDY_OK) break;
+            const int G = (int)map.size();
+            rc = ensure(c0.batch_args, (size_t)m * sizeof(CgArgs));
+            if (rc == MDY_OK) rc = ensure(c0.batch_map, (size_t)G);
+            if (rc != MDY_OK) break;
+            if (hipMemcpyAsync(c0.batch_args.p, args.data(), (size_t)m * sizeof(CgArgs), hipMemcpyHostToDevice,
+                               st) != hipSuccess ||
+                hipMemcpyAsync(c0.batch_map.p, map.data(), (size_t)G * sizeof(int2), hipMemcpyHostToDevice, st) !=
+                    hipSuccess) {
+                rc = MDY_ERR_HIP;
+                break;
+            }
+            const CgArgs *pa = reinterpret_cast<const CgArgs *>(c0.batch_args.p);
+            const int2 *pm = c0.batch_map.p;
+            void *kargs[] = {&pa, &pm};
+            (void)hipEventRecord(e0, st);
+            if (hipLaunchCooperativeKernel(kern, dim3(G), dim3(tpb), kargs, 0, st) != hipSuccess) {
+                rc = MDY_ERR_LAUNCH;
+                break;
+            }
+            (void)hipEventRecord(e1, st);
+            // the host arrays are reused by the next launch: wait for this one
+            if (hipStreamSynchronize(st) != hipSuccess) { rc = MDY_ERR_HIP; break; }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            kernel_ms += ms;
+            for (int i = 0; i < m && rc == MDY_OK; i++) {
+                const int k = todo[lo + i];
+                Ctx &ci = *cs[k];
+                double R[R_SIZE];
+                if (hipMemcpy(R, ci.cg_out.p, sizeof(R), hipMemcpyDeviceToHost) != hipSuccess) {
+                    rc = MDY_ERR_HIP;
+                    break;
+                }
+                if (t_origin < 0 || R[R_T0] < t_origin) t_origin = R[R_T0];
+                collect(ci, R, &p[k], args[i], &res[k]);
+                res[k].force_call_limit = limit[i];
+                ci.coop_blocks = gsz[i];
+                const bool converged = res[k].grad_inf_norm < p[k].grad_tol ||
+                                       res[k].energy_rel_change < p[k].energy_rel_tol;
+                const bool stopped_by_cap = args[i].force_call_limit < limit[i] &&
+                                            res[k].num_force_call > args[i].force_call_limit && !converged &&
+                                            res[k].status == MDY_OK;
+                if (stopped_by_cap) {
+                    if (hipMemcpyAsync(d_x[k], ci.x0.p, sizeof(double) * 3 * (size_t)ci.nb,
+                                       hipMemcpyDeviceToDevice, st) != hipSuccess) {
+                        rc = MDY_ERR_HIP;
+                        break;
+                    }
+                    next.push_back(k);
+                    continue;
+                }
+                if (net_ms) net_ms[k] = (R[R_T1] - t_origin) * 1e-5;   // 100 MHz ticks
+                rc = copy_force_out(ci, R, d_force_out ? d_force_out[k] : nullptr, st);
+            }
         }
-        gsz.assign(m, 1);
-        const int spare = Gmax - m;
-        int used = 0;
-        std::vector<std::pair<double, int>> frac;
-        for (int i = 0; i < m; i++) {
-            const double share = tot > 0 ? spare * w[i] / tot : 0.0;
-            const int whole = (int)share;
-            gsz[i] += whole;
-            used += whole;
-            frac.push_back({share - whole, i});
-        }
-        std::sort(frac.begin(), frac.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
-            return a.first > b.first || (a.first == b.first && a.second < b.second);
-        });
-        for (int k = 0; k < spare - used && k < m; k++) gsz[frac[k].second]++;
-        args.assign(m, CgArgs{});
-        map.clear();
-        for (int i = 0; i < m && rc == MDY_OK; i++) {
-            rc = prepare_cg(*cs[lo + i], &p[lo + i], d_x[lo + i], gsz[i], st, args[i]);
-            for (int b = 0; b < gsz[i]; b++) map.push_back(make_int2(i, b));
-        }
-        if (rc != MDY_OK) break;
-        const int G = (int)map.size();
-        rc = ensure(c0.batch_args, (size_t)m * sizeof(CgArgs));
-        if (rc == MDY_OK) rc = ensure(c0.batch_map, (size_t)G);
-        if (rc != MDY_OK) break;
-        if (hipMemcpyAsync(c0.batch_args.p, args.data(), (size_t)m * sizeof(CgArgs), hipMemcpyHostToDevice, st) !=
-                hipSuccess ||
-            hipMemcpyAsync(c0.batch_map.p, map.data(), (size_t)G * sizeof(int2), hipMemcpyHostToDevice, st) !=
-                hipSuccess) {
-            rc = MDY_ERR_HIP;
-            break;
-        }
-        const CgArgs *pa = reinterpret_cast<const CgArgs *>(c0.batch_args.p);
-        const int2 *pm = c0.batch_map.p;
-        void *kargs[] = {&pa, &pm};
-        if (hipLaunchCooperativeKernel(kern, dim3(G), dim3(tpb), kargs, 0, st) != hipSuccess) {
-            rc = MDY_ERR_LAUNCH;
-            break;
-        }
-        // the host arrays are reused by the next chunk: wait for the copies
-        if (hipStreamSynchronize(st) != hipSuccess) { rc = MDY_ERR_HIP; break; }
-        for (int i = 0; i < m && rc == MDY_OK; i++) {
-            Ctx &ci = *cs[lo + i];
-            double R[R_SIZE];
-            if (hipMemcpy(R, ci.cg_out.p, sizeof(R), hipMemcpyDeviceToHost) != hipSuccess) { rc = MDY_ERR_HIP; break; }
-            collect(ci, R, &p[lo + i], args[i], &res[lo + i]);
-            if (net_ms) net_ms[lo + i] = (R[R_T1] - R[R_T0]) * 1e-5;   // 100 MHz ticks
-            ci.coop_blocks = gsz[i];
-            rc = copy_force_out(ci, R, d_force_out ? d_force_out[lo + i] : nullptr, st);
-        }
+        todo.swap(next);
+        if (cap > 0) cap *= 4;
     }
-    (void)hipEventRecord(e1, st);
     (void)hipStreamSynchronize(st);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (total_ms) *total_ms = ms;
+    if (total_ms) *total_ms = kernel_ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;

@@ -13,7 +13,8 @@ int exclusive_scan(Ctx &c, const int *in, int *out, int n, hipStream_t st);  // 
 // mdy_cg.hip
 int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res);
 int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d_x, double *const *d_force_out,
-                       mdy_min_result *res, double *net_ms, int threads_per_block, hipStream_t st, double *total_ms);
+                       mdy_min_result *res, double *net_ms, int threads_per_block, const double *weights,
+                       int first_pass_calls, hipStream_t st, double *total_ms);
 // mdy_hvp.hip
 int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st);
 int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_out, hipStream_t st);

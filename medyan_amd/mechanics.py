@@ -68,7 +68,8 @@ def load_library(path: str = LIB_PATH):
         "mdy_minimize": (C.c_int, [vp, C.POINTER(_abi.CGParams), f64p, f64p, C.POINTER(_abi.MinResult)]),
         "mdy_minimize_dev": (C.c_int, [vp, C.POINTER(_abi.CGParams), vp, vp, C.POINTER(_abi.MinResult)]),
         "mdy_minimize_batch": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(_abi.CGParams), C.POINTER(vp),
-                                         C.POINTER(vp), C.POINTER(_abi.MinResult), f64p, C.c_int, vp, f64p]),
+                                         C.POINTER(vp), C.POINTER(_abi.MinResult), f64p, C.c_int, f64p, C.c_int,
+                                         vp, f64p]),
         "mdy_get_trace": (C.c_int, [vp, f64p, f64p, f64p, f64p, i32p, C.c_int64, C.POINTER(C.c_int64)]),
         "mdy_set_trace": (C.c_int, [vp, C.c_int64]),
         "mdy_last_timing": (C.c_int, [vp, f64p]),
@@ -314,11 +315,14 @@ class Context:
         return t.value
 
 
-def minimize_batch(ctxs, params, d_coords, d_forces=None, threads_per_block=64, stream=0):
+def minimize_batch(ctxs, params, d_coords, d_forces=None, threads_per_block=64, weights=None, first_pass_calls=0,
+                   stream=0):
     """Batched minimization (mdy_minimize_batch): ctxs[i] holds network i with
     its neighbor lists; d_coords[i] its device coordinates (updated in place).
     params: one CGParams / ConjugateGradientParams for all, or one per
-    network. Returns (results, per-network ms, total ms)."""
+    network. weights (optional): block shares. first_pass_calls > 0: passes
+    with force-call caps growing 4x, restarting capped networks on more
+    blocks. Returns (results, per-network completion ms, total ms)."""
     n = len(ctxs)
     lib = load_library()
     if not isinstance(params, (list, tuple)):
@@ -332,7 +336,11 @@ def minimize_batch(ctxs, params, d_coords, d_forces=None, threads_per_block=64, 
     res = (_abi.MinResult * n)()
     ms = np.zeros(n)
     tot = C.c_double()
+    w = np.ascontiguousarray(weights, dtype=np.float64) if weights is not None else None
+    if w is not None and w.shape != (n,):
+        raise ValueError("one weight per network")
     rc = lib.mdy_minimize_batch(hs, n, ps, xs, fs, res, _f64(ms), int(threads_per_block),
+                                _f64(w) if w is not None else None, int(first_pass_calls),
                                 C.c_void_p(stream) if stream else None, C.byref(tot))
     if rc != 0:
         raise MdyError(rc, lib.mdy_last_error(ctxs[0].h).decode())
