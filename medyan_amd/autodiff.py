@@ -209,10 +209,12 @@ class SymmetryModes:
 
 
 def solve_hessian(ctx: Context, x: torch.Tensor, b: torch.Tensor, rtol=1e-10, maxiter=20000,
-                  modes: "SymmetryModes | None" = None):
-    """w with H(x) w = b by conjugate gradients on the complement of the
-    symmetry directions `modes` (H is positive definite there at a minimum;
-    b is projected first). Returns (w, iterations, relative residual)."""
+                  modes: "SymmetryModes | None" = None, shift=0.0):
+    """w with (H(x) + shift I) w = b by conjugate gradients on the
+    complement of the symmetry directions `modes` (H is positive definite
+    there at a well-determined minimum; b is projected first). shift > 0
+    regularizes near-null mechanisms (Tikhonov). Returns (w, iterations,
+    relative residual)."""
     P = modes.project if modes is not None else (lambda v: v)
     b = P(b.detach().to(torch.float64).contiguous())
     w = torch.zeros_like(b)
@@ -224,7 +226,10 @@ def solve_hessian(ctx: Context, x: torch.Tensor, b: torch.Tensor, rtol=1e-10, ma
         return w, 0, 0.0
     it = 0
     while it < maxiter and rr > rtol * rtol * bb:
-        hp = P(hvp(ctx, x, p))
+        hp = hvp(ctx, x, p)
+        if shift:
+            hp.add_(p, alpha=shift)
+        hp = P(hp)
         php = float(torch.dot(p, hp))
         if not php > 0.0:
             raise RuntimeError(f"Hessian not positive along a conjugate direction (p.Hp = {php:g}, "
@@ -239,14 +244,16 @@ def solve_hessian(ctx: Context, x: torch.Tensor, b: torch.Tensor, rtol=1e-10, ma
     return w, it, math.sqrt(rr / bb)
 
 
-def stiffness_sensitivity(ctx: Context, x_star: torch.Tensor, grad_l: torch.Tensor, rtol=1e-10, maxiter=20000):
+def stiffness_sensitivity(ctx: Context, x_star: torch.Tensor, grad_l: torch.Tensor, rtol=1e-10, maxiter=20000,
+                          shift=0.0):
     """dL/ds_k (k over the force fields) of a quantity L of the minimized
     coordinates x_star, given grad L(x_star); s_k scales force field k's
     stiffness constants (STIFFNESS_FIELDS). L must be invariant under the
-    symmetry directions (rigid motions of free clusters). Returns
-    (dL/ds [NFF], info)."""
+    symmetry directions (rigid motions of free clusters). shift > 0 solves
+    with H + shift I (a regularized derivative, for networks with floppy
+    mechanisms). Returns (dL/ds [NFF], info)."""
     modes = SymmetryModes(ctx, x_star)
-    w, it, res = solve_hessian(ctx, x_star, grad_l, rtol, maxiter, modes)
+    w, it, res = solve_hessian(ctx, x_star, grad_l, rtol, maxiter, modes, shift)
     d = -energy_jvp(ctx, x_star, w)[:NFF]
     return d, dict(cg_iterations=it, relative_residual=res, clusters=modes.n_clusters, symmetry_modes=modes.n_modes)
 

@@ -22,9 +22,9 @@ WORKLOADS = {
     # 4,000 filaments x 10 cylinders = 40k cylinders, 4 um box, 20k linkers, 2k motors, 2k branches
     "net40k": dict(seed=0x4D454459 + 1, n_filaments=4000, cyl_per_filament=10, box=4000.0,
                    n_linkers=20000, n_motors=2000, n_branches=2000),
-    # 5,000 filaments x 10 cylinders, 5 um box
+    # 5,000 filaments x 10 cylinders, 5 um box, 20k linkers, 2k motors (the 50-filament rules)
     "net50k": dict(seed=0x4D454459 + 3, n_filaments=5000, cyl_per_filament=10, box=5000.0,
-                   n_linkers=0, n_motors=0, n_branches=0),
+                   n_linkers=20000, n_motors=2000, n_branches=0),
     # 25,000 filaments x 8 cylinders = 200k cylinders, 10 um box
     "net200k": dict(seed=0x4D454459 + 4, n_filaments=25000, cyl_per_filament=8, box=10000.0,
                     n_linkers=400000, n_motors=20000, n_branches=10000),
