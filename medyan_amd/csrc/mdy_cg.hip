@@ -83,6 +83,7 @@ struct Cg {
     using Slot = typename SlotT<MIXED>::T;   // force records: double, or float in the mixed form
     const CgArgs &A;
     double *lds;
+    unsigned long long *accl;    // LDS copy of the words a phase's decisions read
     int G, blk, tid, gtid, gs;
     int ph = 0;                  // phase counter (uniform)
     bool aborted = false;
@@ -90,7 +91,7 @@ struct Cg {
 
     // blk: this block's index within the cluster of G blocks that runs the
     // minimization (the whole grid, or one network's share of a batch)
-    __device__ Cg(const CgArgs &a, double *l, int blk_, int G_) : A(a), lds(l) {
+    __device__ Cg(const CgArgs &a, double *l, unsigned long long *al, int blk_, int G_) : A(a), lds(l), accl(al) {
         G = G_; blk = blk_; tid = threadIdx.x;
         gtid = blk * blockDim.x + tid; gs = G * blockDim.x;
     }
@@ -198,6 +199,25 @@ struct Cg {
         X.cnt = 0;
         return mdy_xs_value(&X);
     }
+    // After a barrier every thread needs the same accumulator words: the
+    // first wave loads them once per block into LDS (one request instead of
+    // one per wave to the same L2 lines), then every thread reads LDS.
+    // extra0/extra1 (nullable): counters loaded into words 17 and 18.
+    __attribute__((always_inline)) __device__ void acc_fetch(const unsigned long long *base, const int *extra0,
+                                                             const int *extra1) const {
+        if (tid < 17) accl[tid] = __hip_atomic_load(base + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else if (tid == 17 && extra0) accl[17] = (unsigned long long)(unsigned)ld_i(extra0);
+        else if (tid == 18 && extra1) accl[18] = (unsigned long long)(unsigned)ld_i(extra1);
+        __syncthreads();
+    }
+    __attribute__((always_inline)) __device__ mdy_xsum acc_local(int off) const {
+        mdy_xsum X;
+#pragma unroll
+        for (int k = 0; k < MDY_XS_N; k++) X.d[k] = (long long)accl[off + k];
+        X.flags = (int)accl[off + 7];
+        X.cnt = 0;
+        return X;
+    }
     __attribute__((always_inline)) __device__ mdy_xsum acc_read(const unsigned long long *base) const {
         mdy_xsum X;
 #pragma unroll
@@ -246,10 +266,14 @@ struct Cg {
         block_reduce_xsum(xs, ldsl());
         acc_push(accs(p0) + kAcc0, xs);
         if (!sync(PK_EVAL)) return false;
-        mdy_xsum X = acc_read(accs(p0) + kAcc0);
-        if (with_sd) ev.xa = acc_value(accs(p0) + kAcc1);
-        const int nqe = ld_i(d.qe_n);
-        ev.nqf = ld_i(d.qf_n);
+        acc_fetch(accs(p0), d.qe_n, d.qf_n);
+        mdy_xsum X = acc_local(kAcc0);
+        if (with_sd) {
+            mdy_xsum Y = acc_local(kAcc1);
+            ev.xa = mdy_xs_value(&Y);
+        }
+        const int nqe = (int)accl[17];
+        ev.nqf = (int)accl[18];
         ev.qb = qb;
         if (nqe > kQueueCap || ev.nqf > kQueueCap) {
             status = MDY_ERR_CAPACITY;
@@ -397,11 +421,14 @@ struct Cg {
             acc_push(a + (k == 0 ? kAcc0 : kAcc1), t);
         }
         if (!sync(PK_GATHER)) return false;
-        maxF = __longlong_as_double((long long)__hip_atomic_load(a + kAccMax, __ATOMIC_RELAXED,
-                                                                 __HIP_MEMORY_SCOPE_AGENT));
+        acc_fetch(a, nullptr, nullptr);
+        maxF = __longlong_as_double((long long)accl[kAccMax]);
         double v2[2];
 #pragma clang loop unroll(disable)
-        for (int k = 0; k < 2; k++) v2[k] = acc_value(a + (k == 0 ? kAcc0 : kAcc1));
+        for (int k = 0; k < 2; k++) {
+            mdy_xsum Y = acc_local(k == 0 ? kAcc0 : kAcc1);
+            v2[k] = mdy_xs_value(&Y);
+        }
         FF = v2[0];
         FFp = Fprev ? v2[1] : 0.0;
         return true;
@@ -443,7 +470,8 @@ enum Stage {
 // One minimization run by a cluster of G blocks (blk = index within it).
 template <bool MIXED>
 __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *lds, int blk, int G) {
-    Cg<MIXED> K(A, lds, blk, G);
+    __shared__ unsigned long long accl[24];
+    Cg<MIXED> K(A, lds, accl, blk, G);
     double *R = A.out;
     const bool leader = (K.blk == 0 && K.tid == 0);
     if (leader) R[R_T0] = (double)wall_clock64();

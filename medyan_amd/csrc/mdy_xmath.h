@@ -236,8 +236,7 @@ MDY_HD double mdy_xs_value(const mdy_xsum *in) {
         d2 = (k == h - 2) ? (uint64_t)d[k] : d2;
         sticky |= (k < h - 2) && d[k] != 0;
     }
-    int nb = 0;
-    while (nb < 64 && (top >> nb) != 0) nb++;     /* bit length of the top digit, 1..63 */
+    int nb = 64 - __builtin_clzll(top);            /* bit length of the top digit (top != 0), 1..63 */
     int need = 64 - nb;                           /* bits taken from below the top digit */
     uint64_t W = (nb == 64) ? top : (top << need);
     if (need <= 48) {
