@@ -219,6 +219,25 @@ int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *
  * used for parameter sensitivities of minimized structures. */
 int mdy_energy_jvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_w, double *d_out, void *stream);
 
+/* Post-minimization auxiliary outputs (MController::afterMinimization_,
+ * Controller/MController.h:240-268), at coord:
+ * ForceFieldManager::assignallforcemags -- linker[n_linkers] and
+ * motor[n_motors] stretch forces f0 * d = k (d - l0)
+ * (LinkerStretchingHarmonic.cpp:293, MotorGhostStretchingHarmonic.cpp:339),
+ * branch3[3 * n_branches] branch-point forces
+ * (BranchingStretchingHarmonic.cpp:273-275). Each output nullable. */
+int mdy_stretch_forces(mdy_ctx *ctx, const double *coord, double *linker, double *motor, double *branch3);
+/* ForceFieldManager::computeLoadForces (ForceFieldManager.cpp:292-309) with
+ * BoundaryCylinderRepulsion::computeLoadForces (BoundaryCylinderRepulsion.cpp:124-189):
+ * for every cylinder in a boundary plane's neighbor list (as of the last
+ * neighbor-list build), the plus end loads its second bead and the minus end
+ * its first, at n_mon[type] monomer positions spaced mon_size[type] ahead
+ * along the cylinder: load += max(-n.dir, 0) * K exp(-r/lambda) / lambda,
+ * planes in order. load_plus / load_minus: n_beads x max_mon doubles,
+ * entry [bead * max_mon + i], zero where no load applies. */
+int mdy_load_forces(mdy_ctx *ctx, const double *coord, int n_types, const double *mon_size, const int32_t *n_mon,
+                    int max_mon, double *load_plus, double *load_minus);
+
 /* CGMethod::minimize<method> with backtracking line search
  * (src/Mechanics/Minimizer/CGMethod.hpp:826-1743), run on the device without
  * returning to the host. coord_inout: 3*n_beads doubles (host). force_out

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmdy_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["mdy_api.hip", "mdy_eval.hip", "mdy_nl.hip", "mdy_cg.hip", "mdy_hvp.hip"]
+HIP_SOURCES = ["mdy_api.hip", "mdy_eval.hip", "mdy_nl.hip", "mdy_cg.hip", "mdy_hvp.hip", "mdy_aux.hip"]
 CXX_SOURCES = ["netgen.cpp"]
 ARCH = os.environ.get("MDY_ARCH", "gfx950")
 

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mdy_launch.hpp"
@@ -124,7 +125,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.bs_bs); release(c.bs_k); release(c.bs_l0); release(c.bs_p); release(c.bb_bs); release(c.bb_k);
     release(c.bb_th); release(c.st_off); release(c.st_idx);
     release(c.c_b1); release(c.c_b2); release(c.c_fil); release(c.c_pos); release(c.c_child); release(c.c_id);
-    release(c.c_type); release(c.c_eq); release(c.c_ek); release(c.c_minus);
+    release(c.c_type); release(c.c_eq); release(c.c_ek); release(c.c_minus); release(c.c_plus);
     release(c.planes); release(c.plane_k); release(c.plane_sl);
     release(c.mid); release(c.cell_of); release(c.cell_cnt); release(c.cell_off); release(c.cell_cur);
     release(c.cell_list); release(c.row_cnt); release(c.row_keep); release(c.row_off); release(c.row_keep_off);
@@ -262,6 +263,15 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
         rc |= upload(c.c_eq, cy.eq_length, cy.n, st);
         rc |= upload(c.c_ek, cy.exvol_k, cy.n, st);
         rc |= upload(c.c_minus, cy.minus_end, cy.n, st);
+        // isPlusEnd(): the highest position on its filament
+        std::unordered_map<int, int> last;
+        for (int i = 0; i < cy.n; i++) {
+            auto it = last.find(cy.filament[i]);
+            if (it == last.end() || cy.position[i] > cy.position[it->second]) last[cy.filament[i]] = i;
+        }
+        std::vector<unsigned char> plus(cy.n, 0);
+        for (const auto &kv : last) plus[kv.second] = 1;
+        rc |= upload(c.c_plus, plus.data(), cy.n, st);
         if (hipStreamSynchronize(st) != hipSuccess) return hipfail(ctx, "upload");
     }
     c.ev_cut = s->exvol_cutoff;
@@ -497,6 +507,61 @@ int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params,
         }
     }
     if (worst != MDY_OK) return fail(ctxs[0], worst, "a minimization of the batch failed (see results[].status)");
+    return MDY_OK;
+}
+
+int mdy_stretch_forces(mdy_ctx *ctx, const double *coord, double *linker, double *motor, double *branch3) {
+    if (!ctx || !coord) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.have_sys) return fail(ctx, MDY_ERR_STATE, "no network set");
+    (void)hipSetDevice(c.device);
+    const Dev D = c.dev();
+    const size_t n = (size_t)D.n_ls + (D.n_lm - D.n_ls) + 3 * (size_t)D.n_bs;
+    if (ensure(c.out, n + 16) != MDY_OK) return hipfail(ctx, "allocation");
+    double *dl = c.out.p, *dm = dl + D.n_ls, *db = dm + (D.n_lm - D.n_ls);
+    if (hipMemcpyAsync(c.xt.p, coord, sizeof(double) * 3 * c.nb, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+        return hipfail(ctx, "coord upload");
+    if (launch_stretch_forces(c, c.xt.p, dl, dm, db, c.stream) != MDY_OK) return hipfail(ctx, "stretch forces");
+    if ((linker && D.n_ls &&
+         hipMemcpyAsync(linker, dl, sizeof(double) * D.n_ls, hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
+        (motor && D.n_lm > D.n_ls &&
+         hipMemcpyAsync(motor, dm, sizeof(double) * (D.n_lm - D.n_ls), hipMemcpyDeviceToHost, c.stream) !=
+             hipSuccess) ||
+        (branch3 && D.n_bs &&
+         hipMemcpyAsync(branch3, db, sizeof(double) * 3 * D.n_bs, hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
+        hipStreamSynchronize(c.stream) != hipSuccess)
+        return hipfail(ctx, "stretch force readback");
+    return MDY_OK;
+}
+
+int mdy_load_forces(mdy_ctx *ctx, const double *coord, int n_types, const double *mon_size, const int32_t *n_mon,
+                    int max_mon, double *load_plus, double *load_minus) {
+    if (!ctx || !coord || n_types <= 0 || !mon_size || !n_mon || max_mon <= 0 || !load_plus || !load_minus)
+        return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.have_sys || !c.have_nl) return fail(ctx, MDY_ERR_STATE, "network / neighbor list missing");
+    for (int t = 0; t < n_types; t++)
+        if (n_mon[t] < 0 || n_mon[t] > max_mon) return fail(ctx, MDY_ERR_ARG, "n_mon[type] must lie in [0, max_mon]");
+    (void)hipSetDevice(c.device);
+    const size_t n = (size_t)c.nb * max_mon;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(double) * (2 * n + n_types) + sizeof(int) * n_types) != hipSuccess)
+        return hipfail(ctx, "allocation");
+    double *lp = d, *lm = d + n, *ms = d + 2 * n;
+    int *nm = reinterpret_cast<int *>(ms + n_types);
+    int rc = MDY_OK;
+    if (hipMemcpyAsync(c.xt.p, coord, sizeof(double) * 3 * c.nb, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+        hipMemcpyAsync(ms, mon_size, sizeof(double) * n_types, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+        hipMemcpyAsync(nm, n_mon, sizeof(int) * n_types, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+        rc = MDY_ERR_HIP;
+    if (rc == MDY_OK) rc = launch_load_forces(c, c.xt.p, n_types, ms, nm, max_mon, lp, lm, c.stream);
+    if (rc == MDY_OK &&
+        (hipMemcpyAsync(load_plus, lp, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+         hipMemcpyAsync(load_minus, lm, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+         hipStreamSynchronize(c.stream) != hipSuccess))
+        rc = MDY_ERR_HIP;
+    (void)hipFree(d);
+    if (rc != MDY_OK) return hipfail(ctx, "load forces");
     return MDY_OK;
 }
 

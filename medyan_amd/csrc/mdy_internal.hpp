@@ -70,7 +70,7 @@ struct Ctx {
     int n_fs = 0, n_fb = 0, n_ls = 0, n_ms = 0, n_bs = 0, n_bb = 0;
     // cylinders
     DBuf<int> c_b1, c_b2, c_fil, c_pos, c_child; DBuf<long long> c_id; DBuf<short> c_type;
-    DBuf<double> c_eq, c_ek; DBuf<unsigned char> c_minus;
+    DBuf<double> c_eq, c_ek; DBuf<unsigned char> c_minus, c_plus;   // c_plus: last cylinder of its filament
     float ev_cut = 0, bd_cut = 0;
     int ev_on = 0, bd_on = 0;
     int n_planes = 0;

@@ -18,6 +18,11 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
 // mdy_hvp.hip
 int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st);
 int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_out, hipStream_t st);
+// mdy_aux.hip
+int launch_stretch_forces(Ctx &c, const double *d_x, double *d_linker, double *d_motor, double *d_branch,
+                          hipStream_t st);
+int launch_load_forces(Ctx &c, const double *d_x, int ntypes, const double *d_mon_size, const int *d_n_mon,
+                       int max_mon, double *d_lp, double *d_lm, hipStream_t st);
 
 template <class T>
 int ensure(DBuf<T> &b, size_t n) {

@@ -70,6 +70,8 @@ def load_library(path: str = LIB_PATH):
         "mdy_minimize_batch": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(_abi.CGParams), C.POINTER(vp),
                                          C.POINTER(vp), C.POINTER(_abi.MinResult), f64p, C.c_int, f64p, C.c_int,
                                          vp, f64p]),
+        "mdy_stretch_forces": (C.c_int, [vp, f64p, f64p, f64p, f64p]),
+        "mdy_load_forces": (C.c_int, [vp, f64p, C.c_int, f64p, i32p, C.c_int, f64p, f64p]),
         "mdy_get_trace": (C.c_int, [vp, f64p, f64p, f64p, f64p, i32p, C.c_int64, C.POINTER(C.c_int64)]),
         "mdy_set_trace": (C.c_int, [vp, C.c_int64]),
         "mdy_last_timing": (C.c_int, [vp, f64p]),
@@ -261,6 +263,27 @@ class Context:
     def forces_dev(self, d_coord: int, d_force: int, accumulate=True, stream: int = 0):
         self._check(self.lib.mdy_forces_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_force), 1 if accumulate else 0,
                                             C.c_void_p(stream) if stream else None))
+
+    def stretch_forces(self, coord):
+        """assignallforcemags outputs at coord: (linker, motor, branch [n x 3])."""
+        x = np.ascontiguousarray(coord, dtype=np.float64)
+        n = self.net
+        lk, mt, br = np.zeros(max(len(n.ls_k), 1)), np.zeros(max(len(n.ms_k), 1)), np.zeros(max(3 * len(n.bs_k), 1))
+        self._check(self.lib.mdy_stretch_forces(self.h, _f64(x), _f64(lk), _f64(mt), _f64(br)))
+        return lk[: len(n.ls_k)], mt[: len(n.ms_k)], br[: 3 * len(n.bs_k)].reshape(-1, 3)
+
+    def load_forces(self, coord, mon_size=(2.7,), n_mon=(40,), max_mon=None):
+        """computeLoadForces at coord (boundary neighbor list of the last
+        build): (plus, minus), each n_beads x max_mon."""
+        x = np.ascontiguousarray(coord, dtype=np.float64)
+        ms = np.ascontiguousarray(mon_size, dtype=np.float64)
+        nm = np.ascontiguousarray(n_mon, dtype=np.int32)
+        mm = int(max_mon if max_mon is not None else nm.max())
+        nb = self.net.n_beads
+        lp, lm = np.zeros(nb * mm), np.zeros(nb * mm)
+        self._check(self.lib.mdy_load_forces(self.h, _f64(x), len(ms), _f64(ms),
+                                             nm.ctypes.data_as(C.POINTER(C.c_int32)), mm, _f64(lp), _f64(lm)))
+        return lp.reshape(nb, mm), lm.reshape(nb, mm)
 
     def hvp_dev(self, d_coord: int, d_v: int, d_hv: int, stream: int = 0):
         """H v of the exact-gradient force field at the device coordinates."""

@@ -679,6 +679,78 @@ void orc_bd_forces(const double *x, double *f, int32_t nplanes, const double *pl
     }
 }
 
+/* Boundary load forces: ForceFieldManager::computeLoadForces
+ * (ForceFieldManager.cpp:292-309) -> BoundaryCylinderRepulsion::computeLoadForces
+ * (BoundaryCylinderRepulsion.cpp:167-189) -> boundaryCylinderRepulsionLoadForce
+ * (:124-163): for each plane in order and each cylinder of its neighbor list
+ * (midpoint of x_nl within the float cutoff, NeighborListImpl.cpp:738-751),
+ * a plus end (Cylinder::isPlusEnd: the last cylinder of its filament) loads
+ * its second bead, a minus end its first: dir = normalizedVector(bd - bo)
+ * (Util/Math/Vec.hpp:350-374), monomer i at bd + (i*monSize)*dir,
+ * proj = max(-n.dir, 0) with the raw plane normal (BoundaryElementImpl.h:97-103),
+ * load = K exp(-r/lambda)/lambda (BoundaryCylinderRepulsionExp.cpp:113-118),
+ * loadForces[i] += proj*load. lp/lm: n_beads x max_mon, zeroed first. */
+static void load_one(const double *pl, double krep, double slen, const double *bo, const double *bd, double mon,
+                     int nmon, double *out) {
+    double v[3] = {bd[0] - bo[0], bd[1] - bo[1], bd[2] - bo[2]};
+    double mag2 = 0.0;
+    for (int k = 0; k < 3; k++) mag2 += v[k] * v[k];
+    double norm = sqrt(mag2);
+    double dir[3];
+    for (int k = 0; k < 3; k++) dir[k] = v[k] / norm;
+    for (int i = 0; i < nmon; i++) {
+        double step = i * mon;
+        double nc[3] = {bd[0] + step * dir[0], bd[1] + step * dir[1], bd[2] + step * dir[2]};
+        double dt = 0.0;
+        for (int k = 0; k < 3; k++) dt += pl[k] * dir[k];
+        double proj = (-dt < 0.0) ? 0.0 : -dt;
+        double r = plane_dist(pl, nc);
+        double R = -r / slen;
+        double load = krep * o_exp(R) / slen;
+        out[i] += proj * load;
+    }
+}
+static const mdy_cylinders *g_sort_cyl;
+static int cmp_fil_pos(const void *a, const void *b) {
+    int32_t i = *(const int32_t *)a, j = *(const int32_t *)b;
+    const mdy_cylinders *c = g_sort_cyl;
+    if (c->filament[i] != c->filament[j]) return c->filament[i] < c->filament[j] ? -1 : 1;
+    if (c->position[i] != c->position[j]) return c->position[i] < c->position[j] ? -1 : 1;
+    return (i > j) - (i < j);
+}
+void orc_load_forces(const mdy_cylinders *cyl, int32_t n_beads, const double *x_nl, const double *x,
+                     const mdy_boundary *bnd, int32_t n_types, const double *mon_size, const int32_t *n_mon,
+                     int32_t max_mon, double *lp, double *lm) {
+    for (int64_t k = 0; k < (int64_t)n_beads * max_mon; k++) { lp[k] = 0.0; lm[k] = 0.0; }
+    unsigned char *plus = (unsigned char *)calloc((size_t)(cyl->n > 0 ? cyl->n : 1), 1);
+    {   /* the last cylinder of each filament: sort by (filament, position) */
+        int32_t *ord = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cyl->n > 0 ? cyl->n : 1));
+        for (int32_t i = 0; i < cyl->n; i++) ord[i] = i;
+        g_sort_cyl = cyl;
+        qsort(ord, (size_t)cyl->n, sizeof(int32_t), cmp_fil_pos);
+        for (int32_t k = 0; k < cyl->n; k++)
+            if (k + 1 == cyl->n || cyl->filament[ord[k + 1]] != cyl->filament[ord[k]]) plus[ord[k]] = 1;
+        free(ord);
+    }
+    for (int32_t ib = 0; ib < bnd->n_planes; ib++) {
+        const double *pl = bnd->plane + 4 * ib;
+        for (int32_t i = 0; i < cyl->n; i++) {
+            double mc[3];
+            const double *b1 = x_nl + cyl->bead1[i], *b2 = x_nl + cyl->bead2[i];
+            for (int k = 0; k < 3; k++) mc[k] = b1[k] * (1.0 - 0.5) + 0.5 * b2[k];
+            if (!(plane_dist(pl, mc) < bnd->cutoff)) continue;
+            int ty = (cyl->type && cyl->type[i] < n_types) ? cyl->type[i] : 0;
+            if (plus[i])
+                load_one(pl, bnd->k[ib], bnd->screen[ib], x + cyl->bead1[i], x + cyl->bead2[i], mon_size[ty],
+                         n_mon[ty], lp + (int64_t)(cyl->bead2[i] / 3) * max_mon);
+            if (cyl->minus_end[i])
+                load_one(pl, bnd->k[ib], bnd->screen[ib], x + cyl->bead2[i], x + cyl->bead1[i], mon_size[ty],
+                         n_mon[ty], lm + (int64_t)(cyl->bead1[i] / 3) * max_mon);
+        }
+    }
+    free(plus);
+}
+
 /* ------------------------------------------------------------------------- */
 /* a13: neighbor lists                                                       */
 /* ------------------------------------------------------------------------- */

@@ -55,6 +55,10 @@ void   orc_ls_forces(const double *x, double *f, int32_t n, const int32_t *bs, c
 double orc_bs_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0, const double *p);
 void   orc_bs_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0,
                      const double *p, double *stretch3);
+/* boundary load forces on filament ends (ForceFieldManager::computeLoadForces) */
+void   orc_load_forces(const mdy_cylinders *cyl, int32_t n_beads, const double *x_nl, const double *x,
+                       const mdy_boundary *bnd, int32_t n_types, const double *mon_size, const int32_t *n_mon,
+                       int32_t max_mon, double *lp, double *lm);
 double orc_bb_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *th0);
 void   orc_bb_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *th0,
                      double *stretch3);

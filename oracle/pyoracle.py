@@ -261,6 +261,45 @@ class OracleSystem:
         except Exception:
             pass
 
+    def stretch_forces(self, x=None):
+        """assignforcemags outputs at x: linker and motor stretch forces,
+        branch-point 3-vectors (LinkerStretchingHarmonic.cpp:293,
+        MotorGhostStretchingHarmonic.cpp:339, BranchingStretchingHarmonic.cpp:273-275)."""
+        x = self.coord0 if x is None else np.ascontiguousarray(x, np.float64)
+        n = self.net
+        f = np.zeros_like(x)
+        out = []
+        for bs, k, l0, p1, p2 in ((n.ls_beadset, n.ls_k, n.ls_l0, n.ls_pos1, n.ls_pos2),
+                                  (n.ms_beadset, n.ms_k, n.ms_l0, n.ms_pos1, n.ms_pos2)):
+            st = np.zeros(len(k))
+            if len(k):
+                arr = [np.ascontiguousarray(a, t) for a, t in ((bs, np.int32), (k, np.float64), (l0, np.float64),
+                                                               (p1, np.float64), (p2, np.float64))]
+                self.o.lib.orc_ls_forces(_p(x), _p(f), len(k), _p(arr[0], C.c_int32), _p(arr[1]), _p(arr[2]),
+                                         _p(arr[3]), _p(arr[4]), _p(st))
+            out.append(st)
+        br = np.zeros(3 * len(n.bs_k))
+        if len(n.bs_k):
+            arr = [np.ascontiguousarray(a, t) for a, t in ((n.bs_beadset, np.int32), (n.bs_k, np.float64),
+                                                           (n.bs_l0, np.float64), (n.bs_pos, np.float64))]
+            self.o.lib.orc_bs_forces(_p(x), _p(f), len(n.bs_k), _p(arr[0], C.c_int32), _p(arr[1]), _p(arr[2]),
+                                     _p(arr[3]), _p(br))
+        out.append(br)
+        return tuple(out)
+
+    def load_forces(self, x, mon_size, n_mon, max_mon):
+        """computeLoadForces at x with the boundary neighbor list of the
+        construction coordinates: (plus, minus), each n_beads x max_mon."""
+        x = np.ascontiguousarray(x, np.float64)
+        nb = self.net.n_beads
+        lp = np.zeros(nb * max_mon)
+        lm = np.zeros(nb * max_mon)
+        ms = np.ascontiguousarray(mon_size, np.float64)
+        nm = np.ascontiguousarray(n_mon, np.int32)
+        self.o.lib.orc_load_forces(C.byref(self.sysc.cyl), nb, _p(self.coord0), _p(x), C.byref(self.sysc.bnd),
+                                   len(ms), _p(ms), _p(nm, C.c_int32), int(max_mon), _p(lp), _p(lm))
+        return lp.reshape(nb, max_mon), lm.reshape(nb, max_mon)
+
     @property
     def ev_table(self):
         n = self.s.ev_n

@@ -1,0 +1,69 @@
+"""Post-minimization auxiliary outputs on the device against the oracle:
+stretch forces of linkers, motors and branch points (assignallforcemags)
+and boundary load forces on filament ends (computeLoadForces), at the
+minimized coordinates with the neighbor lists of the starting coordinates,
+as MController::afterMinimization_ computes them. Bit for bit against the
+oracle in exact mode (the shared exp); within 1e-13 in reference mode."""
+import numpy as np
+import pytest
+
+from medyan_amd import _abi, mechanics, network
+
+pytestmark = pytest.mark.gpu
+
+
+def nets(lib):
+    return {
+        "net500": network.generate(lib, "net500"),
+        "branched": network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=2),
+        "net40k": network.generate(lib, "net40k"),
+    }
+
+
+@pytest.fixture(scope="module")
+def all_nets(lib):
+    return nets(lib)
+
+
+def bits_equal(a, b):
+    return a.shape == b.shape and np.array_equal(np.ascontiguousarray(a).view(np.int64),
+                                                 np.ascontiguousarray(b).view(np.int64))
+
+
+@pytest.mark.parametrize("case", ["net500", "branched", "net40k"])
+def test_aux_outputs_after_minimization(oracle, all_nets, case):
+    net = all_nets[case]
+    print(f"[gpu] aux outputs, {case}", flush=True)
+    ctx = mechanics.Context()
+    ctx.set_network(net)
+    ctx.build_neighbors()
+    x, _, res = ctx.minimize(_abi.default_cg_params(net.n_beads, energy_rel_tol=0.0))
+    assert res.status == 0
+    lk, mt, br = ctx.stretch_forces(x)
+    lp, lm = ctx.load_forces(x, [2.7], [40], 40)
+    for exact in (1, 0):
+        oracle.set_exact(exact)
+        try:
+            s = oracle.system(net)                       # lists from the starting coordinates
+            olk, omt, obr = s.stretch_forces(x)
+            olp, olm = s.load_forces(x, [2.7], [40], 40)
+        finally:
+            oracle.set_exact(0)
+        assert bits_equal(lk, olk) and bits_equal(mt, omt) and bits_equal(br.ravel(), obr)
+        if exact:
+            assert bits_equal(lp, olp) and bits_equal(lm, olm)
+        else:
+            np.testing.assert_allclose(lp, olp, rtol=1e-13, atol=0)
+            np.testing.assert_allclose(lm, olm, rtol=1e-13, atol=0)
+    print(f"[gpu] {case}: {np.count_nonzero(lp.any(1))} plus ends and {np.count_nonzero(lm.any(1))} minus ends "
+          f"loaded; max linker stretch force {np.abs(lk).max() if len(lk) else 0:.3f} pN")
+    assert lp.any() or lm.any()
+
+
+def test_load_forces_argument_checks(all_nets):
+    net = all_nets["net500"]
+    ctx = mechanics.Context()
+    ctx.set_network(net)
+    ctx.build_neighbors()
+    with pytest.raises(mechanics.MdyError):
+        ctx.load_forces(net.coord, [2.7], [41], 40)        # n_mon above max_mon
