@@ -3,8 +3,9 @@
 //
 // Names, argument meaning and error behaviour follow the reference:
 //   ForceFieldManager::vectorizeAllForceFields / computeEnergy /
-//     computeIndividualEnergies / computeForces
-//     (src/Mechanics/ForceField/ForceFieldManager.h:48-216, .cpp:35-290)
+//     computeIndividualEnergies / computeForces / assignallforcemags /
+//     computeLoadForces
+//     (src/Mechanics/ForceField/ForceFieldManager.h:48-216, .cpp:35-309)
 //   CGMethod::minimize (src/Mechanics/Minimizer/CGMethod.hpp:826-836)
 //   ConjugateGradientParams (CGMethod.hpp:44-61)
 //   MinimizationResult, EnergyReport (MinimizationTypes.hpp:10-64)
@@ -118,6 +119,32 @@ public:
     void computeForces(const double *coord, double *force, int numVar) const {
         for (int i = 0; i < numVar; i++) force[i] = 0.0;
         check(mdy_forces(ctx_, coord, force));
+    }
+
+    // assignallforcemags (MController::afterMinimization_): stretch forces of
+    // linkers and motors, force vectors of branch points, at coord.
+    struct ForceMags {
+        std::vector<double> linker, motor, branch;   // branch: 3 per branch point
+    };
+    ForceMags assignallforcemags(const double *coord, int nLinkers, int nMotors, int nBranches) const {
+        ForceMags m;
+        m.linker.assign(nLinkers, 0.0);
+        m.motor.assign(nMotors, 0.0);
+        m.branch.assign(3 * (size_t)nBranches, 0.0);
+        check(mdy_stretch_forces(ctx_, coord, m.linker.data(), m.motor.data(), m.branch.data()));
+        return m;
+    }
+    // computeLoadForces: per bead, the boundary load on monomer positions
+    // ahead of a plus end (plus) or minus end (minus), [bead * maxMon + i].
+    void computeLoadForces(const double *coord, const std::vector<double> &monSize, const std::vector<int32_t> &numMon,
+                           std::vector<double> &plus, std::vector<double> &minus) const {
+        int maxMon = 0;
+        for (int32_t v : numMon) maxMon = v > maxMon ? v : maxMon;
+        plus.assign((size_t)(numDof_ / 3) * maxMon, 0.0);
+        minus.assign(plus.size(), 0.0);
+        if (maxMon == 0) return;
+        check(mdy_load_forces(ctx_, coord, (int)monSize.size(), monSize.data(), numMon.data(), maxMon, plus.data(),
+                              minus.data()));
     }
 
     mdy_ctx *ctx() const { return ctx_; }

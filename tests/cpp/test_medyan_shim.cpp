@@ -67,6 +67,13 @@ static int run() {
     REQUIRE(mf == r.gradInfNorm);
     REQUIRE(std::fabs(ffm.computeEnergy(coord.data()) - r.energiesAfter.total) <= 1e-12 * std::fabs(r.energiesAfter.total));
 
+    // post-minimization outputs
+    auto mags = ffm.assignallforcemags(coord.data(), net->sys.ff.ls_n, net->sys.ff.ms_n, net->sys.ff.bs_n);
+    REQUIRE((int)mags.linker.size() == net->sys.ff.ls_n);
+    std::vector<double> lp, lm;
+    ffm.computeLoadForces(coord.data(), {2.7}, {40}, lp, lm);
+    REQUIRE(lp.size() == (size_t)(ndof / 3) * 40);
+
     // misuse is reported like the reference reports it: by exception
     bool threw = false;
     try {
