@@ -37,6 +37,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = "net40k"
 # per-interaction algorithmic bytes: int32 bead offsets + FP64 parameters
 B_FS, B_FB, B_LM, B_BS, B_BB, B_EV, B_BD = 24, 28, 48, 36, 32, 40, 24
+B_BDH, B_BP = 32, 28
 
 
 def parse():
@@ -60,7 +61,8 @@ def algorithmic_bytes(net, n_ev, n_bd, res):
     nb = net.n_beads
     ndof = 3 * nb
     tables = (B_FS * len(net.fs_k) + B_FB * len(net.fb_k) + B_LM * (len(net.ls_k) + len(net.ms_k))
-              + B_BS * len(net.bs_k) + B_BB * len(net.bb_k) + B_EV * n_ev + B_BD * n_bd)
+              + B_BS * len(net.bs_k) + B_BB * len(net.bb_k) + B_BDH * len(net.bdh_k) + B_BP * len(net.bp_k)
+              + B_EV * n_ev + B_BD * n_bd)
     b_energy = 24 * nb + tables + 8
     n_eval = res.num_energy_eval
     trials = max(n_eval - 2, 0)

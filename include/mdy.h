@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MDY_ABI_VERSION 1
+#define MDY_ABI_VERSION 2
 
 enum {
     MDY_OK = 0,
@@ -52,6 +52,8 @@ enum {
     MDY_FF_MS,     /* MotorGhostStretching<Harmonic>         */
     MDY_FF_BS,     /* BranchingStretching<Harmonic>          */
     MDY_FF_BB,     /* BranchingBending<Cosine>               */
+    MDY_FF_BDH,    /* BranchingDihedral<Cosine>              */
+    MDY_FF_BP,     /* BranchingPosition<Cosine>              */
     MDY_FF_EV,     /* CylinderExclVolume<CylinderExclVolRepulsion> */
     MDY_FF_BD,     /* BoundaryCylinderRepulsion<Exp>         */
     MDY_NFF
@@ -90,6 +92,12 @@ typedef struct mdy_ff_tables {
     const double *bs_pos;
     /* BranchingBending::vectorize    (BranchingBending.cpp:29-48) */
     int32_t bb_n; const int32_t *bb_beadset; const double *bb_k; const double *bb_theta0;
+    /* BranchingDihedral::vectorize   (BranchingDihedral.cpp:32-60): mother first and
+     * second bead, daughter first and second bead; kdih; adjusted position */
+    int32_t bdh_n; const int32_t *bdh_beadset; const double *bdh_k; const double *bdh_pos;
+    /* BranchingPosition::vectorize   (BranchingPosition.cpp:27-50): mother first and
+     * second bead, daughter first bead; kpos; adjusted position */
+    int32_t bp_n; const int32_t *bp_beadset; const double *bp_k; const double *bp_pos;
 } mdy_ff_tables;
 
 /* Plane boundary elements (PlaneBoundaryElement, src/Structure/BoundaryElementImpl.h:30-119):
@@ -224,8 +232,11 @@ int mdy_energy_jvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_w, d
  * ForceFieldManager::assignallforcemags -- linker[n_linkers] and
  * motor[n_motors] stretch forces f0 * d = k (d - l0)
  * (LinkerStretchingHarmonic.cpp:293, MotorGhostStretchingHarmonic.cpp:339),
- * branch3[3 * n_branches] branch-point forces
- * (BranchingStretchingHarmonic.cpp:273-275). Each output nullable. */
+ * branch3[3 * n_branches] branch-point forces: the branching stretching,
+ * dihedral and position contributions added in force-field order
+ * (BranchingStretchingHarmonic.cpp:273-275, BranchingDihedralCosine.cpp:503-505,
+ * BranchingPositionCosine.cpp:445-447); entry i of the three branching tables
+ * is the same branch point, n_branches the largest table. Each output nullable. */
 int mdy_stretch_forces(mdy_ctx *ctx, const double *coord, double *linker, double *motor, double *branch3);
 /* ForceFieldManager::computeLoadForces (ForceFieldManager.cpp:292-309) with
  * BoundaryCylinderRepulsion::computeLoadForces (BoundaryCylinderRepulsion.cpp:124-189):
@@ -313,6 +324,7 @@ typedef struct mdy_gen_params {
     double   k_bd, screen_bd; float bd_cutoff;
     double   k_linker, k_motor;
     double   k_br_stretch, l_br_stretch, k_br_bend, theta_br_bend;
+    double   k_br_dihedral, k_br_position;   /* 0 = no dihedral / position tables */
 } mdy_gen_params;
 
 typedef struct mdy_network {

@@ -68,7 +68,8 @@ struct MinimizationResult {
 
 inline const char *const kForceFieldNames[MDY_NFF] = {
     "FilamentStretching", "FilamentBending", "LinkerStretching", "MotorGhostStretching",
-    "BranchingStretching", "BranchingBending", "CylinderExclVolume", "BoundaryCylinderRepulsion"};
+    "BranchingStretching", "BranchingBending", "BranchingDihedral", "BranchingPosition",
+    "CylinderExclVolume", "BoundaryCylinderRepulsion"};
 
 class Error : public std::runtime_error {
 public:

@@ -6,7 +6,7 @@ medyan_amd.mechanics.
 """
 import ctypes as C
 
-MDY_ABI_VERSION = 1
+MDY_ABI_VERSION = 2
 
 MDY_OK = 0
 ERRORS = {
@@ -26,8 +26,11 @@ MDY_ERR_RECOVERY = -9
 MDY_ERR_TIMEOUT = -10
 
 FF_NAMES = ["FilamentStretching", "FilamentBending", "LinkerStretching", "MotorGhostStretching",
-            "BranchingStretching", "BranchingBending", "CylinderExclVolume", "BoundaryCylinderRepulsion"]
-MDY_NFF = 8
+            "BranchingStretching", "BranchingBending", "BranchingDihedral", "BranchingPosition",
+            "CylinderExclVolume", "BoundaryCylinderRepulsion"]
+MDY_NFF = 10
+(MDY_FF_FS, MDY_FF_FB, MDY_FF_LS, MDY_FF_MS, MDY_FF_BS, MDY_FF_BB, MDY_FF_BDH, MDY_FF_BP, MDY_FF_EV,
+ MDY_FF_BD) = range(MDY_NFF)
 
 MDY_CG_STEEPEST, MDY_CG_FLETCHER_RIEVES, MDY_CG_POLAK_RIBIERE = 0, 1, 2
 MDY_PREC_FP64, MDY_PREC_MIXED = 0, 1
@@ -59,6 +62,8 @@ class FFTables(C.Structure):
         ("bs_n", C.c_int32), ("bs_beadset", _i32p), ("bs_k", _f64p), ("bs_l0", _f64p),
         ("bs_pos", _f64p),
         ("bb_n", C.c_int32), ("bb_beadset", _i32p), ("bb_k", _f64p), ("bb_theta0", _f64p),
+        ("bdh_n", C.c_int32), ("bdh_beadset", _i32p), ("bdh_k", _f64p), ("bdh_pos", _f64p),
+        ("bp_n", C.c_int32), ("bp_beadset", _i32p), ("bp_k", _f64p), ("bp_pos", _f64p),
     ]
 
 
@@ -113,7 +118,8 @@ class GenParams(C.Structure):
                 ("k_bd", C.c_double), ("screen_bd", C.c_double), ("bd_cutoff", C.c_float),
                 ("k_linker", C.c_double), ("k_motor", C.c_double),
                 ("k_br_stretch", C.c_double), ("l_br_stretch", C.c_double),
-                ("k_br_bend", C.c_double), ("theta_br_bend", C.c_double)]
+                ("k_br_bend", C.c_double), ("theta_br_bend", C.c_double),
+                ("k_br_dihedral", C.c_double), ("k_br_position", C.c_double)]
 
 
 class Network(C.Structure):

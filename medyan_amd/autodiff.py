@@ -38,8 +38,8 @@ from .network import Network
 NFF = _abi.MDY_NFF
 
 # stiffness arrays of each force field, in force-field order (FS, FB, LS, MS,
-# BS, BB, EV, BD); every energy is linear in these
-STIFFNESS_FIELDS = ("fs_k", "fb_k", "ls_k", "ms_k", "bs_k", "bb_k", "cyl_exvol_k", "plane_k")
+# BS, BB, BDH, BP, EV, BD); every energy is linear in these
+STIFFNESS_FIELDS = ("fs_k", "fb_k", "ls_k", "ms_k", "bs_k", "bb_k", "bdh_k", "bp_k", "cyl_exvol_k", "plane_k")
 
 
 def _stream() -> int:

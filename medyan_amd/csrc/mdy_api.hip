@@ -1,4 +1,5 @@
 // mdy_api.hip -- the C ABI of libmdy_hip.so (declared in include/mdy.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -16,6 +17,7 @@ Dev Ctx::dev() const {
     D.ndof = 3 * nb;
     D.n_ev = have_nl ? n_ev : 0;
     D.n_fs = n_fs; D.n_fb = n_fb; D.n_lm = n_ls + n_ms; D.n_ls = n_ls; D.n_bs = n_bs; D.n_bb = n_bb;
+    D.n_bdh = n_bdh; D.n_bp = n_bp;
     D.n_bd = have_nl ? n_bd : 0;
     D.ev_bs = ev_bs.p; D.ev_kr = ev_kr.p;
     D.fs_bs = fs_bs.p; D.fs_k = fs_k.p; D.fs_l0 = fs_l0.p;
@@ -23,17 +25,23 @@ Dev Ctx::dev() const {
     D.lm_bs = lm_bs.p; D.lm_k = lm_k.p; D.lm_l0 = lm_l0.p; D.lm_p1 = lm_p1.p; D.lm_p2 = lm_p2.p;
     D.bs_bs = bs_bs.p; D.bs_k = bs_k.p; D.bs_l0 = bs_l0.p; D.bs_p = bs_p.p;
     D.bb_bs = bb_bs.p; D.bb_k = bb_k.p; D.bb_th = bb_th.p; D.bb_ct = bb_ct.p; D.bb_st = bb_st.p;
+    D.bdh_bs = bdh_bs.p; D.bdh_k = bdh_k.p; D.bdh_pos = bdh_pos.p;
+    D.bp_bs = bp_bs.p; D.bp_k = bp_k.p; D.bp_pos = bp_pos.p;
     D.bd_bead = bd_bead.p; D.bd_plane = bd_plane.p; D.bd_k = bd_k.p; D.bd_sl = bd_sl.p; D.planes = planes.p;
     D.slot_fs = 0;
     D.slot_fb = D.slot_fs + 2 * n_fs;
     D.slot_lm = D.slot_fb + 3 * n_fb;
     D.slot_bs = D.slot_lm + 4 * (n_ls + n_ms);
     D.slot_bb = D.slot_bs + 3 * n_bs;
-    D.slot_ev = D.slot_bb + 4 * n_bb;
+    D.slot_bdh = D.slot_bb + 4 * n_bb;
+    D.slot_bp = D.slot_bdh + 4 * n_bdh;
+    D.slot_ev = D.slot_bp + 3 * n_bp;
     D.slot_bd = D.slot_ev + 4 * D.n_ev;
     D.n_slots = D.slot_bd + D.n_bd;
     D.st_off = st_off.p; D.st_idx = st_idx.p; D.dy_off = dy_off.p; D.dy_idx = dy_idx.p;
     D.coplanar_mode = opts.coplanar_mode;
+    D.cq = std::cos(0.5 * M_PI);   // BranchingPositionCosine.cpp:235, 372 (posheta = pi/2)
+    D.sq = std::sin(0.5 * M_PI);
     D.qe = qe.p; D.qe_n = qn.p; D.qf = qf.p; D.qf_n = qn.p + 1; D.err = err.p;
     return D;
 }
@@ -124,6 +132,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.lm_bs); release(c.lm_k); release(c.lm_l0); release(c.lm_p1); release(c.lm_p2);
     release(c.bs_bs); release(c.bs_k); release(c.bs_l0); release(c.bs_p); release(c.bb_bs); release(c.bb_k);
     release(c.bb_th); release(c.st_off); release(c.st_idx);
+    release(c.bdh_bs); release(c.bdh_k); release(c.bdh_pos); release(c.bp_bs); release(c.bp_k); release(c.bp_pos);
     release(c.c_b1); release(c.c_b2); release(c.c_fil); release(c.c_pos); release(c.c_child); release(c.c_id);
     release(c.c_type); release(c.c_eq); release(c.c_ek); release(c.c_minus); release(c.c_plus);
     release(c.planes); release(c.plane_k); release(c.plane_sl);
@@ -158,12 +167,13 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
     const mdy_ff_tables &t = s->ff;
     const mdy_cylinders &cy = s->cyl;
     if (nb <= 0) return fail(ctx, MDY_ERR_ARG, "n_beads must be positive");
-    if (t.fs_n < 0 || t.fb_n < 0 || t.ls_n < 0 || t.ms_n < 0 || t.bs_n < 0 || t.bb_n < 0 || cy.n < 0 ||
-        s->bnd.n_planes < 0)
+    if (t.fs_n < 0 || t.fb_n < 0 || t.ls_n < 0 || t.ms_n < 0 || t.bs_n < 0 || t.bb_n < 0 || t.bdh_n < 0 ||
+        t.bp_n < 0 || cy.n < 0 || s->bnd.n_planes < 0)
         return fail(ctx, MDY_ERR_ARG, "negative count");
     if (bad_offsets(t.fs_beadset, 2 * (size_t)t.fs_n, nb) || bad_offsets(t.fb_beadset, 3 * (size_t)t.fb_n, nb) ||
         bad_offsets(t.ls_beadset, 4 * (size_t)t.ls_n, nb) || bad_offsets(t.ms_beadset, 4 * (size_t)t.ms_n, nb) ||
         bad_offsets(t.bs_beadset, 3 * (size_t)t.bs_n, nb) || bad_offsets(t.bb_beadset, 4 * (size_t)t.bb_n, nb) ||
+        bad_offsets(t.bdh_beadset, 4 * (size_t)t.bdh_n, nb) || bad_offsets(t.bp_beadset, 3 * (size_t)t.bp_n, nb) ||
         bad_offsets(cy.bead1, cy.n, nb) || bad_offsets(cy.bead2, cy.n, nb))
         return fail(ctx, MDY_ERR_ARG, "bead offset out of range (must be 3*index < 3*n_beads)");
     for (int i = 0; i < cy.n; i++)
@@ -178,6 +188,7 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
     c.have_nl = false;
     c.nb = nb;
     c.n_fs = t.fs_n; c.n_fb = t.fb_n; c.n_ls = t.ls_n; c.n_ms = t.ms_n; c.n_bs = t.bs_n; c.n_bb = t.bb_n;
+    c.n_bdh = t.bdh_n; c.n_bp = t.bp_n;
     int rc = MDY_OK;
     rc |= upload(c.fs_bs, reinterpret_cast<const int2 *>(t.fs_beadset), t.fs_n, st);
     rc |= upload(c.fs_k, t.fs_k, t.fs_n, st);
@@ -222,6 +233,12 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
     rc |= upload(c.bb_bs, reinterpret_cast<const int4 *>(t.bb_beadset), t.bb_n, st);
     rc |= upload(c.bb_k, t.bb_k, t.bb_n, st);
     rc |= upload(c.bb_th, t.bb_theta0, t.bb_n, st);
+    rc |= upload(c.bdh_bs, reinterpret_cast<const int4 *>(t.bdh_beadset), t.bdh_n, st);
+    rc |= upload(c.bdh_k, t.bdh_k, t.bdh_n, st);
+    rc |= upload(c.bdh_pos, t.bdh_pos, t.bdh_n, st);
+    rc |= upload(c.bp_bs, t.bp_beadset, 3 * (size_t)t.bp_n, st);
+    rc |= upload(c.bp_k, t.bp_k, t.bp_n, st);
+    rc |= upload(c.bp_pos, t.bp_pos, t.bp_n, st);
     // static CSR (bead -> bonded / cross-link slots, rows in slot order)
     {
         std::vector<int> owner;
@@ -233,6 +250,8 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
         for (size_t i = 0; i < 4 * (size_t)t.ms_n; i++) owner.push_back(t.ms_beadset[i] / 3);
         for (size_t i = 0; i < 3 * (size_t)t.bs_n; i++) owner.push_back(t.bs_beadset[i] / 3);
         for (size_t i = 0; i < 4 * (size_t)t.bb_n; i++) owner.push_back(t.bb_beadset[i] / 3);
+        for (size_t i = 0; i < 4 * (size_t)t.bdh_n; i++) owner.push_back(t.bdh_beadset[i] / 3);
+        for (size_t i = 0; i < 3 * (size_t)t.bp_n; i++) owner.push_back(t.bp_beadset[i] / 3);
         std::vector<int> off(nb + 1, 0), idx(owner.size()), cur;
         for (int o : owner) off[o + 1]++;
         for (int b = 0; b < nb; b++) off[b + 1] += off[b];
@@ -516,7 +535,8 @@ int mdy_stretch_forces(mdy_ctx *ctx, const double *coord, double *linker, double
     if (!c.have_sys) return fail(ctx, MDY_ERR_STATE, "no network set");
     (void)hipSetDevice(c.device);
     const Dev D = c.dev();
-    const size_t n = (size_t)D.n_ls + (D.n_lm - D.n_ls) + 3 * (size_t)D.n_bs;
+    const int nbr = std::max(D.n_bs, std::max(D.n_bdh, D.n_bp));
+    const size_t n = (size_t)D.n_ls + (D.n_lm - D.n_ls) + 3 * (size_t)nbr;
     if (ensure(c.out, n + 16) != MDY_OK) return hipfail(ctx, "allocation");
     double *dl = c.out.p, *dm = dl + D.n_ls, *db = dm + (D.n_lm - D.n_ls);
     if (hipMemcpyAsync(c.xt.p, coord, sizeof(double) * 3 * c.nb, hipMemcpyHostToDevice, c.stream) != hipSuccess)
@@ -527,8 +547,8 @@ int mdy_stretch_forces(mdy_ctx *ctx, const double *coord, double *linker, double
         (motor && D.n_lm > D.n_ls &&
          hipMemcpyAsync(motor, dm, sizeof(double) * (D.n_lm - D.n_ls), hipMemcpyDeviceToHost, c.stream) !=
              hipSuccess) ||
-        (branch3 && D.n_bs &&
-         hipMemcpyAsync(branch3, db, sizeof(double) * 3 * D.n_bs, hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
+        (branch3 && nbr &&
+         hipMemcpyAsync(branch3, db, sizeof(double) * 3 * nbr, hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
         hipStreamSynchronize(c.stream) != hipSuccess)
         return hipfail(ctx, "stretch force readback");
     return MDY_OK;

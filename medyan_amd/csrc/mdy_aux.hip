@@ -9,6 +9,8 @@
 //     (BoundaryCylinderRepulsion.cpp:124-189): the boundary load on every
 //     monomer position ahead of each filament end in a boundary plane's
 //     neighbor list.
+#include <algorithm>
+
 #include "mdy_eval.hpp"
 #include "mdy_launch.hpp"
 
@@ -16,7 +18,8 @@ namespace mdy {
 
 __global__ void __launch_bounds__(kBlock) k_stretch_forces(Dev D, const double *__restrict__ x, double *linker,
                                                             double *motor, double *branch3) {
-    const int n = D.n_lm + D.n_bs;
+    const int nbr = max(D.n_bs, max(D.n_bdh, D.n_bp));
+    const int n = D.n_lm + nbr;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < D.n_lm) {
             const int4 b = D.lm_bs[i];
@@ -28,20 +31,37 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(Dev D, const double *
             const double s = f0 / invL;
             if (i < D.n_ls) { if (linker) linker[i] = s; }
             else if (motor) motor[i - D.n_ls] = s;
-        } else {
+        } else if (branch3) {
+            // branch force: the stretching, dihedral and position contributions
+            // of branch point j added in force-field order, starting from 0
             const int j = i - D.n_lm;
-            const int *b = D.bs_bs + 3 * j;
-            const d3 v1 = midpt(ld3(x + b[0]), ld3(x + b[1]), D.bs_p[j]);
-            const d3 c3 = ld3(x + b[2]);
-            const double dist = tpdist(v1, c3);
-            const double invL = 1 / dist;
-            const double f0 = D.bs_k[j] * (dist - D.bs_l0[j]) * invL;
-            const d3 r = sub(c3, v1);
-            if (branch3) {
-                branch3[3 * j] = -f0 * (r.x);
-                branch3[3 * j + 1] = -f0 * (r.y);
-                branch3[3 * j + 2] = -f0 * (r.z);
+            d3 acc{0.0, 0.0, 0.0};
+            if (j < D.n_bs) {
+                const int *b = D.bs_bs + 3 * j;
+                const d3 v1 = midpt(ld3(x + b[0]), ld3(x + b[1]), D.bs_p[j]);
+                const d3 c3 = ld3(x + b[2]);
+                const double dist = tpdist(v1, c3);
+                const double invL = 1 / dist;
+                const double f0 = D.bs_k[j] * (dist - D.bs_l0[j]) * invL;
+                const d3 r = sub(c3, v1);
+                acc = d3{acc.x + -f0 * (r.x), acc.y + -f0 * (r.y), acc.z + -f0 * (r.z)};
             }
+            if (j < D.n_bdh) {
+                const int4 b = D.bdh_bs[j];
+                d3 f1, f2, f3, f4;
+                bdh_force(ld3(x + b.x), ld3(x + b.y), ld3(x + b.z), ld3(x + b.w), D.bdh_k[j], D.bdh_pos[j], f1, f2, f3,
+                          f4);
+                acc = d3{acc.x + f3.x, acc.y + f3.y, acc.z + f3.z};
+            }
+            if (j < D.n_bp) {
+                const int *b = D.bp_bs + 3 * j;
+                d3 f1, f2, f3;
+                bp_force(ld3(x + b[0]), ld3(x + b[1]), ld3(x + b[2]), D.bp_k[j], D.bp_pos[j], D.cq, D.sq, f1, f2, f3);
+                acc = d3{acc.x + f3.x, acc.y + f3.y, acc.z + f3.z};
+            }
+            branch3[3 * j] = acc.x;
+            branch3[3 * j + 1] = acc.y;
+            branch3[3 * j + 2] = acc.z;
         }
     }
 }
@@ -109,7 +129,8 @@ static int grid_n(const Ctx &c, long long n) {
 int launch_stretch_forces(Ctx &c, const double *d_x, double *d_linker, double *d_motor, double *d_branch,
                           hipStream_t st) {
     Dev D = c.dev();
-    hipLaunchKernelGGL(k_stretch_forces, dim3(grid_n(c, (long long)D.n_lm + D.n_bs)), dim3(kBlock), 0, st, D, d_x,
+    const long long nbr = std::max(D.n_bs, std::max(D.n_bdh, D.n_bp));
+    hipLaunchKernelGGL(k_stretch_forces, dim3(grid_n(c, (long long)D.n_lm + nbr)), dim3(kBlock), 0, st, D, d_x,
                        d_linker, d_motor, d_branch);
     return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
 }

@@ -252,6 +252,154 @@ __device__ __forceinline__ void bs_force(V3<T> c1, V3<T> c2, V3<T> c3, double k,
 }
 
 // ---------------------------------------------------------------------------
+// Branching position and dihedral, cosine forms. A branch point at the
+// mother's plus end (pos == 1 within ZERO_PREC) is placed at 0.5 of a
+// virtual cylinder whose plus end c2' = (1/0.5) (c2 - 0.5 c1) extends the
+// mother, and the forces are mapped back to c1, c2 afterwards.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool are_equal1(double a) { return fabs(a - 1.0) < 1e-6; }
+template <class T> __device__ __forceinline__ V3<T> branch_plus_end(V3<T> c1, V3<T> c2, double position) {
+    return V3<T>{(1 / position) * (c2.x - (1 - position) * c1.x), (1 / position) * (c2.y - (1 - position) * c1.y),
+                 (1 / position) * (c2.z - (1 - position) * c1.z)};
+}
+// BranchingPositionCosine::energy (BranchingPositionCosine.cpp:190-248);
+// cq, sq: cos and sin of pi/2 from the host C library
+template <class T>
+__device__ __forceinline__ T bp_energy(V3<T> c1, V3<T> c2, V3<T> c3, double kpos, double pos, double cq, double sq) {
+    double position = pos;
+    if (are_equal1(position)) {
+        position = 0.5;
+        c2 = branch_plus_end(c1, c2, position);
+    }
+    const V3<T> mp = midpt(c1, c2, position);
+    const T X = xsqrt(sprod(mp, c2, mp, c2));
+    const T D = xsqrt(sprod(mp, c3, mp, c3));
+    const T XD = X * D;
+    T xd = sprod(mp, c2, mp, c3);
+    T x = xd / XD;
+    if (xfabs(xfabs(x) - 1.0) < 0.001) {
+        xd = 0.999 * XD;
+        x = xd / XD;
+    }
+    if (x < -1.0) x = T(-1.0);
+    else if (x > 1.0) x = T(1.0);
+    const T cosp = x;
+    const T sinp = xsqrt(smax(T(1 - cosp * cosp), T(0.0)));
+    const T cospminusq = cosp * cq + sinp * sq;
+    return kpos * (1 - cospminusq);
+}
+// BranchingPositionCosine::forces (BranchingPositionCosine.cpp:306-531):
+// forces on c1, c2, c3
+template <class T>
+__device__ __forceinline__ void bp_force(V3<T> c1, V3<T> c2, V3<T> c3, double kpos, double pos, double cq, double sq,
+                                         V3<T> &f1, V3<T> &f2, V3<T> &f3) {
+    double position = pos;
+    if (are_equal1(position)) {
+        position = 0.5;
+        c2 = branch_plus_end(c1, c2, position);
+    }
+    const V3<T> mp = midpt(c1, c2, position);
+    const T Xsq = sprod(mp, c2, mp, c2);
+    const T Dsq = sprod(mp, c3, mp, c3);
+    const T xd = sprod(mp, c2, mp, c3);
+    const T A = 1 / xsqrt(Xsq * Dsq);
+    T x = xd * A;
+    const T B = x / Xsq;
+    const T C = x / Dsq;
+    if (xfabs(xfabs(x) - 1.0) < 0.001) x = 0.999 * x;
+    if (x < -1.0) x = T(-1.0);
+    else if (x > 1.0) x = T(1.0);
+    const T cosp = x;
+    const T sinp = xsqrt(smax(T(1 - cosp * cosp), T(0.0)));
+    const T sinpminusq = sinp * cq - cosp * sq;
+    const T k = kpos * sinpminusq / sinp;
+    const V3<T> r1 = sub(c2, mp), r2 = sub(c3, mp);
+    const V3<T> Fr1{k * (r2.x * A - r1.x * B), k * (r2.y * A - r1.y * B), k * (r2.z * A - r1.z * B)};
+    const V3<T> Fr2{k * (r1.x * A - r2.x * C), k * (r1.y * A - r2.y * C), k * (r1.z * A - r2.z * C)};
+    const V3<T> fmp{-Fr1.x - Fr2.x, -Fr1.y - Fr2.y, -Fr1.z - Fr2.z};
+    const V3<T> f1t{(1 - position) * fmp.x, (1 - position) * fmp.y, (1 - position) * fmp.z};
+    const V3<T> f2t{Fr1.x + position * fmp.x, Fr1.y + position * fmp.y, Fr1.z + position * fmp.z};
+    f3 = Fr2;
+    if (are_equal1(pos)) {
+        const double factor = (position - 1) / position;
+        f1 = V3<T>{f1t.x + f2t.x * factor, f1t.y + f2t.y * factor, f1t.z + f2t.z * factor};
+        f2 = V3<T>{f2t.x * (1 / position), f2t.y * (1 / position), f2t.z * (1 / position)};
+    } else {
+        f1 = f1t;
+        f2 = f2t;
+    }
+}
+// BranchingDihedralCosine::energy (BranchingDihedralCosine.cpp:211-269)
+template <class T>
+__device__ __forceinline__ T bdh_energy(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double kdih, double pos) {
+    double position = pos;
+    if (are_equal1(position)) {
+        position = 0.5;
+        c2 = branch_plus_end(c1, c2, position);
+    }
+    const V3<T> mp = midpt(c1, c2, position);
+    const V3<T> n1 = normalize3(cross3(sub(c2, mp), sub(c3, mp)));
+    const V3<T> n2 = normalize3(cross3(sub(c4, c3), sub(c3, mp)));
+    const T n1n2 = n1.x * n2.x + n1.y * n2.y + n1.z * n2.z;
+    return kdih * (1 - n1n2);
+}
+// BranchingDihedralCosine::forces (BranchingDihedralCosine.cpp:271-661),
+// the LAMMPS dihedral form: forces on c1..c4
+template <class T>
+__device__ __forceinline__ void bdh_force(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double kdih, double pos, V3<T> &f1,
+                                          V3<T> &f2, V3<T> &f3, V3<T> &f4) {
+    double position = pos;
+    if (are_equal1(position)) {
+        position = 0.5;
+        c2 = branch_plus_end(c1, c2, position);
+    }
+    const V3<T> mp = midpt(c1, c2, position);
+    const V3<T> vb1 = sub(c2, mp), vb2 = sub(c3, mp);
+    const V3<T> vb2m{-vb2.x, -vb2.y, -vb2.z};
+    const V3<T> vb3 = sub(c4, c3);
+    const T ax = vb1.y * vb2m.z - vb1.z * vb2m.y, ay = vb1.z * vb2m.x - vb1.x * vb2m.z,
+            az = vb1.x * vb2m.y - vb1.y * vb2m.x;
+    const T bx = vb3.y * vb2m.z - vb3.z * vb2m.y, by = vb3.z * vb2m.x - vb3.x * vb2m.z,
+            bz = vb3.x * vb2m.y - vb3.y * vb2m.x;
+    const T rasq = ax * ax + ay * ay + az * az;
+    const T rbsq = bx * bx + by * by + bz * bz;
+    const T rgsq = vb2m.x * vb2m.x + vb2m.y * vb2m.y + vb2m.z * vb2m.z;
+    const T rg = xsqrt(rgsq);
+    T rginv = T(0.0), ra2inv = T(0.0), rb2inv = T(0.0);
+    if (rg > 0) rginv = 1.0 / rg;
+    if (rasq > 0) ra2inv = 1.0 / rasq;
+    if (rbsq > 0) rb2inv = 1.0 / rbsq;
+    const T rabinv = xsqrt(ra2inv * rb2inv);
+    const T s = rg * rabinv * (ax * vb3.x + ay * vb3.y + az * vb3.z);
+    const T fg = vb1.x * vb2m.x + vb1.y * vb2m.y + vb1.z * vb2m.z;
+    const T hg = vb3.x * vb2m.x + vb3.y * vb2m.y + vb3.z * vb2m.z;
+    const T fga = fg * ra2inv * rginv;
+    const T hgb = hg * rb2inv * rginv;
+    const T gaa = -ra2inv * rg;
+    const T gbb = rb2inv * rg;
+    const V3<T> dtf{gaa * ax, gaa * ay, gaa * az};
+    const V3<T> dtg{fga * ax - hgb * bx, fga * ay - hgb * by, fga * az - hgb * bz};
+    const V3<T> dth{gbb * bx, gbb * by, gbb * bz};
+    const T df = -kdih * s;
+    const V3<T> s2{df * dtg.x, df * dtg.y, df * dtg.z};
+    f4 = V3<T>{df * dth.x, df * dth.y, df * dth.z};
+    f3 = V3<T>{-s2.x - f4.x, -s2.y - f4.y, -s2.z - f4.z};
+    if (are_equal1(pos)) {
+        const V3<T> l2p{df * dtf.x, df * dtf.y, df * dtf.z};
+        V3<T> l2{s2.x - l2p.x, s2.y - l2p.y, s2.z - l2p.z};
+        const double factor = (position - 1) / position;
+        f1 = V3<T>{l2p.x * factor, l2p.y * factor, l2p.z * factor};
+        const double factor2 = (1 / position);
+        f2 = V3<T>{l2.x + l2p.x * factor2, l2.y + l2p.y * factor2, l2.z + l2p.z * factor2};
+    } else {
+        const V3<T> l2{df * dtf.x, df * dtf.y, df * dtf.z};
+        const V3<T> lmp{s2.x - l2.x, s2.y - l2.y, s2.z - l2.z};
+        f1 = V3<T>{(1 - position) * lmp.x, (1 - position) * lmp.y, (1 - position) * lmp.z};
+        f2 = V3<T>{l2.x + (position)*lmp.x, l2.y + (position)*lmp.y, l2.z + (position)*lmp.z};
+    }
+}
+
+// ---------------------------------------------------------------------------
 // a10 BoundaryCylinderRepulsionExp (BoundaryCylinderRepulsionExp.cpp:27-110)
 // plane = (a, b, c, d); distance per BoundaryElementImpl.h:71-74
 // ---------------------------------------------------------------------------

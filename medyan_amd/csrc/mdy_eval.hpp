@@ -84,7 +84,7 @@ template <> struct SlotT<true> { using T = float; };
 // BD]. An item's record (its bead offsets and parameters) and its beads'
 // coordinates are fetched one item ahead of the one being computed, so each
 // thread's chain of dependent memory accesses overlaps with its arithmetic.
-enum { IT_EV = 0, IT_FS, IT_FB, IT_LM, IT_BS, IT_BB, IT_BD, IT_NONE };
+enum { IT_EV = 0, IT_FS, IT_FB, IT_LM, IT_BS, IT_BB, IT_BDH, IT_BP, IT_BD, IT_NONE };
 struct ItemRec {
     int type, j;      // kind and index within its table
     int4 b;           // bead offsets (x3); unused entries repeat the first
@@ -120,7 +120,16 @@ __device__ __forceinline__ ItemRec fetch_rec(const Dev &D, int i) {
         r.type = IT_BB; r.j = i;
         r.b = D.bb_bs[i];
         r.p0 = D.bb_k[i]; r.p1 = D.bb_th[i]; r.p2 = D.bb_ct[i]; r.p3 = D.bb_st[i];
-    } else if ((i -= D.n_bb) < D.n_bd) {
+    } else if ((i -= D.n_bb) < D.n_bdh) {
+        r.type = IT_BDH; r.j = i;
+        r.b = D.bdh_bs[i];
+        r.p0 = D.bdh_k[i]; r.p1 = D.bdh_pos[i];
+    } else if ((i -= D.n_bdh) < D.n_bp) {
+        r.type = IT_BP; r.j = i;
+        const int *b = D.bp_bs + 3 * i;
+        r.b = make_int4(b[0], b[1], b[2], b[0]);
+        r.p0 = D.bp_k[i]; r.p1 = D.bp_pos[i];
+    } else if ((i -= D.n_bp) < D.n_bd) {
         r.type = IT_BD; r.j = i;
         const int bead = D.bd_bead[i];
         r.b = make_int4(bead, D.bd_plane[i], bead, bead);
@@ -132,7 +141,8 @@ __device__ __forceinline__ ItemRec fetch_rec(const Dev &D, int i) {
     return r;
 }
 __device__ __forceinline__ int rec_nbeads(int type) {
-    return type == IT_EV || type == IT_LM || type == IT_BB ? 4 : type == IT_FB || type == IT_BS ? 3
+    return type == IT_EV || type == IT_LM || type == IT_BB || type == IT_BDH ? 4
+         : type == IT_FB || type == IT_BS || type == IT_BP ? 3
          : type == IT_FS ? 2 : type == IT_BD ? 1 : 0;
 }
 // bead k's position (the unused entries of a record repeat its first bead)
@@ -253,6 +263,24 @@ __device__ __forceinline__ void item_compute(const Dev &D, const ItemRec &r, con
             bb_force(c[0], c[1], c[2], c[3], r.p0, r.p1, r.p2, r.p3, f1, f2, f3, f4);
             int s = D.slot_bb + 4 * j;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
+        }
+    } else if (r.type == IT_BDH) {
+        ff = MDY_FF_BDH;
+        U = bdh_energy(c[0], c[1], c[2], c[3], r.p0, r.p1);
+        if (FORCES) {
+            d3 f1, f2, f3, f4;
+            bdh_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f1, f2, f3, f4);
+            int s = D.slot_bdh + 4 * j;
+            put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
+        }
+    } else if (r.type == IT_BP) {
+        ff = MDY_FF_BP;
+        U = bp_energy(c[0], c[1], c[2], r.p0, r.p1, D.cq, D.sq);
+        if (FORCES) {
+            d3 f1, f2, f3;
+            bp_force(c[0], c[1], c[2], r.p0, r.p1, D.cq, D.sq, f1, f2, f3);
+            int s = D.slot_bp + 3 * j;
+            put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3);
         }
     } else if (r.type == IT_BD) {
         const double *pl = D.planes + 4 * r.b.y;
@@ -467,8 +495,9 @@ __device__ __forceinline__ void grid_partials_reduce(const double *red, int nblk
 // both, in increasing pair order. e[0..7] per FF (reported values), e[8] =
 // computeEnergy total (+inf when a force field returned its -1 sentinel),
 // e[9] = the sum without the sentinel.
-constexpr int kXsE = 8;    // exact energy total in a reduction record
-constexpr int kXsA = 16;   // second exact accumulator in a reduction record
+constexpr int kXsE = 16;   // exact energy total in a reduction record (after the kNFF per-FF sums)
+constexpr int kXsA = 24;   // second exact accumulator in a reduction record
+static_assert(kNFF <= kXsE && kXsA + 8 <= kRed, "reduction record layout");
 __device__ __forceinline__ void energy_finish(mdy_xsum X, const double v_in[kNFF], int nq, const int *qe,
                                               const double *qe_u, double e[kNFF + 2]) {
     double v[kNFF];

@@ -37,6 +37,12 @@ __device__ __forceinline__ int item_forces(const Dev &D, const ItemRec &r, const
     } else if (r.type == IT_BB) {
         bb_force(c[0], c[1], c[2], c[3], r.p0, r.p1, r.p2, r.p3, f[0], f[1], f[2], f[3]);
         return MDY_FF_BB;
+    } else if (r.type == IT_BDH) {
+        bdh_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f[0], f[1], f[2], f[3]);
+        return MDY_FF_BDH;
+    } else if (r.type == IT_BP) {
+        bp_force(c[0], c[1], c[2], r.p0, r.p1, D.cq, D.sq, f[0], f[1], f[2]);
+        return MDY_FF_BP;
     } else if (r.type == IT_BD) {
         f[0] = bd_force(D.planes + 4 * r.b.y, c[0], r.p0, r.p1);
         return MDY_FF_BD;
@@ -52,6 +58,8 @@ __device__ __forceinline__ int item_slot(const Dev &D, const ItemRec &r) {
         case IT_LM: return D.slot_lm + 4 * r.j;
         case IT_BS: return D.slot_bs + 3 * r.j;
         case IT_BB: return D.slot_bb + 4 * r.j;
+        case IT_BDH: return D.slot_bdh + 4 * r.j;
+        case IT_BP: return D.slot_bp + 3 * r.j;
         default: return D.slot_bd + r.j;
     }
 }

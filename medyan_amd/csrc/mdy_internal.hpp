@@ -23,23 +23,26 @@ constexpr int kRed = 32;             // 8-byte words per block in a reduction re
 // the order ForceFieldManager::computeForces would.
 struct Dev {
     int nb, ndof;
-    int n_ev, n_fs, n_fb, n_lm, n_ls, n_bs, n_bb, n_bd;
+    int n_ev, n_fs, n_fb, n_lm, n_ls, n_bs, n_bb, n_bdh, n_bp, n_bd;
     const int4 *ev_bs; const double *ev_kr;
     const int2 *fs_bs; const double *fs_k, *fs_l0;
     const int *fb_bs; const double *fb_k, *fb_th, *fb_ct, *fb_st;   // ct/st: cos/sin(theta0) from the host
     const int4 *lm_bs; const double *lm_k, *lm_l0, *lm_p1, *lm_p2;
     const int *bs_bs; const double *bs_k, *bs_l0, *bs_p;
     const int4 *bb_bs; const double *bb_k, *bb_th, *bb_ct, *bb_st;
+    const int4 *bdh_bs; const double *bdh_k, *bdh_pos;           // branching dihedral
+    const int *bp_bs; const double *bp_k, *bp_pos;               // branching position
     const int *bd_bead; const int *bd_plane; const double *bd_k, *bd_sl; const double *planes;
-    int slot_fs, slot_fb, slot_lm, slot_bs, slot_bb, slot_ev, slot_bd, n_slots;
+    int slot_fs, slot_fb, slot_lm, slot_bs, slot_bb, slot_bdh, slot_bp, slot_ev, slot_bd, n_slots;
     const int *st_off, *st_idx;   // static CSR: bead -> bonded/cross-link slots
     const int *dy_off, *dy_idx;   // dynamic CSR: bead -> excluded-volume/boundary slots
     int coplanar_mode;
+    double cq, sq;                // cos and sin of pi/2 (C library), branching position
     // excluded-volume quadrature fallback queues (pair indices)
     int *qe, *qe_n;               // energy
     int *qf, *qf_n;               // force
     int *err;                     // sticky device error word (MDY_ERR_*)
-    __host__ __device__ int n_items() const { return n_ev + n_fs + n_fb + n_lm + n_bs + n_bb + n_bd; }
+    __host__ __device__ int n_items() const { return n_ev + n_fs + n_fb + n_lm + n_bs + n_bb + n_bdh + n_bp + n_bd; }
 };
 
 template <class T>
@@ -66,8 +69,10 @@ struct Ctx {
     DBuf<int4> lm_bs; DBuf<double> lm_k, lm_l0, lm_p1, lm_p2;
     DBuf<int> bs_bs; DBuf<double> bs_k, bs_l0, bs_p;
     DBuf<int4> bb_bs; DBuf<double> bb_k, bb_th, bb_ct, bb_st;
+    DBuf<int4> bdh_bs; DBuf<double> bdh_k, bdh_pos;
+    DBuf<int> bp_bs; DBuf<double> bp_k, bp_pos;
     DBuf<int> st_off, st_idx;
-    int n_fs = 0, n_fb = 0, n_ls = 0, n_ms = 0, n_bs = 0, n_bb = 0;
+    int n_fs = 0, n_fb = 0, n_ls = 0, n_ms = 0, n_bs = 0, n_bb = 0, n_bdh = 0, n_bp = 0;
     // cylinders
     DBuf<int> c_b1, c_b2, c_fil, c_pos, c_child; DBuf<long long> c_id; DBuf<short> c_type;
     DBuf<double> c_eq, c_ek; DBuf<unsigned char> c_minus, c_plus;   // c_plus: last cylinder of its filament

@@ -153,6 +153,8 @@ struct Storage {
     std::vector<int32_t> fs_bs, fb_bs, ls_bs, ms_bs, bs_bs, bb_bs;
     std::vector<double> fs_k, fs_l0, fb_k, fb_t0, ls_k, ls_l0, ls_p1, ls_p2, ms_k, ms_l0, ms_p1, ms_p2;
     std::vector<double> bs_k, bs_l0, bs_p, bb_k, bb_t0;
+    std::vector<int32_t> dh_bs, bp_bs;
+    std::vector<double> dh_k, dh_p, bp_k, bp_p;
     std::vector<double> plane, pk, ps;
 };
 
@@ -180,6 +182,7 @@ extern "C" void mdy_gen_defaults(mdy_gen_params *p) {
     p->k_linker = 8.0; p->k_motor = 2.5;
     p->k_br_stretch = 100.0; p->l_br_stretch = 6.0; p->k_br_bend = 10.0;
     p->theta_br_bend = 70.0 * M_PI / 180.0;
+    p->k_br_dihedral = 0.0; p->k_br_position = 0.0;
 }
 
 extern "C" int mdy_generate(const mdy_gen_params *p, mdy_network **out) {
@@ -369,6 +372,16 @@ extern "C" int mdy_generate(const mdy_gen_params *p, mdy_network **out) {
         st->bb_bs.push_back(st->cb1[mc]); st->bb_bs.push_back(st->cb2[mc]);
         st->bb_bs.push_back(st->cb1[dc]); st->bb_bs.push_back(st->cb2[dc]);
         st->bb_k.push_back(p->k_br_bend); st->bb_t0.push_back(p->theta_br_bend);
+        // branching dihedral / position (BranchingDihedral.cpp:41-50, BranchingPosition.cpp:37-45)
+        if (p->k_br_dihedral != 0.0) {
+            st->dh_bs.push_back(st->cb1[mc]); st->dh_bs.push_back(st->cb2[mc]);
+            st->dh_bs.push_back(st->cb1[dc]); st->dh_bs.push_back(st->cb2[dc]);
+            st->dh_k.push_back(p->k_br_dihedral); st->dh_p.push_back(branch_pos[q]);
+        }
+        if (p->k_br_position != 0.0) {
+            st->bp_bs.push_back(st->cb1[mc]); st->bp_bs.push_back(st->cb2[mc]); st->bp_bs.push_back(st->cb1[dc]);
+            st->bp_k.push_back(p->k_br_position); st->bp_p.push_back(branch_pos[q]);
+        }
     }
     // boundary planes, BoundaryCubic (BoundaryImpl.cpp:25-52) and the plane
     // equation of PlaneBoundaryElement (BoundaryElementImpl.h:44-49)
@@ -411,6 +424,10 @@ extern "C" int mdy_generate(const mdy_gen_params *p, mdy_network **out) {
     t.bs_n = (int32_t)st->bs_k.size(); t.bs_beadset = st->bs_bs.data(); t.bs_k = st->bs_k.data(); t.bs_l0 = st->bs_l0.data();
     t.bs_pos = st->bs_p.data();
     t.bb_n = (int32_t)st->bb_k.size(); t.bb_beadset = st->bb_bs.data(); t.bb_k = st->bb_k.data(); t.bb_theta0 = st->bb_t0.data();
+    t.bdh_n = (int32_t)st->dh_k.size(); t.bdh_beadset = st->dh_bs.data(); t.bdh_k = st->dh_k.data();
+    t.bdh_pos = st->dh_p.data();
+    t.bp_n = (int32_t)st->bp_k.size(); t.bp_beadset = st->bp_bs.data(); t.bp_k = st->bp_k.data();
+    t.bp_pos = st->bp_p.data();
     s.bnd.n_planes = 6; s.bnd.plane = st->plane.data(); s.bnd.k = st->pk.data(); s.bnd.screen = st->ps.data();
     s.bnd.cutoff = p->bd_cutoff;
     s.exvol_cutoff = p->exvol_cutoff;

@@ -268,9 +268,10 @@ class Context:
         """assignallforcemags outputs at coord: (linker, motor, branch [n x 3])."""
         x = np.ascontiguousarray(coord, dtype=np.float64)
         n = self.net
-        lk, mt, br = np.zeros(max(len(n.ls_k), 1)), np.zeros(max(len(n.ms_k), 1)), np.zeros(max(3 * len(n.bs_k), 1))
+        nbr = max(len(n.bs_k), len(n.bdh_k), len(n.bp_k))
+        lk, mt, br = np.zeros(max(len(n.ls_k), 1)), np.zeros(max(len(n.ms_k), 1)), np.zeros(max(3 * nbr, 1))
         self._check(self.lib.mdy_stretch_forces(self.h, _f64(x), _f64(lk), _f64(mt), _f64(br)))
-        return lk[: len(n.ls_k)], mt[: len(n.ms_k)], br[: 3 * len(n.bs_k)].reshape(-1, 3)
+        return lk[: len(n.ls_k)], mt[: len(n.ms_k)], br[: 3 * nbr].reshape(-1, 3)
 
     def load_forces(self, coord, mon_size=(2.7,), n_mon=(40,), max_mon=None):
         """computeLoadForces at coord (boundary neighbor list of the last

@@ -78,6 +78,13 @@ class Network:
     exvol_enabled: int = 1
     boundary_enabled: int = 1
     box: float = 0.0
+    # branching dihedral / position (empty unless those force fields are on)
+    bdh_beadset: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    bdh_k: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    bdh_pos: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    bp_beadset: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    bp_k: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    bp_pos: np.ndarray = field(default_factory=lambda: np.zeros(0))
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -131,6 +138,10 @@ class Network:
         t.bs_l0 = f64(self.bs_l0); t.bs_pos = f64(self.bs_pos)
         t.bb_n = len(self.bb_k); t.bb_beadset = i32(self.bb_beadset); t.bb_k = f64(self.bb_k)
         t.bb_theta0 = f64(self.bb_theta0)
+        t.bdh_n = len(self.bdh_k); t.bdh_beadset = i32(self.bdh_beadset); t.bdh_k = f64(self.bdh_k)
+        t.bdh_pos = f64(self.bdh_pos)
+        t.bp_n = len(self.bp_k); t.bp_beadset = i32(self.bp_beadset); t.bp_k = f64(self.bp_k)
+        t.bp_pos = f64(self.bp_pos)
         b = s.bnd
         b.n_planes = len(self.plane_k)
         b.plane, b.k, b.screen = f64(self.plane), f64(self.plane_k), f64(self.plane_screen)
@@ -173,6 +184,10 @@ def from_c(net: _abi.Network) -> Network:
         bs_l0=_arr(t.bs_l0, t.bs_n, np.float64), bs_pos=_arr(t.bs_pos, t.bs_n, np.float64),
         bb_beadset=_arr(t.bb_beadset, 4 * t.bb_n, np.int32), bb_k=_arr(t.bb_k, t.bb_n, np.float64),
         bb_theta0=_arr(t.bb_theta0, t.bb_n, np.float64),
+        bdh_beadset=_arr(t.bdh_beadset, 4 * t.bdh_n, np.int32), bdh_k=_arr(t.bdh_k, t.bdh_n, np.float64),
+        bdh_pos=_arr(t.bdh_pos, t.bdh_n, np.float64),
+        bp_beadset=_arr(t.bp_beadset, 3 * t.bp_n, np.int32), bp_k=_arr(t.bp_k, t.bp_n, np.float64),
+        bp_pos=_arr(t.bp_pos, t.bp_n, np.float64),
         plane=_arr(s.bnd.plane, 4 * s.bnd.n_planes, np.float64),
         plane_k=_arr(s.bnd.k, s.bnd.n_planes, np.float64),
         plane_screen=_arr(s.bnd.screen, s.bnd.n_planes, np.float64),
@@ -209,7 +224,7 @@ def generate(lib, workload: Optional[str] = None, **overrides) -> Network:
 def from_filaments(filaments, box, linkers=(), motors=(), branches=(), cyl_size=108.0, k_stretch=100.0,
                    k_bend=672.0, theta_bend=0.0, k_exvol=1e5, exvol_cutoff=108.0, k_bd=41.0, screen_bd=2.7,
                    bd_cutoff=125.0, k_br_stretch=100.0, l_br_stretch=6.0, k_br_bend=10.0,
-                   theta_br_bend=70.0 * np.pi / 180.0) -> Network:
+                   theta_br_bend=70.0 * np.pi / 180.0, k_br_dihedral=0.0, k_br_position=0.0) -> Network:
     """A network from explicit filament bead coordinates (one (n_i+1) x 3 array
     per filament, minus end first), with the generator's rules for the
     derived tables: one stretching term per cylinder, one bending term per
@@ -218,7 +233,8 @@ def from_filaments(filaments, box, linkers=(), motors=(), branches=(), cyl_size=
     linkers / motors: tuples (fil1, cyl1, pos1, fil2, cyl2, pos2, k, l0), the
     cylinder given by its index along its filament. branches: tuples
     (mother_fil, mother_cyl, pos, daughter_fil); the daughter filament's first
-    cylinder is the branch child.
+    cylinder is the branch child. k_br_dihedral / k_br_position > 0 add the
+    branching dihedral and position terms of every branch point.
     """
     fil = [np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 3) for f in filaments]
     first, coord = [], []
@@ -259,6 +275,7 @@ def from_filaments(filaments, box, linkers=(), motors=(), branches=(), cyl_size=
     ls = links(linkers)
     ms = links(motors)
     bs_bs, bs_k, bs_l0, bs_p, bb_bs, bb_k, bb_t = [], [], [], [], [], [], []
+    dh_bs, dh_k, dh_p, bp_bs, bp_k, bp_p = [], [], [], [], [], []
     for (mf, mc, q, df) in branches:
         m, d = cfirst[mf] + mc, cfirst[df]
         child[m] = d
@@ -266,6 +283,12 @@ def from_filaments(filaments, box, linkers=(), motors=(), branches=(), cyl_size=
         bs_k.append(k_br_stretch); bs_l0.append(l_br_stretch); bs_p.append(q)
         bb_bs += [b1[m], b2[m], b1[d], b2[d]]
         bb_k.append(k_br_bend); bb_t.append(theta_br_bend)
+        if k_br_dihedral:
+            dh_bs += [b1[m], b2[m], b1[d], b2[d]]
+            dh_k.append(k_br_dihedral); dh_p.append(q)
+        if k_br_position:
+            bp_bs += [b1[m], b2[m], b1[d]]
+            bp_k.append(k_br_position); bp_p.append(q)
     L = float(box)
     z0, hi = 25.0, L - 25.0
     pts = [(z0, hi / 2, hi / 2), (hi, hi / 2, hi / 2), (hi / 2, z0, hi / 2), (hi / 2, hi, hi / 2),
@@ -288,4 +311,6 @@ def from_filaments(filaments, box, linkers=(), motors=(), branches=(), cyl_size=
         bs_beadset=np.array(bs_bs, np.int32), bs_k=f64(bs_k), bs_l0=f64(bs_l0), bs_pos=f64(bs_p),
         bb_beadset=np.array(bb_bs, np.int32), bb_k=f64(bb_k), bb_theta0=f64(bb_t),
         plane=f64(plane), plane_k=np.full(6, k_bd), plane_screen=np.full(6, screen_bd),
-        bd_cutoff=float(bd_cutoff), exvol_cutoff=float(exvol_cutoff), box=L)
+        bd_cutoff=float(bd_cutoff), exvol_cutoff=float(exvol_cutoff), box=L,
+        bdh_beadset=np.array(dh_bs, np.int32), bdh_k=f64(dh_k), bdh_pos=f64(dh_p),
+        bp_beadset=np.array(bp_bs, np.int32), bp_k=f64(bp_k), bp_pos=f64(bp_p))

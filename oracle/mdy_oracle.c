@@ -27,6 +27,9 @@
  * the shared mdy_atan/mdy_exp and order-independent exact sums (mdy_xsum),
  * the numerics the HIP kernels use, so results can be compared bit for bit.
  * The two modes differ by rounding only (checked in tests/test_oracle_numerics.py). */
+#ifndef M_PI
+#define M_PI 3.14159265358979323846   /* the C library's value (not in strict C11) */
+#endif
 static int g_exact = 0;
 void orc_set_exact(int on) { g_exact = on; }
 int orc_get_exact(void) { return g_exact; }
@@ -319,6 +322,195 @@ void orc_bs_forces(const double *x, double *f, int32_t n, const int32_t *bs, con
             f2[d] += f0 * (r) * p[i];
             f3[d] += -f0 * (r);
             if (stretch3) stretch3[3 * i + d] = -f0 * (r);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Branching position / dihedral, cosine forms                               */
+/* BranchingPositionCosine.cpp:190-248 (energy), 306-531 (forces);           */
+/* BranchingDihedralCosine.cpp:211-269 (energy), 271-661 (forces).           */
+/* A branch point at the mother's plus end (areEqual(pos, 1)) sits at 0.5 of */
+/* a virtual cylinder with plus end c2' = (1/0.5)(c2 - 0.5 c1).               */
+/* ------------------------------------------------------------------------- */
+static inline int are_equal1(double a) { return fabs(a - 1.0) < 1e-6; }
+static void extend_plus(double *c2p, const double *c1, const double *c2, double position) {
+    for (int d = 0; d < 3; d++) c2p[d] = (1 / position) * (c2[d] - (1 - position) * c1[d]);
+}
+double orc_bp_energy(const double *x, int32_t n, const int32_t *bs, const double *kpos, const double *pos) {
+    osum U;
+    os_zero(&U);
+    const double posheta = 0.5 * M_PI;
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[3 * i], *c2 = x + bs[3 * i + 1], *c3 = x + bs[3 * i + 2];
+        double c2p[3], mp[3];
+        double position = pos[i];
+        if (are_equal1(position)) {
+            position = 0.5;
+            extend_plus(c2p, c1, c2, position);
+            c2 = c2p;
+        }
+        midpt(mp, c1, c2, position);
+        double X = sqrt(sprod(mp, c2, mp, c2));
+        double D = sqrt(sprod(mp, c3, mp, c3));
+        double XD = X * D;
+        double xd = sprod(mp, c2, mp, c3);
+        double xx = xd / XD;
+        if (fabs(fabs(xx) - 1.0) < 0.001) {
+            xd = 0.999 * XD;
+            xx = xd / XD;
+        }
+        if (xx < -1.0) xx = -1.0;
+        else if (xx > 1.0) xx = 1.0;
+        double cosp = xx;
+        double sinp = sqrt(smax(1 - cosp * cosp, 0.0));
+        double cospminusq = cosp * cos(posheta) + sinp * sin(posheta);
+        os_add(&U, kpos[i] * (1 - cospminusq));
+    }
+    return os_val(&U);
+}
+void orc_bp_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kpos, const double *pos,
+                   double *stretch3) {
+    const double posheta = 0.5 * M_PI;
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[3 * i], *c2 = x + bs[3 * i + 1], *c3 = x + bs[3 * i + 2];
+        double *f1 = f + bs[3 * i], *f2 = f + bs[3 * i + 1], *f3 = f + bs[3 * i + 2];
+        double c2p[3], mp[3];
+        double position = pos[i];
+        if (are_equal1(position)) {
+            position = 0.5;
+            extend_plus(c2p, c1, c2, position);
+            c2 = c2p;
+        }
+        midpt(mp, c1, c2, position);
+        double Xsq = sprod(mp, c2, mp, c2);
+        double Dsq = sprod(mp, c3, mp, c3);
+        double xd = sprod(mp, c2, mp, c3);
+        double A = 1 / sqrt(Xsq * Dsq);
+        double xx = xd * A;
+        double B = xx / Xsq;
+        double C = xx / Dsq;
+        if (fabs(fabs(xx) - 1.0) < 0.001) xx = 0.999 * xx;
+        if (xx < -1.0) xx = -1.0;
+        else if (xx > 1.0) xx = 1.0;
+        double cosp = xx;
+        double sinp = sqrt(smax(1 - cosp * cosp, 0.0));
+        double sinpminusq = sinp * cos(posheta) - cosp * sin(posheta);
+        double k = kpos[i] * sinpminusq / sinp;
+        double f1t[3], f2t[3], f3t[3];
+        for (int d = 0; d < 3; d++) {
+            double r1 = c2[d] - mp[d], r2 = c3[d] - mp[d];
+            double Fr1 = k * (r2 * A - r1 * B);
+            double Fr2 = k * (r1 * A - r2 * C);
+            double fmp = -Fr1 - Fr2;
+            f1t[d] = (1 - position) * fmp;
+            f2t[d] = Fr1 + position * fmp;
+            f3t[d] = Fr2;
+        }
+        for (int d = 0; d < 3; d++) {
+            f3[d] += f3t[d];
+            if (stretch3) stretch3[3 * i + d] = f3t[d];
+        }
+        if (are_equal1(pos[i])) {
+            double factor = (position - 1) / position;
+            for (int d = 0; d < 3; d++) {
+                f1[d] += f1t[d] + f2t[d] * factor;
+                f2[d] += f2t[d] * (1 / position);
+            }
+        } else {
+            for (int d = 0; d < 3; d++) {
+                f1[d] += f1t[d];
+                f2[d] += f2t[d];
+            }
+        }
+    }
+}
+double orc_bdh_energy(const double *x, int32_t n, const int32_t *bs, const double *kdih, const double *pos) {
+    osum U;
+    os_zero(&U);
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
+        double c2p[3], mp[3], a[3], b[3], n1[3], n2[3];
+        double position = pos[i];
+        if (are_equal1(position)) {
+            position = 0.5;
+            extend_plus(c2p, c1, c2, position);
+            c2 = c2p;
+        }
+        midpt(mp, c1, c2, position);
+        /* vectorProduct(v, v1, v2, v3, v4) = (v2 - v1) x (v4 - v3), MathFunctions.h:695-716 */
+        for (int d = 0; d < 3; d++) { a[d] = c2[d] - mp[d]; b[d] = c3[d] - mp[d]; }
+        cross3(n1, a, b);
+        for (int d = 0; d < 3; d++) { a[d] = c4[d] - c3[d]; b[d] = c3[d] - mp[d]; }
+        cross3(n2, a, b);
+        normalize3(n1);
+        normalize3(n2);
+        double n1n2 = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+        os_add(&U, kdih[i] * (1 - n1n2));
+    }
+    return os_val(&U);
+}
+void orc_bdh_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kdih, const double *pos,
+                    double *stretch3) {
+    for (int32_t i = 0; i < n; i++) {
+        double c1[3], c2[3], c3[3], c4[3], mp[3];
+        for (int d = 0; d < 3; d++) {
+            c1[d] = x[bs[4 * i] + d]; c2[d] = x[bs[4 * i + 1] + d];
+            c3[d] = x[bs[4 * i + 2] + d]; c4[d] = x[bs[4 * i + 3] + d];
+        }
+        double position = pos[i];
+        if (are_equal1(position)) {
+            position = 0.5;
+            for (int d = 0; d < 3; d++) c2[d] = (1 / position) * (c2[d] - (1 - position) * c1[d]);
+        }
+        midpt(mp, c1, c2, position);
+        double *f1 = f + bs[4 * i], *f2 = f + bs[4 * i + 1], *f3 = f + bs[4 * i + 2], *f4 = f + bs[4 * i + 3];
+        double vb1x = c2[0] - mp[0], vb1y = c2[1] - mp[1], vb1z = c2[2] - mp[2];
+        double vb2x = c3[0] - mp[0], vb2y = c3[1] - mp[1], vb2z = c3[2] - mp[2];
+        double vb2xm = -vb2x, vb2ym = -vb2y, vb2zm = -vb2z;
+        double vb3x = c4[0] - c3[0], vb3y = c4[1] - c3[1], vb3z = c4[2] - c3[2];
+        double ax = vb1y * vb2zm - vb1z * vb2ym, ay = vb1z * vb2xm - vb1x * vb2zm, az = vb1x * vb2ym - vb1y * vb2xm;
+        double bx = vb3y * vb2zm - vb3z * vb2ym, by = vb3z * vb2xm - vb3x * vb2zm, bz = vb3x * vb2ym - vb3y * vb2xm;
+        double rasq = ax * ax + ay * ay + az * az;
+        double rbsq = bx * bx + by * by + bz * bz;
+        double rgsq = vb2xm * vb2xm + vb2ym * vb2ym + vb2zm * vb2zm;
+        double rg = sqrt(rgsq);
+        double rginv = 0.0, ra2inv = 0.0, rb2inv = 0.0;
+        if (rg > 0) rginv = 1.0 / rg;
+        if (rasq > 0) ra2inv = 1.0 / rasq;
+        if (rbsq > 0) rb2inv = 1.0 / rbsq;
+        double rabinv = sqrt(ra2inv * rb2inv);
+        double s = rg * rabinv * (ax * vb3x + ay * vb3y + az * vb3z);
+        double fg = vb1x * vb2xm + vb1y * vb2ym + vb1z * vb2zm;
+        double hg = vb3x * vb2xm + vb3y * vb2ym + vb3z * vb2zm;
+        double fga = fg * ra2inv * rginv;
+        double hgb = hg * rb2inv * rginv;
+        double gaa = -ra2inv * rg;
+        double gbb = rb2inv * rg;
+        double dtf[3] = {gaa * ax, gaa * ay, gaa * az};
+        double dtg[3] = {fga * ax - hgb * bx, fga * ay - hgb * by, fga * az - hgb * bz};
+        double dth[3] = {gbb * bx, gbb * by, gbb * bz};
+        double df = -kdih[i] * s;
+        double L1[3], L2[3], L3[3], L4[3];
+        for (int d = 0; d < 3; d++) {
+            double s2 = df * dtg[d];
+            L4[d] = df * dth[d];
+            L3[d] = -s2 - L4[d];
+            if (are_equal1(pos[i])) {
+                double l2p = df * dtf[d];
+                L2[d] = s2 - l2p;
+                L1[d] = l2p * ((position - 1) / position);
+                L2[d] += l2p * (1 / position);
+            } else {
+                L2[d] = df * dtf[d];
+                double lmp = s2 - L2[d];
+                L1[d] = (1 - position) * lmp;
+                L2[d] += (position) * lmp;
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            f1[d] += L1[d]; f2[d] += L2[d]; f3[d] += L3[d]; f4[d] += L4[d];
+            if (stretch3) stretch3[3 * i + d] = L3[d];
         }
     }
 }
@@ -888,6 +1080,8 @@ static void ffm_each(const orc_system *o, const double *x, double *e) {
     e[MDY_FF_MS] = orc_ls_energy(x, t->ms_n, t->ms_beadset, t->ms_k, t->ms_l0, t->ms_pos1, t->ms_pos2);
     e[MDY_FF_BS] = orc_bs_energy(x, t->bs_n, t->bs_beadset, t->bs_k, t->bs_l0, t->bs_pos);
     e[MDY_FF_BB] = orc_bb_energy(x, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0);
+    e[MDY_FF_BDH] = orc_bdh_energy(x, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos);
+    e[MDY_FF_BP] = orc_bp_energy(x, t->bp_n, t->bp_beadset, t->bp_k, t->bp_pos);
     e[MDY_FF_EV] = o->sys.exvol_enabled ? orc_ev_energy(x, (int32_t)o->ev_n, o->ev_bs, o->ev_krep, o->ev_eql) : 0.0;
     e[MDY_FF_BD] = (o->sys.boundary_enabled && o->sys.bnd.n_planes > 0)
                        ? orc_bd_energy(x, o->sys.bnd.n_planes, o->sys.bnd.plane, o->bd_nneigh, o->bd_bs, o->bd_krep,
@@ -935,6 +1129,8 @@ int orc_ffm_forces(const orc_system *o, const double *x, double *f, int32_t nvar
     orc_ls_forces(x, f, t->ms_n, t->ms_beadset, t->ms_k, t->ms_l0, t->ms_pos1, t->ms_pos2, NULL);
     orc_bs_forces(x, f, t->bs_n, t->bs_beadset, t->bs_k, t->bs_l0, t->bs_pos, NULL);
     orc_bb_forces(x, f, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0, NULL);
+    orc_bdh_forces(x, f, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos, NULL);
+    orc_bp_forces(x, f, t->bp_n, t->bp_beadset, t->bp_k, t->bp_pos, NULL);
     if (o->sys.exvol_enabled) {
         int rc = orc_ev_forces(x, f, (int32_t)o->ev_n, o->ev_bs, o->ev_krep, o->ev_eql, o->coplanar_mode);
         if (rc) return rc;

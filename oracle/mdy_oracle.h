@@ -55,6 +55,12 @@ void   orc_ls_forces(const double *x, double *f, int32_t n, const int32_t *bs, c
 double orc_bs_energy(const double *x, int32_t n, const int32_t *bs, const double *k, const double *l0, const double *p);
 void   orc_bs_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *k, const double *l0,
                      const double *p, double *stretch3);
+double orc_bp_energy(const double *x, int32_t n, const int32_t *bs, const double *kpos, const double *pos);
+void   orc_bp_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kpos, const double *pos,
+                     double *stretch3);
+double orc_bdh_energy(const double *x, int32_t n, const int32_t *bs, const double *kdih, const double *pos);
+void   orc_bdh_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kdih, const double *pos,
+                      double *stretch3);
 /* boundary load forces on filament ends (ForceFieldManager::computeLoadForces) */
 void   orc_load_forces(const mdy_cylinders *cyl, int32_t n_beads, const double *x_nl, const double *x,
                        const mdy_boundary *bnd, int32_t n_types, const double *mon_size, const int32_t *n_mon,

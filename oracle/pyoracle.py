@@ -58,7 +58,7 @@ class Oracle:
         L.orc_ev_pair_energy_analytic.restype = C.c_double
         L.orc_ev_pair_energy_simpson.restype = C.c_double
         for fn in ("orc_fs_energy", "orc_fb_energy", "orc_ls_energy", "orc_bs_energy", "orc_bb_energy",
-                   "orc_bd_energy"):
+                   "orc_bd_energy", "orc_bdh_energy", "orc_bp_energy"):
             getattr(L, fn).restype = C.c_double
         L.orc_nl_pairs.restype = C.c_int64
         L.orc_ev_vectorize.restype = C.c_int64
@@ -125,8 +125,8 @@ class Oracle:
     # bonded / cross-link force fields: name in fs, fb, ls, bs, bb; params in
     # the C order (fs: k, l0; fb: k, theta0; ls: k, l0, pos1, pos2;
     # bs: k, l0, pos; bb: k, theta0); bead offsets already x3
-    _NPAR = {"fs": 2, "fb": 2, "ls": 4, "bs": 3, "bb": 2}
-    _FSTRETCH = {"ls": 1, "bs": 3, "bb": 3}
+    _NPAR = {"fs": 2, "fb": 2, "ls": 4, "bs": 3, "bb": 2, "bdh": 2, "bp": 2}
+    _FSTRETCH = {"ls": 1, "bs": 3, "bb": 3, "bdh": 3, "bp": 3}
 
     def _ff_args(self, name, x, bs, params):
         if len(params) != self._NPAR[name]:
@@ -278,12 +278,25 @@ class OracleSystem:
                 self.o.lib.orc_ls_forces(_p(x), _p(f), len(k), _p(arr[0], C.c_int32), _p(arr[1]), _p(arr[2]),
                                          _p(arr[3]), _p(arr[4]), _p(st))
             out.append(st)
-        br = np.zeros(3 * len(n.bs_k))
+        # branch force: stretching, dihedral and position contributions added
+        # in force-field order (assignforcemags of each, BranchingStretching.cpp:86-97,
+        # BranchingDihedral.cpp:89-98, BranchingPosition.cpp)
+        nbr = max(len(n.bs_k), len(n.bdh_k), len(n.bp_k))
+        br = np.zeros(3 * nbr)
         if len(n.bs_k):
+            part = np.zeros(3 * len(n.bs_k))
             arr = [np.ascontiguousarray(a, t) for a, t in ((n.bs_beadset, np.int32), (n.bs_k, np.float64),
                                                            (n.bs_l0, np.float64), (n.bs_pos, np.float64))]
             self.o.lib.orc_bs_forces(_p(x), _p(f), len(n.bs_k), _p(arr[0], C.c_int32), _p(arr[1]), _p(arr[2]),
-                                     _p(arr[3]), _p(br))
+                                     _p(arr[3]), _p(part))
+            br[: part.size] += part
+        for fn, bs, k, pos in ((self.o.lib.orc_bdh_forces, n.bdh_beadset, n.bdh_k, n.bdh_pos),
+                               (self.o.lib.orc_bp_forces, n.bp_beadset, n.bp_k, n.bp_pos)):
+            if len(k):
+                part = np.zeros(3 * len(k))
+                arr = [np.ascontiguousarray(a, t) for a, t in ((bs, np.int32), (k, np.float64), (pos, np.float64))]
+                fn(_p(x), _p(f), len(k), _p(arr[0], C.c_int32), _p(arr[1]), _p(arr[2]), _p(part))
+                br[: part.size] += part
         out.append(br)
         return tuple(out)
 

@@ -191,4 +191,4 @@ def test_implicit_stiffness_derivative_matches_reminimization():
         fd = (vals[0] - vals[1]) / 2e-4
         print(f"[gpu] force field {k}: implicit {d[k]:.8g}  differences {fd:.8g}")
         assert abs(d[k] - fd) <= 1e-5 * np.abs(d).max()
-    assert d[4] == 0 and d[5] == 0 and d[6] == 0 and d[7] == 0   # no branches, pairs or boundary terms
+    assert all(d[k] == 0 for k in range(4, NFF))   # no branch points, pairs or boundary terms

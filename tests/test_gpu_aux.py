@@ -16,6 +16,8 @@ def nets(lib):
     return {
         "net500": network.generate(lib, "net500"),
         "branched": network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=2),
+        "branched_full": network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=4,
+                                          k_br_dihedral=10.0, k_br_position=20.0),
         "net40k": network.generate(lib, "net40k"),
     }
 
@@ -30,7 +32,7 @@ def bits_equal(a, b):
                                                  np.ascontiguousarray(b).view(np.int64))
 
 
-@pytest.mark.parametrize("case", ["net500", "branched", "net40k"])
+@pytest.mark.parametrize("case", ["net500", "branched", "branched_full", "net40k"])
 def test_aux_outputs_after_minimization(oracle, all_nets, case):
     net = all_nets[case]
     print(f"[gpu] aux outputs, {case}", flush=True)

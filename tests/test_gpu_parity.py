@@ -41,6 +41,9 @@ def small_networks(lib):
         "branched": network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=2),
         "dense": network.generate(lib, "net500", n_filaments=60, cyl_per_filament=3, box=500.0, n_linkers=40,
                                   n_motors=4, seed=3),
+        # all four branching force fields (dihedral k 10, position k 20 as in the shipped examples)
+        "branched_full": network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=4,
+                                          k_br_dihedral=10.0, k_br_position=20.0),
     }
 
 
@@ -51,7 +54,7 @@ def nets(lib):
     return d
 
 
-CASES = ["net500", "branched", "dense", "net40k"]
+CASES = ["net500", "branched", "dense", "branched_full", "net40k"]
 
 
 def ctx_for(net):
@@ -150,7 +153,7 @@ def test_forces_accumulate(nets):
     np.testing.assert_allclose(f - base, ctx.forces(net.coord), rtol=0, atol=1e-9)
 
 
-@pytest.mark.parametrize("case", ["net500", "branched", "dense"])
+@pytest.mark.parametrize("case", ["net500", "branched", "dense", "branched_full"])
 def test_minimization_trajectory_bitwise(exact, nets, case):
     net = nets[case]
     ctx = ctx_for(net)
@@ -233,7 +236,7 @@ def test_disabled_force_fields(exact, lib):
     s = exact.system(net)
     e, per = ctx.energy(net.coord)
     eo, pero = s.energy()
-    assert bits_equal([e], [eo]) and per[6] == 0.0 and per[7] == 0.0
+    assert bits_equal([e], [eo]) and per[_abi.MDY_FF_EV] == 0.0 and per[_abi.MDY_FF_BD] == 0.0
     assert bits_equal(ctx.forces(net.coord), s.forces())
 
 
