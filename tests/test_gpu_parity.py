@@ -173,6 +173,26 @@ def test_minimization_trajectory_bitwise(exact, nets, case):
     assert bits_equal(res.energies_before[NFF:NFF + 1], reso.energies_before[NFF:NFF + 1])
 
 
+@pytest.mark.parametrize("method", [_abi.MDY_CG_STEEPEST, _abi.MDY_CG_FLETCHER_RIEVES])
+def test_other_directions_trajectory_bitwise(exact, nets, method):
+    """CGMethod::minimize<steepestDescent> and <FletcherRieves>
+    (CGMethod.hpp:1746-1759): bit-identical trajectories to the oracle."""
+    net = nets["branched_full"]
+    ctx = ctx_for(net)
+    s = exact.system(net)
+    p = _abi.default_cg_params(net.n_beads, method=method, force_call_limit=3000)
+    x, f, res = ctx.minimize(p, trace_cap=100000)
+    tr = ctx.trace()
+    xo, fo, reso, tro = s.minimize(p, trace_cap=100000)
+    assert res.status == 0 and reso.status == 0
+    assert (res.num_iter, res.num_energy_call, res.num_force_call, res.error_flags) == (
+        reso.num_iter, reso.num_energy_call, reso.num_force_call, reso.error_flags)
+    for key in ("lambda_", "energy", "maxf", "beta"):
+        assert bits_equal(tr[key], tro[key]), key
+    assert bits_equal(x, xo) and bits_equal(f, fo)
+    print(f"[gpu] method {method}: {res.num_iter} iterations, max|F| {res.grad_inf_norm:.3f}")
+
+
 def test_net40k_first_iterations_bitwise_and_end_state(exact, nets):
     net = nets["net40k"]
     ctx = ctx_for(net)
