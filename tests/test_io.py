@@ -1,0 +1,44 @@
+"""MEDYAN's filament input format (medyan_amd.io), pinned by the reference's
+2filaments example: its FILAMENT input file and the first filament's bead
+coordinates in the first model of its sample output (fixtures written by
+tests/golden/make_filament_fixture.py)."""
+import os
+
+import numpy as np
+import pytest
+
+from medyan_amd import io
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_reader_and_straight_projection_match_the_example():
+    fils = io.read_filament_file(os.path.join(GOLDEN, "filament_input_2filaments.txt"))
+    assert [t for t, _ in fils] == [0, 0]
+    beads = io.filament_beads(fils[0][1])
+    ref = np.loadtxt(os.path.join(GOLDEN, "filament_2filaments_model1_fil0.txt"))[:, 1:]
+    np.testing.assert_allclose(beads, ref, atol=5e-4)          # the PDB keeps 3 decimals of nm / 10
+    assert len(io.filament_beads(fils[1][1])) == 10            # round(1000 / 108) = 9 cylinders
+
+
+def test_reader_rules():
+    text = ("# comment FILAMENT 0 0 0 0 1 1 1\n"
+            "FILAMENT 1 0 0 0 0 0 300 # trailing comment skips the line\n"
+            "FILAMENT 2 0 0 0 0 0 10\n"                          # short filament: one cylinder
+            "FILAMENT 0 0 0 0 0 0 108 0 0 216 5 0 324\n"         # four points: three cylinders
+            "LINKER 0 0 0 0 1 1 1\n")
+    fils = io.read_filament_file(text)
+    assert [t for t, _ in fils] == [2, 0]
+    assert len(io.filament_beads(fils[0][1])) == 2
+    b = io.filament_beads(fils[1][1], projection="PREDEFINED")
+    np.testing.assert_array_equal(b, fils[1][1])
+    s = io.filament_beads(fils[1][1])                            # STRAIGHT uses the first two points
+    np.testing.assert_allclose(s[:, 2], [0, 108, 216, 324])
+    with pytest.raises(ValueError):
+        io.read_filament_file("FILAMENT 0 0 0 0 1 1 1 2\n")
+
+
+def test_network_from_file_minimizable_shape(lib):
+    net = io.network_from_filament_file(os.path.join(GOLDEN, "filament_input_2filaments.txt"), 2000.0)
+    assert net.n_cyl == 18 and net.n_beads == 20
+    assert len(net.fs_k) == 18 and len(net.fb_k) == 16
