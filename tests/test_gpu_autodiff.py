@@ -75,14 +75,17 @@ def dev(a):
     return torch.tensor(np.ascontiguousarray(a, np.float64), device="cuda")
 
 
-@pytest.mark.parametrize("case", ["ladder", "net500", "branched"])
+@pytest.mark.parametrize("case", ["ladder", "net500", "branched", "branched_full"])
 def test_hvp_matches_force_differences(oracle, lib, case):
     if case == "ladder":
         net = ladder()
     elif case == "net500":
         net = network.generate(lib, "net500")
-    else:
+    elif case == "branched":
         net = network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=2)
+    else:   # with the branching dihedral and position terms
+        net = network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=4,
+                               k_br_dihedral=10.0, k_br_position=20.0)
     print(f"[gpu] hvp vs differences, {case}", flush=True)
     ctx = exact_ctx(net)
     s = oracle.system(net, coplanar_mode=_abi.MDY_COPLANAR_EXACT)
