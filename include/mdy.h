@@ -40,7 +40,8 @@ enum {
     MDY_ERR_FORCE = -8,      /* excluded-volume force stayed non-finite after the
                                 quadrature fallback (reference exits) */
     MDY_ERR_RECOVERY = -9,   /* recovery move had infinite step (reference throws) */
-    MDY_ERR_TIMEOUT = -10    /* device-side grid barrier watchdog fired */
+    MDY_ERR_TIMEOUT = -10,   /* device-side grid barrier watchdog fired */
+    MDY_ERR_QUADRATIC = -11  /* quadratic line search: positive ideal energy change (the reference exits) */
 };
 
 /* Force-field slots, in the order MController::initializeFF registers them
@@ -133,7 +134,12 @@ typedef struct mdy_cg_params {
     int32_t force_call_limit;     /* effective limit is max(this, 3 * n_beads_limit)   */
     int32_t n_beads_limit;        /* Bead::numBeads() used by the call limit           */
     int32_t try_to_recover;       /* tryToRecoverInLineSearchError                     */
+    int32_t line_search;          /* MDY_LS_*: lineSearchAlgorithm (CGMethod.hpp:51)   */
 } mdy_cg_params;
+/* BACKTRACKING (backtrackingLineSearch, the default) or QUADRATIC
+ * (quadraticLineSearchV2 + quadraticoptimization outside safe mode,
+ * CGMethod.hpp:522-765, 1319-1331). */
+enum { MDY_LS_BACKTRACKING = 0, MDY_LS_QUADRATIC = 1 };
 
 /* MinimizationResult (src/Mechanics/Minimizer/MinimizationTypes.hpp:10-64). */
 #define MDY_ERROR_LINESEARCH 1u

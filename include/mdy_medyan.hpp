@@ -39,6 +39,9 @@ struct ConjugateGradientParams {
     int energyCallLimit = 0;
     int forceCallLimit = 10000;
     bool tryToRecoverInLineSearchError = true;
+    // "BACKTRACKING" or "QUADRATIC"; CGMethod::minimize (CGMethod.hpp:1319-1331)
+    // takes any other name as QUADRATIC
+    std::string lineSearchAlgorithm = "BACKTRACKING";
 };
 
 struct EnergyReport {
@@ -180,6 +183,7 @@ public:
         cp.force_call_limit = p.forceCallLimit;
         cp.n_beads_limit = numDof / 3;
         cp.try_to_recover = p.tryToRecoverInLineSearchError ? 1 : 0;
+        cp.line_search = p.lineSearchAlgorithm == "BACKTRACKING" ? MDY_LS_BACKTRACKING : MDY_LS_QUADRATIC;
         if (outForce) outForce->assign(numDof, 0.0);
         mdy_min_result r{};
         ffm.check(mdy_minimize(ffm.ctx(), &cp, coord.data(), outForce ? outForce->data() : nullptr, &r));

@@ -12,7 +12,7 @@ MDY_OK = 0
 ERRORS = {
     -1: "MDY_ERR_ARG", -2: "MDY_ERR_HIP", -3: "MDY_ERR_STATE", -4: "MDY_ERR_NONFINITE",
     -5: "MDY_ERR_CAPACITY", -6: "MDY_ERR_NOGPU", -7: "MDY_ERR_LAUNCH", -8: "MDY_ERR_FORCE",
-    -9: "MDY_ERR_RECOVERY", -10: "MDY_ERR_TIMEOUT",
+    -9: "MDY_ERR_RECOVERY", -10: "MDY_ERR_TIMEOUT", -11: "MDY_ERR_QUADRATIC",
 }
 MDY_ERR_ARG = -1
 MDY_ERR_HIP = -2
@@ -24,6 +24,8 @@ MDY_ERR_LAUNCH = -7
 MDY_ERR_FORCE = -8
 MDY_ERR_RECOVERY = -9
 MDY_ERR_TIMEOUT = -10
+MDY_ERR_QUADRATIC = -11
+MDY_LS_BACKTRACKING, MDY_LS_QUADRATIC = 0, 1
 
 FF_NAMES = ["FilamentStretching", "FilamentBending", "LinkerStretching", "MotorGhostStretching",
             "BranchingStretching", "BranchingBending", "BranchingDihedral", "BranchingPosition",
@@ -82,7 +84,7 @@ class CGParams(C.Structure):
     _fields_ = [("method", C.c_int32), ("max_dist", C.c_double), ("lambda_max", C.c_double),
                 ("grad_tol", C.c_double), ("energy_rel_tol", C.c_double),
                 ("energy_call_limit", C.c_int32), ("force_call_limit", C.c_int32),
-                ("n_beads_limit", C.c_int32), ("try_to_recover", C.c_int32)]
+                ("n_beads_limit", C.c_int32), ("try_to_recover", C.c_int32), ("line_search", C.c_int32)]
 
 
 class MinResult(C.Structure):
@@ -129,7 +131,8 @@ class Network(C.Structure):
 
 def default_cg_params(n_beads, grad_tol=1.0, max_dist=1.0, lambda_max=0.01,
                       force_call_limit=100000, method=MDY_CG_POLAK_RIBIERE,
-                      energy_rel_tol=1e-9, try_to_recover=1, energy_call_limit=0):
+                      energy_rel_tol=1e-9, try_to_recover=1, energy_call_limit=0,
+                      line_search=MDY_LS_BACKTRACKING):
     """CG parameters of the benchmark workloads: GRADIENTTOLERANCE 1, MAXDISTANCE 1,
     LAMBDAMAX 0.01 (the shipped example values), energy-change tolerance 1e-9."""
     p = CGParams()
@@ -142,4 +145,5 @@ def default_cg_params(n_beads, grad_tol=1.0, max_dist=1.0, lambda_max=0.01,
     p.force_call_limit = force_call_limit
     p.n_beads_limit = n_beads
     p.try_to_recover = try_to_recover
+    p.line_search = line_search
     return p

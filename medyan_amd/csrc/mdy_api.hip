@@ -466,6 +466,8 @@ int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *p, double *d_coord, doub
     Ctx &c = ctx->c;
     if (!c.have_sys || !c.have_nl) return fail(ctx, MDY_ERR_STATE, "network / neighbor list missing");
     if (p->method < MDY_CG_STEEPEST || p->method > MDY_CG_POLAK_RIBIERE) return fail(ctx, MDY_ERR_ARG, "method");
+    if (p->line_search != MDY_LS_BACKTRACKING && p->line_search != MDY_LS_QUADRATIC)
+        return fail(ctx, MDY_ERR_ARG, "line_search");
     (void)hipSetDevice(c.device);
     int rc = run_minimize(c, p, d_coord, d_force_out, result);
     if (rc == MDY_ERR_LAUNCH) return fail(ctx, rc, "cooperative launch failed");
@@ -503,6 +505,8 @@ int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params,
             return fail(ctxs[0], MDY_ERR_ARG, "batched contexts must share device and precision");
         if (params[i].method < MDY_CG_STEEPEST || params[i].method > MDY_CG_POLAK_RIBIERE)
             return fail(ctxs[0], MDY_ERR_ARG, "method");
+        if (params[i].line_search != MDY_LS_BACKTRACKING && params[i].line_search != MDY_LS_QUADRATIC)
+            return fail(ctxs[0], MDY_ERR_ARG, "line_search");
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return fail(ctxs[0], MDY_ERR_ARG, "a context appears twice in the batch");
         cs[i] = &c;

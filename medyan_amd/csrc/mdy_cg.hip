@@ -1,6 +1,7 @@
-// mdy_cg.hip -- CGMethod::minimize<method> with backtracking line search,
-// resident on the device for the whole minimization
-// (src/Mechanics/Minimizer/CGMethod.hpp:238-373, 826-1743; CGMethod.cpp:43-360).
+// mdy_cg.hip -- CGMethod::minimize<method> with the backtracking or the
+// quadratic line search, resident on the device for the whole minimization
+// (src/Mechanics/Minimizer/CGMethod.hpp:238-373, 522-765, 826-1743;
+// CGMethod.cpp:43-360).
 //
 // One cooperative launch; every block runs the same control flow and derives
 // every decision from grid reductions that all blocks evaluate in the same
@@ -44,7 +45,7 @@ struct CgArgs {
     int *tr_safe;
     long long tr_cap;
     // parameters
-    int method, try_recover, energy_call_limit, force_call_limit;
+    int method, line_search, try_recover, energy_call_limit, force_call_limit;
     double max_dist, lambda_max, grad_tol, energy_rel_tol;
     long long spin_limit;
     int grid;             // blocks of this minimization's cluster (batched launch)
@@ -463,12 +464,14 @@ __device__ __attribute__((always_inline)) void spec_direction(Cg<MIXED> &K, cons
 // (resetCGParams) that trial is discarded and redone along F.
 enum Op { OP_NONE, OP_EVAL_FF, OP_EVAL, OP_GATHER, OP_MINSTEP, OP_SDONLY, OP_RESTORE };
 enum Stage {
-    ST_INIT_FF, ST_INIT_F, ST_INIT_G, ST_ITER, ST_TRIAL, ST_SD, ST_LS_DONE, ST_ACCEPT, ST_REC_STEP, ST_REC_EVAL,
-    ST_REC_GATHER, ST_FINAL, ST_FINAL_DONE, ST_FINISH
+    ST_INIT_FF, ST_INIT_F, ST_INIT_G, ST_ITER, ST_TRIAL, ST_SD, ST_QUAD, ST_QUAD_EVAL, ST_QUAD_DONE, ST_QUAD_AT,
+    ST_LS_DONE, ST_ACCEPT, ST_REC_STEP, ST_REC_EVAL, ST_REC_GATHER, ST_FINAL, ST_FINAL_DONE, ST_FINISH
 };
 
 // One minimization run by a cluster of G blocks (blk = index within it).
-template <bool MIXED>
+// QUAD: compiled with the quadratic line-search stages (a separate kernel, so
+// the backtracking kernel carries none of their state in registers).
+template <bool MIXED, bool QUAD>
 __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *lds, int blk, int G) {
     __shared__ unsigned long long accl[24];
     Cg<MIXED> K(A, lds, accl, blk, G);
@@ -490,6 +493,14 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
     double minimumE = INFINITY;
     bool haveMin = false;
     double lam = 0, newE = 0, armijo = 0, reducedSlope = 0, sdF = 0, stepRec = 0, beta = 0;
+    // quadratic line search (quadraticLineSearchV2): trial count, the last
+    // rejected trial, the first record, the highest later record, the
+    // parabola's bracket, its refinement count, and the last evaluated step
+    bool quadLS = false;
+    int qIter = 0, qRef = 0;
+    double qPrevL = 0, qPrevE = 0, qL0 = 0, qE0 = 0, qMax = 0, qBt = 0;
+    double qx0 = 0, qx1 = 0, qx2 = 0, qy0 = 0, qy1 = 0, qy2 = 0, qLam = 0;
+    double lamEval = 0, Eacc = 0;    // step of the last trial and the energy at the accepted point
     bool pendingSpec = false;        // an accepted step's s is not formed yet
     const double *dprev = nullptr;   // the direction s extends
     double *Snew = nullptr;          // where s goes
@@ -590,6 +601,9 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             }
             nIter++;
             armijo = safeMode ? 0.0 : 0.4;
+            quadLS = QUAD && A.line_search == MDY_LS_QUADRATIC && !safeMode;
+            qIter = 0;
+            qMax = -INFINITY;
             lam = 0.0; newE = 0.0;
             if (maxF != 0.0) {
                 lam = smin(A.lambda_max, A.max_dist / maxF);
@@ -625,6 +639,33 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             }
             double Et = ev.E;
             nEC++;
+            if (quadLS) {   // CGMethod.hpp:567-617: Armijo slope 0.4, every trial recorded
+                lamEval = lam; Eacc = Et;
+                qIter++;
+                const double idealq = -0.4 * lam * slope;
+                if (idealq > 0) { K.status = MDY_ERR_QUADRATIC; after = false; stage = ST_FINISH; break; }
+                bool conv = false;
+                if (Et - E0 <= idealq) conv = true;
+                else { qPrevL = lam; qPrevE = Et; lam *= 0.5; }
+                double recL, recE;
+                if (lam <= 0.0 || lam <= 1e-8) { conv = true; lam = 0.0; recL = 0.0; recE = E0; }
+                else if (conv) { recL = lam; recE = Et; }
+                else { recL = qPrevL; recE = qPrevE; }
+                if (qIter == 1) { qL0 = recL; qE0 = recE; }
+                else qMax = fmax(qMax, recE);
+                if (!conv) { op = OP_EVAL; P = PosAny::line(A.x, dvec, lam); break; }
+                newE = Et;
+                // CGMethod.hpp:621-645: refine when an earlier record lies above
+                // the accepted energy; the third point is the first record
+                if (lam > 0.0 && qIter > 1 && Et < E0 && Et < qMax) {
+                    qx0 = 0.0; qx1 = lam; qx2 = qL0; qy0 = E0; qy1 = Et; qy2 = qE0;
+                    qBt = lam; qRef = 0;
+                    stage = ST_QUAD;
+                } else {
+                    stage = ST_LS_DONE;
+                }
+                break;
+            }
             const double ideal = -reducedSlope * lam;
             bool done = false;
             if (lam < 0 || ideal > 0) { done = true; lam = 0.0; }
@@ -637,6 +678,52 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             stage = ST_LS_DONE;
             break;
         }
+        case ST_QUAD: {   // quadraticoptimization, CGMethod.hpp:663-765 (at most 200 refinements)
+            if constexpr (!QUAD) break;
+            if (qRef >= 200) { qLam = qx1; stage = ST_QUAD_DONE; break; }
+            qRef++;
+            const double d = (qx0 - qx1) * (qx0 - qx2) * (qx1 - qx2);
+            const double a = qy0 * (qx1 - qx2) - qy1 * (qx0 - qx2) + qy2 * (qx0 - qx1);
+            const double b = -(qy0 * (qx1 + qx2) * (qx1 - qx2) - qy1 * (qx0 + qx2) * (qx0 - qx2) +
+                               qy2 * (qx0 + qx1) * (qx0 - qx1));
+            qLam = -b / (2 * a);
+            if (fabs(qLam - qx1) / qx1 < 1e-3) { stage = ST_QUAD_DONE; break; }
+            if (a / d < 0) { qLam = qBt; stage = ST_QUAD_DONE; break; }
+            op = OP_EVAL; P = PosAny::line(A.x, dvec, qLam); stage = ST_QUAD_EVAL;
+            break;
+        }
+        case ST_QUAD_EVAL: {
+            if constexpr (!QUAD) break;
+            const double eq = ev.E;
+            nEC++;
+            lamEval = qLam; Eacc = eq;
+            if (fabs(eq - qy1) / qy1 < 1e-3) { stage = ST_QUAD_DONE; break; }
+            if (qLam < qx1) {
+                if (eq < qy0 && eq < qy1) { qx1 = qLam; qy1 = eq; }
+                else { qx0 = qLam; qy0 = eq; }
+            } else if (qx1 < qLam) {
+                if (eq < qy1 && eq < qy2) { qx1 = qLam; qy1 = eq; }
+                else { qx2 = qLam; qy2 = eq; }
+            } else {
+                qLam = qx1; stage = ST_QUAD_DONE; break;
+            }
+            stage = ST_QUAD;
+            break;
+        }
+        case ST_QUAD_DONE:   // newE stays the backtracking energy (CGMethod.hpp:655)
+            if constexpr (!QUAD) break;
+            lam = (qLam <= 0.0 || qLam <= 1e-8) ? 0.0 : qLam;
+            if (lam != 0.0 && lam != lamEval) {   // forces and energy at the accepted point
+                op = OP_EVAL; P = PosAny::line(A.x, dvec, lam); stage = ST_QUAD_AT;
+                break;
+            }
+            stage = ST_LS_DONE;
+            break;
+        case ST_QUAD_AT:
+            if constexpr (!QUAD) break;
+            Eacc = ev.E;
+            stage = ST_LS_DONE;
+            break;
         case ST_LS_DONE:
             if (lam == 0.0) {
                 if (!A.try_recover) {
@@ -682,7 +769,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             nFC++;
             gradInf = maxF;
             if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
-            E0 = newE;
+            E0 = quadLS ? Eacc : newE;   // E(x) of the next line search
             relE = fabs(newE - prevEnergy) / fabs(prevEnergy);
             prevEnergy = newE;
             const double newGrad = FF, prevGrad = FFp;
@@ -732,10 +819,10 @@ aborted:
 
 // The arguments come by value: pointers read from the kernel-argument segment
 // are known to be global, so every access is a global (not flat) access.
-template <bool MIXED>
+template <bool MIXED, bool QUAD>
 __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
     __shared__ double lds[17 * 16];
-    cg_run<MIXED>(Av, lds, blockIdx.x, gridDim.x);
+    cg_run<MIXED, QUAD>(Av, lds, blockIdx.x, gridDim.x);
 }
 
 // Batched form: independent minimizations in one cooperative launch. Block b
@@ -746,7 +833,7 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
 // global, so the arguments are read through a constant-space view and every
 // access through them uses global instructions, as in the single-network
 // kernel.
-template <bool MIXED>
+template <bool MIXED, bool QUAD>
 __global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ all, const int2 *__restrict__ map) {
     __shared__ double lds[17 * 16];
     const int2 m = map[blockIdx.x];
@@ -756,7 +843,19 @@ __global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ 
 #else
     CgArgs A = all[m.x];
 #endif
-    cg_run<MIXED>(A, lds, m.y, A.grid);
+    cg_run<MIXED, QUAD>(A, lds, m.y, A.grid);
+}
+
+static const void *cg_kernel(bool mixed, bool quad) {
+    if (mixed) return quad ? reinterpret_cast<const void *>(k_cg<true, true>)
+                           : reinterpret_cast<const void *>(k_cg<true, false>);
+    return quad ? reinterpret_cast<const void *>(k_cg<false, true>) : reinterpret_cast<const void *>(k_cg<false, false>);
+}
+static const void *cg_batch_kernel(bool mixed, bool quad) {
+    if (mixed) return quad ? reinterpret_cast<const void *>(k_cg_batch<true, true>)
+                           : reinterpret_cast<const void *>(k_cg_batch<true, false>);
+    return quad ? reinterpret_cast<const void *>(k_cg_batch<false, true>)
+                : reinterpret_cast<const void *>(k_cg_batch<false, false>);
 }
 
 // ---------------------------------------------------------------------------
@@ -792,7 +891,7 @@ static int prepare_cg(Ctx &c, const mdy_cg_params *p, double *d_x, int G, hipStr
     A.tr_lam = c.tr_lam.p; A.tr_e = c.tr_e.p; A.tr_mf = c.tr_mf.p; A.tr_beta = c.tr_beta.p; A.tr_safe = c.tr_safe.p;
     A.tr_cap = c.trace_cap;
     A.prof = c.prof.p; A.prof_kind = c.prof_kind.p; A.prof_cap = c.prof_cap;
-    A.method = p->method; A.try_recover = p->try_to_recover; A.energy_call_limit = p->energy_call_limit;
+    A.method = p->method; A.line_search = p->line_search; A.try_recover = p->try_to_recover; A.energy_call_limit = p->energy_call_limit;
     A.force_call_limit = p->force_call_limit > 3 * p->n_beads_limit ? p->force_call_limit : 3 * p->n_beads_limit;
     A.max_dist = p->max_dist; A.lambda_max = p->lambda_max; A.grad_tol = p->grad_tol;
     A.energy_rel_tol = p->energy_rel_tol;
@@ -836,7 +935,7 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     // persistent grid: all blocks resident
     int per_cu = 0;
     const bool mixed = c.opts.precision == MDY_PREC_MIXED;
-    const void *kern = mixed ? reinterpret_cast<const void *>(k_cg<true>) : reinterpret_cast<const void *>(k_cg<false>);
+    const void *kern = cg_kernel(mixed, p->line_search == MDY_LS_QUADRATIC);
     MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
     if (per_cu < 1) return MDY_ERR_LAUNCH;
     if (c.opts.blocks_per_cu > 0 && c.opts.blocks_per_cu < per_cu) per_cu = c.opts.blocks_per_cu;
@@ -889,8 +988,9 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
                        int first_pass_calls, hipStream_t st, double *total_ms) {
     Ctx &c0 = *cs[0];
     const bool mixed = c0.opts.precision == MDY_PREC_MIXED;
-    const void *kern = mixed ? reinterpret_cast<const void *>(k_cg_batch<true>)
-                             : reinterpret_cast<const void *>(k_cg_batch<false>);
+    bool quad = false;
+    for (int i = 0; i < n; i++) quad = quad || p[i].line_search == MDY_LS_QUADRATIC;
+    const void *kern = cg_batch_kernel(mixed, quad);
     const int tpb = threads_per_block;
     int per_cu = 0;
     MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, tpb, 0));

@@ -114,6 +114,9 @@ class ConjugateGradientParams:
     energyCallLimit: int = 0
     forceCallLimit: int = 10000
     tryToRecoverInLineSearchError: bool = True
+    # "BACKTRACKING" or "QUADRATIC"; CGMethod::minimize (CGMethod.hpp:1319-1331)
+    # takes any other name as QUADRATIC
+    lineSearchAlgorithm: str = "BACKTRACKING"
 
     def to_c(self, n_beads) -> _abi.CGParams:
         p = _abi.CGParams()
@@ -126,6 +129,8 @@ class ConjugateGradientParams:
         p.force_call_limit = int(self.forceCallLimit)
         p.n_beads_limit = int(n_beads)
         p.try_to_recover = 1 if self.tryToRecoverInLineSearchError else 0
+        p.line_search = (_abi.MDY_LS_BACKTRACKING if self.lineSearchAlgorithm == "BACKTRACKING"
+                         else _abi.MDY_LS_QUADRATIC)
         return p
 
 

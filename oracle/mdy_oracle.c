@@ -1210,6 +1210,95 @@ static double line_search(cg_state *s, const double *x, orc_energy_fn efn, void 
     return lambda;
 }
 
+/* quadraticoptimization, CGMethod.hpp:663-765: a parabola through three
+ * (lambda, E) points, refined with energy evaluations until lambda or the
+ * energy changes by less than LAMBDAQUADTOL = 1e-3. The reference loops
+ * without a bound; this stops after 200 refinements with the middle point. */
+static double quad_opt(cg_state *s, const double *x, orc_energy_fn efn, void *user, mdy_min_result *r, double x0,
+                       double x1, double x2, double y0, double y1, double y2) {
+    const double btlambda = x1;
+    for (int iter = 0; iter < 200; iter++) {
+        double d = (x0 - x1) * (x0 - x2) * (x1 - x2);
+        double a = y0 * (x1 - x2) - y1 * (x0 - x2) + y2 * (x0 - x1);
+        double b = -(y0 * (x1 + x2) * (x1 - x2) - y1 * (x0 + x2) * (x0 - x2) + y2 * (x0 + x1) * (x0 - x1));
+        double lq = -b / (2 * a);
+        if (fabs(lq - x1) / x1 < 1e-3) return lq;
+        if (a / d < 0) return btlambda;
+        for (int32_t i = 0; i < s->ndof; i++) s->coordLS[i] = x[i] + lq * s->searchDir[i];
+        double eq = efn(user, s->coordLS, NULL, 0);
+        r->num_energy_call++;
+        if (fabs(eq - y1) / y1 < 1e-3) return lq;
+        if (lq < x1) {
+            if (eq < y0 && eq < y1) { x1 = lq; y1 = eq; }
+            else { x0 = lq; y0 = eq; }
+        } else if (x1 < lq) {
+            if (eq < y1 && eq < y2) { x1 = lq; y1 = eq; }
+            else { x2 = lq; y2 = eq; }
+        } else {
+            return x1;
+        }
+    }
+    return x1;
+}
+
+/* quadraticLineSearchV2, CGMethod.hpp:522-658: backtracking with the Armijo
+ * slope 0.4 that records (lambda, E) of every step, then, if the accepted
+ * step was not the first and improved the energy while an earlier step's
+ * energy was higher, a quadratic refinement through (0, E0), the accepted
+ * point and the first recorded point (the search loop leaves its index at 0).
+ * newEnergy is the backtracking step's energy whatever lambda is returned.
+ * *fatal: the ideal energy change was positive (the reference exits). */
+static double line_search_quad(cg_state *s, const double *x, orc_energy_fn efn, void *user, double MAXDIST,
+                               double maxForce, double LAMBDAMAX, mdy_min_result *r, double *newEnergy, int *fatal) {
+    *newEnergy = 0;
+    *fatal = 0;
+    if (maxForce == 0.0) return 0.0;
+    double lambda = smin(LAMBDAMAX, MAXDIST / maxForce);
+    double currentEnergy = efn(user, x, NULL, 0);
+    r->num_energy_call++;
+    double energyLambda = 0.0, prevEnergy = 0.0, prevLambda = 0.0;
+    /* lambdavec / energyvec of the reference */
+    int nrec = 0, caprec = 0, quadstatus = 0;
+    double *lrec = NULL, *erec = NULL;
+    int iter = 0, conv = 0;
+    while (!conv) {
+        iter++;
+        for (int32_t i = 0; i < s->ndof; i++) s->coordLS[i] = x[i] + lambda * s->searchDir[i];
+        energyLambda = efn(user, s->coordLS, NULL, 0);
+        r->num_energy_call++;
+        double ideal = -0.4 * lambda * cg_SdotF(s);
+        if (ideal > 0) { *fatal = 1; free(lrec); free(erec); return 0.0; }
+        double change = energyLambda - currentEnergy;
+        if (change <= ideal) conv = 1;
+        else { prevLambda = lambda; prevEnergy = energyLambda; lambda *= 0.5; }
+        double pl, pe;
+        if (lambda <= 0.0 || lambda <= 1e-8) { conv = 1; lambda = 0.0; pl = 0.0; pe = currentEnergy; }
+        else if (conv) { pl = lambda; pe = energyLambda; }
+        else { pl = prevLambda; pe = prevEnergy; }
+        if (nrec == caprec) {
+            caprec = caprec ? 2 * caprec : 16;
+            lrec = (double *)realloc(lrec, sizeof(double) * (size_t)caprec);
+            erec = (double *)realloc(erec, sizeof(double) * (size_t)caprec);
+        }
+        lrec[nrec] = pl; erec[nrec] = pe;
+        nrec++;
+    }
+    if (lambda > 0.0 && iter > 1 && energyLambda < currentEnergy) {
+        int i;
+        for (i = nrec - 1; i >= 1; i--)
+            if (energyLambda < erec[i]) quadstatus = 1;
+        /* i == 0 here: the third point is the first record */
+        if (quadstatus) {
+            double lq = quad_opt(s, x, efn, user, r, 0.0, lambda, lrec[i], currentEnergy, energyLambda, erec[i]);
+            lambda = (lq <= 0.0 || lq <= 1e-8) ? 0.0 : lq;
+        }
+    }
+    free(lrec); free(erec);
+    s->totalEnergy = lambda > 0 ? energyLambda : currentEnergy;
+    *newEnergy = energyLambda;
+    return lambda;
+}
+
 static void reset_cg(cg_state *s, int method, double *lastFDotF) {
     cg_shift(s, 0);
     if (method == MDY_CG_FLETCHER_RIEVES || method == MDY_CG_POLAK_RIBIERE) *lastFDotF = cg_FdotF(s);
@@ -1262,9 +1351,16 @@ int orc_minimize(const mdy_cg_params *p, double *x, int32_t nvar, int32_t ndof, 
         if (iter_cap > 0 && numIter >= iter_cap) break;
         double beta = 0, newGrad = 0, prevGrad = 0;
         numIter++;
-        double newE;
-        double lambda = line_search(&s, x, efn, user, p->max_dist, r->grad_inf_norm, p->lambda_max,
-                                    safeMode ? 0 : 0.4, r, &newE);
+        double newE, lambda;
+        if (p->line_search == MDY_LS_QUADRATIC && !safeMode) {
+            int fatal = 0;
+            lambda = line_search_quad(&s, x, efn, user, p->max_dist, r->grad_inf_norm, p->lambda_max, r, &newE,
+                                      &fatal);
+            if (fatal) { status = MDY_ERR_QUADRATIC; goto done; }
+        } else {
+            lambda = line_search(&s, x, efn, user, p->max_dist, r->grad_inf_norm, p->lambda_max,
+                                 safeMode ? 0 : 0.4, r, &newE);
+        }
         if (lambda == 0) {
             if (p->try_to_recover) {
                 if (safeMode) {

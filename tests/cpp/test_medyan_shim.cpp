@@ -52,6 +52,7 @@ static int run() {
     REQUIRE(f1 == f2);
 
     // minimize to the benchmark tolerance
+    const std::vector<double> start = coord;
     ConjugateGradientParams p;
     p.gradTol = 1.0;
     p.maxDist = 1.0;
@@ -83,6 +84,14 @@ static int run() {
         threw = err.code() == MDY_ERR_ARG;
     }
     REQUIRE(threw);
+
+    // the quadratic line search from the same start
+    std::vector<double> cq = start;
+    p.lineSearchAlgorithm = "QUADRATIC";
+    MinimizationResult rq = CGMethod{}.minimize(p, cq, ndof, ffm);
+    REQUIRE(rq.success());
+    REQUIRE(rq.converged());
+    REQUIRE(rq.energiesAfter.total < rq.energiesBefore.total);
 
     mdy_network_free(net);
     std::printf("ALL OK: %d CG iterations, max|F| %.4f pN\n", r.numIter, r.gradInfNorm);

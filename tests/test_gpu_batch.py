@@ -41,24 +41,25 @@ def same(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
 
 
-@pytest.mark.parametrize("precision,tpb,passes", [(_abi.MDY_PREC_FP64, 64, 0), (_abi.MDY_PREC_FP64, 256, 0),
-                                                  (_abi.MDY_PREC_MIXED, 128, 0), (_abi.MDY_PREC_FP64, 64, 40),
-                                                  (_abi.MDY_PREC_MIXED, 64, 40)])
-def test_batch_equals_single_minimizations(lib, precision, tpb, passes):
+@pytest.mark.parametrize("precision,tpb,passes,ls", [(_abi.MDY_PREC_FP64, 64, 0, 0), (_abi.MDY_PREC_FP64, 256, 0, 0),
+                                                     (_abi.MDY_PREC_MIXED, 128, 0, 0), (_abi.MDY_PREC_FP64, 64, 40, 0),
+                                                     (_abi.MDY_PREC_MIXED, 64, 40, 0),
+                                                     (_abi.MDY_PREC_FP64, 128, 0, _abi.MDY_LS_QUADRATIC)])
+def test_batch_equals_single_minimizations(lib, precision, tpb, passes, ls):
     """passes > 0: a first pass capped at 40 force calls (six networks get
-    64 blocks or more each, so the pass runs uncapped)."""
+    64 blocks or more each, so the pass runs uncapped). ls: line search."""
     nets = nets_small(lib)
-    print(f"[gpu] batch of {len(nets)}, precision {precision}, {tpb} threads per block, first pass {passes}",
-          flush=True)
+    print(f"[gpu] batch of {len(nets)}, precision {precision}, {tpb} threads per block, first pass {passes}, "
+          f"line search {ls}", flush=True)
     singles = []
     for net in nets:
         c = ctx_for(net, precision)
-        x, f, r = c.minimize(_abi.default_cg_params(net.n_beads))
+        x, f, r = c.minimize(_abi.default_cg_params(net.n_beads, line_search=ls))
         singles.append((x, f, r))
     ctxs = [ctx_for(net, precision) for net in nets]
     xs = [torch.tensor(net.coord, device="cuda") for net in nets]
     fs = [torch.zeros_like(x) for x in xs]
-    params = [_abi.default_cg_params(net.n_beads) for net in nets]
+    params = [_abi.default_cg_params(net.n_beads, line_search=ls) for net in nets]
     res, ms, tot = mechanics.minimize_batch(ctxs, params, [x.data_ptr() for x in xs], [f.data_ptr() for f in fs],
                                             threads_per_block=tpb, first_pass_calls=passes)
     torch.cuda.synchronize()

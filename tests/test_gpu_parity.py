@@ -193,6 +193,32 @@ def test_other_directions_trajectory_bitwise(exact, nets, method):
     print(f"[gpu] method {method}: {res.num_iter} iterations, max|F| {res.grad_inf_norm:.3f}")
 
 
+@pytest.mark.parametrize("case", ["net500", "branched_full"])
+def test_quadratic_line_search_trajectory_bitwise(exact, nets, case):
+    """lineSearchAlgorithm QUADRATIC (quadraticLineSearchV2 and
+    quadraticoptimization, CGMethod.hpp:522-765): bit-identical trajectories
+    to the oracle, with parabola refinements among the accepted steps."""
+    from test_oracle_cg import refined_steps
+    net = nets[case]
+    ctx = ctx_for(net)
+    s = exact.system(net)
+    p = _abi.default_cg_params(net.n_beads, line_search=_abi.MDY_LS_QUADRATIC, force_call_limit=3000)
+    x, f, res = ctx.minimize(p, trace_cap=100000)
+    tr = ctx.trace()
+    xo, fo, reso, tro = s.minimize(p, trace_cap=100000)
+    assert res.status == 0 and reso.status == 0
+    assert (res.num_iter, res.num_energy_call, res.num_force_call, res.error_flags, res.num_safe_mode) == (
+        reso.num_iter, reso.num_energy_call, reso.num_force_call, reso.error_flags, reso.num_safe_mode)
+    for key in ("lambda_", "energy", "maxf", "beta", "safe"):
+        assert bits_equal(tr[key], tro[key]), key
+    assert bits_equal(x, xo) and bits_equal(f, fo)
+    assert bits_equal(res.energies_after[NFF:NFF + 1], reso.energies_after[NFF:NFF + 1])
+    nref = refined_steps(tro, p)
+    print(f"[gpu] quadratic {case}: {res.num_iter} iterations, {nref} refined steps, "
+          f"max|F| {res.grad_inf_norm:.3f}")
+    assert nref > 0
+
+
 def test_net40k_first_iterations_bitwise_and_end_state(exact, nets):
     net = nets["net40k"]
     ctx = ctx_for(net)
@@ -276,6 +302,14 @@ def test_single_filament_hairpin(exact):
     xo, fo, reso, _ = s.minimize(p)
     assert res.status == reso.status == 0
     assert bits_equal(x, xo) and res.num_iter == reso.num_iter
+
+
+def test_line_search_argument_checked(nets):
+    net = nets["net500"]
+    ctx = ctx_for(net)
+    with pytest.raises(mechanics.MdyError) as e:
+        ctx.minimize(_abi.default_cg_params(net.n_beads, line_search=7))
+    assert e.value.code == _abi.MDY_ERR_ARG
 
 
 def test_error_paths(lib, nets):

@@ -2,9 +2,21 @@
 
 The reference's own minimizer test (src/TESTS/Mechanics/Minimizer/
 TestCGMethod.cpp:20-92: a 10-D weighted quadratic, three initial points, three
-direction rules) fixes convergence; the Armijo condition fixes monotone
-energies over accepted steps; the exit ladders (call limit, line-search error
-flag, safe mode, recovery, non-finite force) are forced from the host.
+direction rules) fixes convergence, under both line searches; the Armijo
+condition fixes monotone energies over accepted steps; the exit ladders (call
+limit, line-search error flag, safe mode, recovery, non-finite force) are
+forced from the host.
+
+The quadratic line search (quadraticLineSearchV2 + quadraticoptimization,
+CGMethod.hpp:522-765) is pinned where mathematics fixes it: along an exactly
+quadratic energy the Armijo test with slope 0.4 accepts exactly the steps
+lambda <= 1.2 lambda*, and a parabola through any three points of the line is
+the energy itself, so a search that starts at 3 lambda* must reject twice,
+accept 0.75 lambda*, refine, and return the line minimizer lambda* =
+s.F / s.W.s to rounding. Which earlier record serves as the third point (the
+reference's loop leaves its index at 0) is not visible on an exact parabola;
+that choice follows CGMethod.hpp:621-636 as read. The positive-ideal-change
+exit cannot be reached: every search direction has s.F > 0 (resetCGParams).
 """
 import ctypes as C
 
@@ -42,8 +54,9 @@ INITS = {
 
 @pytest.mark.parametrize("method", [_abi.MDY_CG_STEEPEST, _abi.MDY_CG_FLETCHER_RIEVES, _abi.MDY_CG_POLAK_RIBIERE])
 @pytest.mark.parametrize("init", list(INITS))
-def test_reference_quadratic(oracle, method, init):
-    p = quad_params(method, energy_rel_tol=0.0)
+@pytest.mark.parametrize("ls", [_abi.MDY_LS_BACKTRACKING, _abi.MDY_LS_QUADRATIC])
+def test_reference_quadratic(oracle, method, init, ls):
+    p = quad_params(method, energy_rel_tol=0.0, line_search=ls)
     x, f, res, _ = oracle.minimize(p, INITS[init], 10, quad_energy, quad_force)
     assert res.status == 0 and res.error_flags == 0
     assert np.all(np.abs(f) <= p.grad_tol)
@@ -130,3 +143,97 @@ def test_recovery_move(oracle):
     np.testing.assert_array_equal(x, [1.5, 1.0, 3.25])
     z = np.zeros(3)
     assert oracle.lib.orc_recover_move(None, pyoracle._p(x), pyoracle._p(z), 3, C.c_double(1.0)) == _abi.MDY_ERR_RECOVERY
+
+
+@pytest.mark.parametrize("start,refine", [(2.5, False), (3.0, True), (4.0, True), (4.5, True)])
+def test_quadratic_search_lands_on_line_minimum(oracle, start, refine):
+    """First iteration along s = F of E = sum w (x - opt)^2 / 2 from lambda0 =
+    start * lambda*, 2.4 < start <= 4.8: two rejections (lambda0/2 > 1.2
+    lambda*) and acceptance at lambda0/4. The refinement runs when the record
+    at lambda0/2 lies above the accepted energy, |lambda0/4 - lambda*| <
+    |lambda0/2 - lambda*|, i.e. start > 8/3, and returns lambda*; otherwise
+    the step stays lambda0/4 as in backtracking."""
+    x0 = INITS["input_2"]
+    F = -W * (x0 - OPT)
+    lstar = float(F @ F) / float(F @ (W * F))
+    runs = {}
+    for ls in (_abi.MDY_LS_BACKTRACKING, _abi.MDY_LS_QUADRATIC):
+        p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0, max_dist=1e12,
+                        lambda_max=start * lstar, line_search=ls)
+        x, f, res, tr = oracle.minimize(p, x0, 10, quad_energy, quad_force, iter_cap=1, trace_cap=4)
+        assert res.status == 0 and res.num_iter == 1
+        runs[ls] = (x, res, tr)
+    x, res, tr = runs[_abi.MDY_LS_BACKTRACKING]
+    assert tr["lambda_"][0] == start * lstar / 4
+    assert res.num_energy_call == 1 + 1 + 3 + 1       # before, E(x), three trials, after
+    x, res, tr = runs[_abi.MDY_LS_QUADRATIC]
+    if not refine:
+        assert tr["lambda_"][0] == start * lstar / 4 and res.num_energy_call == 6
+        return
+    assert abs(tr["lambda_"][0] - lstar) <= 1e-9 * lstar
+    np.testing.assert_allclose(x, x0 + tr["lambda_"][0] * F, rtol=0, atol=1e-12 * np.abs(x0).max())
+    # refinement evaluations: none when lambda0/4 is lambda* already (the first
+    # parabola's vertex is its middle point), else one or two
+    assert res.num_energy_call - 6 in ((0,) if start == 4.0 else (1, 2))
+    # newEnergy stays the backtracking step's energy (CGMethod.hpp:655)
+    assert tr["energy"][0] == quad_energy(x0 + (start * lstar / 4) * F)
+
+
+def test_quadratic_search_single_trial_is_backtracking(oracle):
+    """A first trial that passes the Armijo test leaves nothing to refine."""
+    x0 = INITS["input_2"]
+    F = -W * (x0 - OPT)
+    lstar = float(F @ F) / float(F @ (W * F))
+    out = []
+    for ls in (_abi.MDY_LS_BACKTRACKING, _abi.MDY_LS_QUADRATIC):
+        p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0, max_dist=1e12, lambda_max=0.9 * lstar,
+                        line_search=ls)
+        out.append(oracle.minimize(p, x0, 10, quad_energy, quad_force, iter_cap=1, trace_cap=4))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][3]["lambda_"][0] == out[1][3]["lambda_"][0] == 0.9 * lstar
+    assert out[0][2].num_energy_call == out[1][2].num_energy_call
+
+
+def test_quadratic_search_safe_mode_uses_backtracking(oracle):
+    """CGMethod.hpp:1322-1325: in safe mode the quadratic setting runs the
+    zero-slope backtracking search; the failure ladder is the same."""
+    out = []
+    for ls in (_abi.MDY_LS_BACKTRACKING, _abi.MDY_LS_QUADRATIC):
+        p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0, try_to_recover=1, force_call_limit=4,
+                        n_beads_limit=1, line_search=ls)
+        out.append(oracle.minimize(p, INITS["input_2"], 10, quad_energy, uphill_force,
+                                   recover=lambda x, d, m: x.__iadd__(np.min(m / np.abs(d)) * d), trace_cap=20))
+    (x0, _, r0, t0), (x1, _, r1, t1) = out
+    assert r0.num_safe_mode >= 1 and r0.num_recovery >= 1
+    np.testing.assert_array_equal(x0, x1)
+    assert (r0.num_iter, r0.num_energy_call, r0.num_force_call, r0.num_safe_mode, r0.num_recovery) == (
+        r1.num_iter, r1.num_energy_call, r1.num_force_call, r1.num_safe_mode, r1.num_recovery)
+    np.testing.assert_array_equal(t0["lambda_"], t1["lambda_"])
+    np.testing.assert_array_equal(t0["safe"], t1["safe"])
+
+
+def refined_steps(tr, p):
+    """Accepted steps that are not lambda0 / 2^k, lambda0 = min(LAMBDAMAX,
+    MAXDIST / max|F|) of the iteration: the parabola refinement ran."""
+    n = 0
+    for j in range(1, len(tr["lambda_"])):
+        lam = tr["lambda_"][j]
+        if lam <= 0 or tr["safe"][j]:
+            continue
+        l0 = min(p.lambda_max, p.max_dist / tr["maxf"][j - 1])
+        k = np.log2(l0 / lam)
+        if abs(k - round(k)) > 1e-9:
+            n += 1
+    return n
+
+
+def test_quadratic_search_on_network(oracle, lib):
+    from medyan_amd import network
+    net = network.generate(lib, "net500", n_filaments=12, n_linkers=20, n_motors=4, seed=11)
+    s = oracle.system(net)
+    p = _abi.default_cg_params(net.n_beads, line_search=_abi.MDY_LS_QUADRATIC)
+    x, f, res, tr = s.minimize(p, trace_cap=100000)
+    assert res.status == 0 and res.error_flags == 0
+    assert res.grad_inf_norm < p.grad_tol or res.energy_rel_change < p.energy_rel_tol
+    assert refined_steps(tr, p) > 0
+    np.testing.assert_array_equal(f, s.forces(x))
