@@ -74,6 +74,12 @@ typedef struct mdy_cylinders {
     const double  *exvol_k;      /* MCylinder::getExVolConst                        */
     const int32_t *branch_child; /* index of getBranchingCylinder() or -1           */
     const uint8_t *minus_end;    /* isMinusEnd()                                    */
+    /* monomer-sum excluded volume only (CylinderVolumeMon::vectorize,
+     * Volume/CylinderVolumeMon.hpp:149-163); may be NULL otherwise */
+    const int32_t *mon_minus;    /* getFirstBead()->monomerSerial                   */
+    const int32_t *mon_plus;     /* getSecondBead()->monomerSerial                  */
+    const double  *mon_size;     /* GeoParams::monomerSize[type]                    */
+    const int32_t *mon_interval; /* volumeExclusionMonomerInterval[type], >= 1      */
 } mdy_cylinders;
 
 /* Vectorized bonded/cross-link tables (each FF's vectorize()). */
@@ -120,7 +126,14 @@ typedef struct mdy_system {
     float exvol_cutoff;        /* VolumeCutoff, rMax of the hybrid NL (float in the reference) */
     int32_t exvol_enabled;
     int32_t boundary_enabled;
+    int32_t exvol_type;        /* MDY_EXVOL_*: cylinderVolumeExclusionFFType */
 } mdy_system;
+/* Cylinder excluded volume form (MController::initializeFF,
+ * Controller/MController.cpp:128-153): the analytic integral
+ * (CylinderExclVolume<CylinderExclVolRepulsion>) or the monomer-pair sum
+ * (CylinderVolumeMon, Volume/CylinderVolumeMon.hpp:31-100). Both occupy the
+ * MDY_FF_EV slot. */
+enum { MDY_EXVOL_INTEGRAL = 0, MDY_EXVOL_MONOMER = 1 };
 
 /* ConjugateGradientParams (src/Mechanics/Minimizer/CGMethod.hpp:44-61). */
 enum { MDY_CG_STEEPEST = 0, MDY_CG_FLETCHER_RIEVES = 1, MDY_CG_POLAK_RIBIERE = 2 };
@@ -206,7 +219,10 @@ int mdy_get_pairs(mdy_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n);
 /* Excluded-volume interaction table built from those pairs
  * (CylinderExclVolume::vectorize, CylinderExclVolume.cpp:34-179: pairs whose
  * neighbor is the cylinder's mother dropped): 4 bead offsets per pair and
- * kR = k * l1 * l2. beadset / krep may be NULL to query the count. */
+ * kR = k * l1 * l2. In the monomer-sum form (CylinderVolumeMon::vectorize,
+ * CylinderVolumeMon.hpp:126-185: mother-daughter pairs dropped in either
+ * order) kR = max(k1, k2) * seg1 * seg2, seg = interval * monomer size.
+ * beadset / krep may be NULL to query the count. */
 int mdy_get_exvol_table(mdy_ctx *ctx, int32_t *beadset, double *krep, int64_t cap, int64_t *n);
 /* Boundary interaction table: per-plane counts and bead offsets (reference order). */
 int mdy_get_boundary_table(mdy_ctx *ctx, int32_t *nneighbors, int32_t *beadset, int64_t cap, int64_t *n);

@@ -26,6 +26,7 @@ MDY_ERR_RECOVERY = -9
 MDY_ERR_TIMEOUT = -10
 MDY_ERR_QUADRATIC = -11
 MDY_LS_BACKTRACKING, MDY_LS_QUADRATIC = 0, 1
+MDY_EXVOL_INTEGRAL, MDY_EXVOL_MONOMER = 0, 1
 
 FF_NAMES = ["FilamentStretching", "FilamentBending", "LinkerStretching", "MotorGhostStretching",
             "BranchingStretching", "BranchingBending", "BranchingDihedral", "BranchingPosition",
@@ -50,7 +51,8 @@ class Cylinders(C.Structure):
     _fields_ = [("n", C.c_int32), ("bead1", _i32p), ("bead2", _i32p), ("id", _i64p),
                 ("filament", _i32p), ("position", _i32p), ("type", _i16p),
                 ("eq_length", _f64p), ("exvol_k", _f64p), ("branch_child", _i32p),
-                ("minus_end", _u8p)]
+                ("minus_end", _u8p), ("mon_minus", _i32p), ("mon_plus", _i32p), ("mon_size", _f64p),
+                ("mon_interval", _i32p)]
 
 
 class FFTables(C.Structure):
@@ -77,7 +79,7 @@ class Boundary(C.Structure):
 class System(C.Structure):
     _fields_ = [("n_beads", C.c_int32), ("cyl", Cylinders), ("ff", FFTables), ("bnd", Boundary),
                 ("exvol_cutoff", C.c_float), ("exvol_enabled", C.c_int32),
-                ("boundary_enabled", C.c_int32)]
+                ("boundary_enabled", C.c_int32), ("exvol_type", C.c_int32)]
 
 
 class CGParams(C.Structure):

@@ -20,6 +20,7 @@ Dev Ctx::dev() const {
     D.n_bdh = n_bdh; D.n_bp = n_bp;
     D.n_bd = have_nl ? n_bd : 0;
     D.ev_bs = ev_bs.p; D.ev_kr = ev_kr.p;
+    D.ev_mon = ev_mon; D.ev_cyl = ev_cyl.p; D.cyl_mon = c_mon.p;
     D.fs_bs = fs_bs.p; D.fs_k = fs_k.p; D.fs_l0 = fs_l0.p;
     D.fb_bs = fb_bs.p; D.fb_k = fb_k.p; D.fb_th = fb_th.p; D.fb_ct = fb_ct.p; D.fb_st = fb_st.p;
     D.lm_bs = lm_bs.p; D.lm_k = lm_k.p; D.lm_l0 = lm_l0.p; D.lm_p1 = lm_p1.p; D.lm_p2 = lm_p2.p;
@@ -135,6 +136,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.bdh_bs); release(c.bdh_k); release(c.bdh_pos); release(c.bp_bs); release(c.bp_k); release(c.bp_pos);
     release(c.c_b1); release(c.c_b2); release(c.c_fil); release(c.c_pos); release(c.c_child); release(c.c_id);
     release(c.c_type); release(c.c_eq); release(c.c_ek); release(c.c_minus); release(c.c_plus);
+    release(c.c_mon); release(c.c_seg); release(c.ev_cyl);
     release(c.planes); release(c.plane_k); release(c.plane_sl);
     release(c.mid); release(c.cell_of); release(c.cell_cnt); release(c.cell_off); release(c.cell_cur);
     release(c.cell_list); release(c.row_cnt); release(c.row_keep); release(c.row_off); release(c.row_keep_off);
@@ -183,6 +185,16 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
         return fail(ctx, MDY_ERR_ARG, "incomplete cylinder table");
     if (s->bnd.n_planes > 0 && (!s->bnd.plane || !s->bnd.k || !s->bnd.screen))
         return fail(ctx, MDY_ERR_ARG, "incomplete boundary table");
+    if (s->exvol_type != MDY_EXVOL_INTEGRAL && s->exvol_type != MDY_EXVOL_MONOMER)
+        return fail(ctx, MDY_ERR_ARG, "exvol_type");
+    const bool mon = s->exvol_enabled && s->exvol_type == MDY_EXVOL_MONOMER && cy.n > 0;
+    if (mon) {
+        if (!cy.mon_minus || !cy.mon_plus || !cy.mon_size || !cy.mon_interval)
+            return fail(ctx, MDY_ERR_ARG, "monomer-sum excluded volume needs the monomer tables");
+        for (int i = 0; i < cy.n; i++)
+            if (cy.mon_interval[i] < 1 || cy.mon_plus[i] < cy.mon_minus[i] || !std::isfinite(cy.mon_size[i]))
+                return fail(ctx, MDY_ERR_ARG, "monomer table: interval >= 1 and plus >= minus serial required");
+    }
     hipStream_t st = c.stream;
     c.have_sys = false;
     c.have_nl = false;
@@ -291,6 +303,18 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
         std::vector<unsigned char> plus(cy.n, 0);
         for (const auto &kv : last) plus[kv.second] = 1;
         rc |= upload(c.c_plus, plus.data(), cy.n, st);
+        c.ev_mon = mon ? 1 : 0;
+        if (mon) {
+            // segEqLen = monInterval * monEqLen (CylinderVolumeMon.hpp:41-43)
+            std::vector<int4> m(cy.n);
+            std::vector<double> seg(cy.n);
+            for (int i = 0; i < cy.n; i++) {
+                m[i] = make_int4(cy.mon_minus[i], cy.mon_plus[i], cy.mon_interval[i], 0);
+                seg[i] = (double)cy.mon_interval[i] * cy.mon_size[i];
+            }
+            rc |= upload(c.c_mon, m.data(), cy.n, st);
+            rc |= upload(c.c_seg, seg.data(), cy.n, st);
+        }
         if (hipStreamSynchronize(st) != hipSuccess) return hipfail(ctx, "upload");
     }
     c.ev_cut = s->exvol_cutoff;

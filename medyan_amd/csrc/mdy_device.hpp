@@ -589,4 +589,58 @@ __device__ __forceinline__ double ev_r2(double a, double b, double c, double d, 
     return c + 2 * s * e + s * s * a - 2 * t * F - 2 * s * t * d + t * t * b;
 }
 
+// ---------------------------------------------------------------------------
+// Monomer-sum excluded volume (CylinderVolumeMon.hpp:31-100): the sum over
+// sampled monomer pairs of coeff / d^4. Cylinder i samples the monomers m in
+// [minus, plus) that are multiples of its interval, at the fraction
+// (m - minus + 0.5) / (plus - minus) of the way from its first bead;
+// coeff = kvol * seg1 * seg2 comes from the neighbor list. With FORCES, each
+// monomer pair's gradient 4 coeff r / d^6 (r from the first point to the
+// second) is shared between the two beads of each cylinder by the sample's
+// fraction and subtracted, in loop order, from forces that start at zero.
+// m = (minus serial, plus serial, interval, unused).
+// ---------------------------------------------------------------------------
+// smallest multiple of w (>= 1) that is >= a
+__device__ __forceinline__ int first_multiple(int a, int w) {
+    const int q = a >= 0 ? (a + w - 1) / w : -((-a) / w);
+    return q * w;
+}
+template <class T>
+__device__ __forceinline__ V3<T> along(V3<T> c1, V3<T> c2, double loc) {
+    return V3<T>{c1.x + loc * (c2.x - c1.x), c1.y + loc * (c2.y - c1.y), c1.z + loc * (c2.z - c1.z)};
+}
+template <class T, bool FORCES>
+__device__ __forceinline__ T evm_eval(V3<T> c11, V3<T> c12, V3<T> c21, V3<T> c22, int4 m1, int4 m2, double coeff,
+                                      V3<T> f[4]) {
+    const double n1 = (double)(m1.y - m1.x), n2 = (double)(m2.y - m2.x);
+    const int s1 = first_multiple(m1.x, m1.z), s2 = first_multiple(m2.x, m2.z);
+    const double coeff4 = coeff * 4;
+    T en(0.0);
+    if (FORCES) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) f[k] = V3<T>{T(0.0), T(0.0), T(0.0)};
+    }
+    for (int a = s1; a < m1.y; a += m1.z) {
+        const double loc1 = ((double)(a - m1.x) + 0.5) / n1;
+        const V3<T> p1 = along(c11, c12, loc1);
+        for (int b = s2; b < m2.y; b += m2.z) {
+            const double loc2 = ((double)(b - m2.x) + 0.5) / n2;
+            const V3<T> p2 = along(c21, c22, loc2);
+            const T rx = p2.x - p1.x, ry = p2.y - p1.y, rz = p2.z - p1.z;
+            const T d2 = rx * rx + ry * ry + rz * rz;
+            en = en + coeff / (d2 * d2);
+            if (FORCES) {
+                const T g = coeff4 / (d2 * d2 * d2);
+                const T gx = g * rx, gy = g * ry, gz = g * rz;
+                const double w11 = 1 - loc1, w21 = -(1 - loc2), w22 = -loc2;
+                f[0].x = f[0].x - w11 * gx; f[0].y = f[0].y - w11 * gy; f[0].z = f[0].z - w11 * gz;
+                f[1].x = f[1].x - loc1 * gx; f[1].y = f[1].y - loc1 * gy; f[1].z = f[1].z - loc1 * gz;
+                f[2].x = f[2].x - w21 * gx; f[2].y = f[2].y - w21 * gy; f[2].z = f[2].z - w21 * gz;
+                f[3].x = f[3].x - w22 * gx; f[3].y = f[3].y - w22 * gy; f[3].z = f[3].z - w22 * gz;
+            }
+        }
+    }
+    return en;
+}
+
 }  // namespace mdy

@@ -84,7 +84,8 @@ template <> struct SlotT<true> { using T = float; };
 // BD]. An item's record (its bead offsets and parameters) and its beads'
 // coordinates are fetched one item ahead of the one being computed, so each
 // thread's chain of dependent memory accesses overlaps with its arithmetic.
-enum { IT_EV = 0, IT_FS, IT_FB, IT_LM, IT_BS, IT_BB, IT_BDH, IT_BP, IT_BD, IT_NONE };
+// IT_EVM: an excluded-volume pair of the monomer-sum form (same table and slots as IT_EV)
+enum { IT_EV = 0, IT_FS, IT_FB, IT_LM, IT_BS, IT_BB, IT_BDH, IT_BP, IT_BD, IT_EVM, IT_NONE };
 struct ItemRec {
     int type, j;      // kind and index within its table
     int4 b;           // bead offsets (x3); unused entries repeat the first
@@ -94,7 +95,7 @@ __device__ __forceinline__ ItemRec fetch_rec(const Dev &D, int i) {
     ItemRec r;
     r.p0 = r.p1 = r.p2 = r.p3 = 0.0;
     if (i < D.n_ev) {
-        r.type = IT_EV; r.j = i;
+        r.type = D.ev_mon ? IT_EVM : IT_EV; r.j = i;
         r.b = D.ev_bs[i];
         r.p0 = D.ev_kr[i];
     } else if ((i -= D.n_ev) < D.n_fs) {
@@ -141,7 +142,7 @@ __device__ __forceinline__ ItemRec fetch_rec(const Dev &D, int i) {
     return r;
 }
 __device__ __forceinline__ int rec_nbeads(int type) {
-    return type == IT_EV || type == IT_LM || type == IT_BB || type == IT_BDH ? 4
+    return type == IT_EV || type == IT_EVM || type == IT_LM || type == IT_BB || type == IT_BDH ? 4
          : type == IT_FB || type == IT_BS || type == IT_BP ? 3
          : type == IT_FS ? 2 : type == IT_BD ? 1 : 0;
 }
@@ -218,6 +219,18 @@ __device__ __forceinline__ void item_compute(const Dev &D, const ItemRec &r, con
                     put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
                 }
             }
+        }
+    } else if (r.type == IT_EVM) {   // monomer-pair sum, in FP64 in both forms; no fallbacks
+        ff = MDY_FF_EV;
+        const int2 cc = D.ev_cyl[j];
+        const int4 m1 = D.cyl_mon[cc.x], m2 = D.cyl_mon[cc.y];
+        d3 f[4];
+        if (FORCES) {
+            U = evm_eval<double, true>(c[0], c[1], c[2], c[3], m1, m2, r.p0, f);
+            const int s = D.slot_ev + 4 * j;
+            put3(contrib, s, f[0]); put3(contrib, s + 1, f[1]); put3(contrib, s + 2, f[2]); put3(contrib, s + 3, f[3]);
+        } else {
+            U = evm_eval<double, false>(c[0], c[1], c[2], c[3], m1, m2, r.p0, f);
         }
     } else if (r.type == IT_FS) {
         ff = MDY_FF_FS;

@@ -22,6 +22,10 @@ __device__ __forceinline__ int item_forces(const Dev &D, const ItemRec &r, const
         ev_invariants(c[0], c[1], c[2], c[3], inv);
         ev_force_inv_t<T>(inv, c[0], c[1], c[2], c[3], r.p0, f);
         return MDY_FF_EV;
+    } else if (r.type == IT_EVM) {
+        const int2 cc = D.ev_cyl[r.j];
+        (void)evm_eval<T, true>(c[0], c[1], c[2], c[3], D.cyl_mon[cc.x], D.cyl_mon[cc.y], r.p0, f);
+        return MDY_FF_EV;
     } else if (r.type == IT_FS) {
         fs_force(c[0], c[1], r.p0, r.p1, f[0], f[1]);
         return MDY_FF_FS;
@@ -52,7 +56,8 @@ __device__ __forceinline__ int item_forces(const Dev &D, const ItemRec &r, const
 
 __device__ __forceinline__ int item_slot(const Dev &D, const ItemRec &r) {
     switch (r.type) {
-        case IT_EV: return D.slot_ev + 4 * r.j;
+        case IT_EV:
+        case IT_EVM: return D.slot_ev + 4 * r.j;
         case IT_FS: return D.slot_fs + 2 * r.j;
         case IT_FB: return D.slot_fb + 3 * r.j;
         case IT_LM: return D.slot_lm + 4 * r.j;

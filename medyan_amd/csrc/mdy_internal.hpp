@@ -24,7 +24,10 @@ constexpr int kRed = 32;             // 8-byte words per block in a reduction re
 struct Dev {
     int nb, ndof;
     int n_ev, n_fs, n_fb, n_lm, n_ls, n_bs, n_bb, n_bdh, n_bp, n_bd;
-    const int4 *ev_bs; const double *ev_kr;
+    const int4 *ev_bs; const double *ev_kr;   // monomer-sum form: ev_kr holds kvol * seg1 * seg2
+    int ev_mon;                               // excluded volume as the monomer-pair sum
+    const int2 *ev_cyl;                       // monomer-sum form: the pair's cylinders
+    const int4 *cyl_mon;                      // per cylinder: minus serial, plus serial, interval
     const int2 *fs_bs; const double *fs_k, *fs_l0;
     const int *fb_bs; const double *fb_k, *fb_th, *fb_ct, *fb_st;   // ct/st: cos/sin(theta0) from the host
     const int4 *lm_bs; const double *lm_k, *lm_l0, *lm_p1, *lm_p2;
@@ -76,6 +79,9 @@ struct Ctx {
     // cylinders
     DBuf<int> c_b1, c_b2, c_fil, c_pos, c_child; DBuf<long long> c_id; DBuf<short> c_type;
     DBuf<double> c_eq, c_ek; DBuf<unsigned char> c_minus, c_plus;   // c_plus: last cylinder of its filament
+    int ev_mon = 0;                // monomer-sum excluded volume
+    DBuf<int4> c_mon;              // minus serial, plus serial, interval, 0
+    DBuf<double> c_seg;            // interval * monomer size
     float ev_cut = 0, bd_cut = 0;
     int ev_on = 0, bd_on = 0;
     int n_planes = 0;
@@ -87,6 +93,7 @@ struct Ctx {
     DBuf<int> pairs;              // 2 * npairs (NL, before mother-daughter filtering)
     long long n_pairs = 0;
     DBuf<int4> ev_bs; DBuf<double> ev_kr; int n_ev = 0;
+    DBuf<int2> ev_cyl;            // monomer-sum form: cylinders of each pair
     DBuf<int> bd_flag, bd_off, bd_bead, bd_plane; DBuf<double> bd_k, bd_sl; int n_bd = 0;
     std::vector<int> h_bd_nneigh;
     DBuf<int> dy_cnt, dy_off, dy_cur, dy_idx;

@@ -81,7 +81,18 @@ struct CylView {
     const long long *id;
     const short *type;
     const double *eq, *ek;
+    const double *seg;   // monomer-sum form: interval * monomer size
+    int mon;             // monomer-sum excluded volume
 };
+
+// Pairs the excluded-volume force field drops: the integral form skips a
+// neighbour that is the cylinder's branching mother
+// (cin->getBranchingCylinder() == ci, CylinderExclVolume.cpp:85, 116); the
+// monomer-sum form skips mother-daughter pairs in either order
+// (CylinderVolumeMon.hpp:134-136).
+__device__ __forceinline__ bool ev_dropped(const CylView &C, int c, int n) {
+    return C.child[n] == c || (C.mon && C.child[c] == n);
+}
 
 // the reference predicate for candidate n of cylinder c
 __device__ __forceinline__ bool nl_pred(const CylView &C, const double *mid, int c, int n, double rminsq,
@@ -128,7 +139,7 @@ __global__ void k_pair_count(int ncyl, CylView C, const double *mid, Grid g, con
         for_candidates(g, mid, cell_off, cell_list, c, [&](int n) {
             if (!nl_pred(C, mid, c, n, rminsq, rmaxsq)) return;
             cnt++;
-            if (C.child[n] != c) keep++;
+            if (!ev_dropped(C, c, n)) keep++;
         });
         row_cnt[c] = cnt;
         row_keep[c] = keep;
@@ -137,7 +148,7 @@ __global__ void k_pair_count(int ncyl, CylView C, const double *mid, Grid g, con
 
 __global__ void k_pair_emit(int ncyl, CylView C, const double *mid, Grid g, const int *cell_off,
                             const int *cell_list, double rminsq, double rmaxsq, const int *row_off,
-                            const int *row_keep_off, int2 *pairs, int4 *ev_bs, double *ev_kr) {
+                            const int *row_keep_off, int2 *pairs, int4 *ev_bs, double *ev_kr, int2 *ev_cyl) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncyl; c += gridDim.x * blockDim.x) {
         int2 *row = pairs + row_off[c];
         int len = 0;
@@ -154,10 +165,17 @@ __global__ void k_pair_emit(int ncyl, CylView C, const double *mid, Grid g, cons
         int o = row_keep_off[c];
         for (int a = 0; a < len; a++) {
             int n = row[a].y;
-            if (C.child[n] == c) continue;
+            if (ev_dropped(C, c, n)) continue;
             ev_bs[o] = make_int4(C.b1[c], C.b2[c], C.b1[n], C.b2[n]);
-            double krep = (C.type[c] != C.type[n]) ? smax(C.ek[c], C.ek[n]) : C.ek[c];
-            ev_kr[o] = krep * C.eq[c] * C.eq[n];
+            if (C.mon) {
+                // kvol = max of the two constants; coeff = kvol * segEqLen1 * segEqLen2
+                // (CylinderVolumeMon.hpp:43, 160-163)
+                ev_kr[o] = smax(C.ek[c], C.ek[n]) * C.seg[c] * C.seg[n];
+                ev_cyl[o] = make_int2(c, n);
+            } else {
+                double krep = (C.type[c] != C.type[n]) ? smax(C.ek[c], C.ek[n]) : C.ek[c];
+                ev_kr[o] = krep * C.eq[c] * C.eq[n];
+            }
             o++;
         }
     }
@@ -258,7 +276,8 @@ int build_neighbors(Ctx &c, const double *d_x) {
     c.n_ev = 0;
     c.n_bd = 0;
     c.n_pairs = 0;
-    CylView C{c.c_b1.p, c.c_b2.p, c.c_fil.p, c.c_pos.p, c.c_child.p, c.c_id.p, c.c_type.p, c.c_eq.p, c.c_ek.p};
+    CylView C{c.c_b1.p, c.c_b2.p, c.c_fil.p, c.c_pos.p, c.c_child.p, c.c_id.p, c.c_type.p, c.c_eq.p, c.c_ek.p,
+              c.c_seg.p, c.ev_mon};
     if (c.ev_on && ncyl > 1) {
         MDY_TRY(ensure(c.mid, (size_t)3 * ncyl));
         const int g = grid1(c, ncyl);
@@ -314,9 +333,10 @@ int build_neighbors(Ctx &c, const double *d_x) {
         MDY_TRY(ensure(c.pairs, (size_t)2 * tot[0] + 2));
         MDY_TRY(ensure(c.ev_bs, (size_t)tot[1] + 1));
         MDY_TRY(ensure(c.ev_kr, (size_t)tot[1] + 1));
+        if (c.ev_mon) MDY_TRY(ensure(c.ev_cyl, (size_t)tot[1] + 1));
         hipLaunchKernelGGL(k_pair_emit, dim3(g), dim3(kBlock), 0, st, ncyl, C, c.mid.p, gr, c.cell_off.p,
                            c.cell_list.p, (double)rminsq_f, (double)rmaxsq_f, c.row_off.p, c.row_keep_off.p,
-                           (int2 *)c.pairs.p, c.ev_bs.p, c.ev_kr.p);
+                           (int2 *)c.pairs.p, c.ev_bs.p, c.ev_kr.p, c.ev_mon ? c.ev_cyl.p : nullptr);
         MDY_HIP(hipGetLastError());
     }
     if (c.bd_on && c.n_planes > 0 && ncyl > 0) {
@@ -348,6 +368,7 @@ int build_neighbors(Ctx &c, const double *d_x) {
     // dynamic CSR over the new slots
     MDY_TRY(ensure(c.ev_bs, 1));
     MDY_TRY(ensure(c.ev_kr, 1));
+    if (c.ev_mon) MDY_TRY(ensure(c.ev_cyl, 1));
     MDY_TRY(ensure(c.bd_bead, 1));
     MDY_TRY(ensure(c.bd_plane, 1));
     MDY_TRY(ensure(c.bd_k, 1));

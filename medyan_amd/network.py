@@ -85,6 +85,13 @@ class Network:
     bp_beadset: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     bp_k: np.ndarray = field(default_factory=lambda: np.zeros(0))
     bp_pos: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    # monomer-sum excluded volume (exvol_type MDY_EXVOL_MONOMER): per-cylinder
+    # monomer serials of its beads, monomer size and sampling interval
+    exvol_type: int = _abi.MDY_EXVOL_INTEGRAL
+    cyl_mon_minus: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    cyl_mon_plus: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    cyl_mon_size: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    cyl_mon_interval: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -126,6 +133,8 @@ class Network:
         c.eq_length, c.exvol_k = f64(self.cyl_eq_length), f64(self.cyl_exvol_k)
         c.branch_child = i32(self.cyl_branch_child)
         c.minus_end = ptr(self.cyl_minus_end, C.c_uint8, np.uint8)
+        c.mon_minus, c.mon_plus = i32(self.cyl_mon_minus), i32(self.cyl_mon_plus)
+        c.mon_size, c.mon_interval = f64(self.cyl_mon_size), i32(self.cyl_mon_interval)
         t = s.ff
         t.fs_n = len(self.fs_k); t.fs_beadset = i32(self.fs_beadset); t.fs_k = f64(self.fs_k); t.fs_l0 = f64(self.fs_l0)
         t.fb_n = len(self.fb_k); t.fb_beadset = i32(self.fb_beadset); t.fb_k = f64(self.fb_k)
@@ -149,8 +158,30 @@ class Network:
         s.exvol_cutoff = float(self.exvol_cutoff)
         s.exvol_enabled = int(self.exvol_enabled)
         s.boundary_enabled = int(self.boundary_enabled)
+        s.exvol_type = int(self.exvol_type)
         self._keep = keep
         return s
+
+    def with_monomer_exvol(self, interval, mon_size=2.7):
+        """A copy whose excluded volume is the monomer-pair sum
+        (CylinderVolumeMon, cylinderVolumeExclusionFFType MONOMER) sampling
+        every `interval`-th monomer. Monomer serials follow a filament built
+        from a bead list: its first bead has serial 0 and each cylinder adds
+        cylinderNumMon = eq_length / monomer size (Filament.cpp:127-133, 197)."""
+        n = self.copy()
+        nc = self.n_cyl
+        num = np.rint(self.cyl_eq_length / mon_size).astype(np.int32)
+        minus = np.zeros(nc, np.int32)
+        for f in np.unique(self.cyl_filament):
+            cs = np.nonzero(self.cyl_filament == f)[0]
+            cs = cs[np.argsort(self.cyl_position[cs], kind="stable")]
+            minus[cs] = np.concatenate([[0], np.cumsum(num[cs])[:-1]])
+        n.exvol_type = _abi.MDY_EXVOL_MONOMER
+        n.cyl_mon_minus = minus
+        n.cyl_mon_plus = minus + num
+        n.cyl_mon_size = np.full(nc, float(mon_size))
+        n.cyl_mon_interval = np.broadcast_to(np.asarray(interval, np.int32), (nc,)).copy()
+        return n
 
 
 def _arr(p, n, dtype):

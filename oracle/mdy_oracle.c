@@ -830,6 +830,105 @@ int orc_ev_forces(const double *x, double *f, int32_t n, const int32_t *bs, cons
 }
 
 /* ------------------------------------------------------------------------- */
+/* monomer-sum excluded volume: CylinderVolumeMon                            */
+/* (Volume/CylinderVolumeMon.hpp:31-100 energy/force, :126-185 vectorize)    */
+/* Per interaction: mon[6] = minus serial, plus serial, interval of the first */
+/* cylinder, then of the second; meq[2] = monomer sizes; kvol.               */
+/* ------------------------------------------------------------------------- */
+/* ceildiv(minus, w) * w: the first sampled monomer serial */
+static int64_t first_sample(int64_t minus, int64_t w) {
+    int64_t q = minus / w;       /* truncates toward zero: a ceiling below zero */
+    if (q * w < minus) q++;
+    return q * w;
+}
+/* energy(info, coords), :31-59 */
+static double evm_pair_energy(const double *c11, const double *c12, const double *c21, const double *c22,
+                              const int32_t *m, const double *meq, double kvol) {
+    const int64_t num1 = (int64_t)m[1] - m[0], num2 = (int64_t)m[4] - m[3];
+    const int64_t s1 = first_sample(m[0], m[2]), s2 = first_sample(m[3], m[5]);
+    const double seg1 = m[2] * meq[0], seg2 = m[5] * meq[1];
+    const double coeff = kvol * seg1 * seg2;
+    double en = 0;
+    for (int64_t a = s1; a < m[1]; a += m[2]) {
+        const double loc1 = (a - m[0] + 0.5) / num1;
+        double p1[3];
+        for (int k = 0; k < 3; k++) p1[k] = c11[k] + loc1 * (c12[k] - c11[k]);
+        for (int64_t b = s2; b < m[4]; b += m[5]) {
+            const double loc2 = (b - m[3] + 0.5) / num2;
+            double p2[3], d2 = 0;
+            for (int k = 0; k < 3; k++) p2[k] = c21[k] + loc2 * (c22[k] - c21[k]);
+            for (int k = 0; k < 3; k++) d2 += (p2[k] - p1[k]) * (p2[k] - p1[k]);
+            en += coeff / (d2 * d2);
+        }
+    }
+    return en;
+}
+/* force(info, coords, forces), :61-100: subtracts into f11..f22 */
+static void evm_pair_force(const double *c11, const double *c12, const double *c21, const double *c22,
+                           const int32_t *m, const double *meq, double kvol, double *f11, double *f12, double *f21,
+                           double *f22) {
+    const int64_t num1 = (int64_t)m[1] - m[0], num2 = (int64_t)m[4] - m[3];
+    const int64_t s1 = first_sample(m[0], m[2]), s2 = first_sample(m[3], m[5]);
+    const double seg1 = m[2] * meq[0], seg2 = m[5] * meq[1];
+    const double coeff = kvol * seg1 * seg2;
+    const double coeff4 = coeff * 4;
+    for (int64_t a = s1; a < m[1]; a += m[2]) {
+        const double loc1 = (a - m[0] + 0.5) / num1;
+        double p1[3];
+        for (int k = 0; k < 3; k++) p1[k] = c11[k] + loc1 * (c12[k] - c11[k]);
+        for (int64_t b = s2; b < m[4]; b += m[5]) {
+            const double loc2 = (b - m[3] + 0.5) / num2;
+            double p2[3], r12[3], d2 = 0;
+            for (int k = 0; k < 3; k++) p2[k] = c21[k] + loc2 * (c22[k] - c21[k]);
+            for (int k = 0; k < 3; k++) r12[k] = p2[k] - p1[k];
+            for (int k = 0; k < 3; k++) d2 += r12[k] * r12[k];
+            const double g = coeff4 / (d2 * d2 * d2);
+            for (int k = 0; k < 3; k++) {
+                const double grad = g * r12[k];
+                f11[k] -= (1 - loc1) * grad;
+                f12[k] -= loc1 * grad;
+                f21[k] -= -(1 - loc2) * grad;
+                f22[k] -= -loc2 * grad;
+            }
+        }
+    }
+}
+/* CylinderVolumeMon::computeEnergy, :189-195 */
+double orc_evm_energy(const double *x, int32_t n, const int32_t *bs, const int32_t *mon, const double *meq,
+                      const double *kvol) {
+    osum U;
+    os_zero(&U);
+    for (int32_t i = 0; i < n; i++)
+        os_add(&U, evm_pair_energy(x + bs[4 * i], x + bs[4 * i + 1], x + bs[4 * i + 2], x + bs[4 * i + 3], mon + 6 * i,
+                                   meq + 2 * i, kvol[i]));
+    return os_val(&U);
+}
+/* CylinderVolumeMon::computeForces, :196-200. The reference subtracts every
+ * monomer pair's share straight into the force array; in the exact mode each
+ * interaction's four bead forces start from zero and are added afterwards, the
+ * association of the per-bead gather (the two differ by rounding). */
+void orc_evm_forces(const double *x, double *f, int32_t n, const int32_t *bs, const int32_t *mon, const double *meq,
+                    const double *kvol) {
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t *b = bs + 4 * i;
+        if (g_exact) {
+            double loc[12] = {0};
+            evm_pair_force(x + b[0], x + b[1], x + b[2], x + b[3], mon + 6 * i, meq + 2 * i, kvol[i], loc, loc + 3,
+                           loc + 6, loc + 9);
+            for (int k = 0; k < 3; k++) {
+                f[b[0] + k] += loc[k];
+                f[b[1] + k] += loc[3 + k];
+                f[b[2] + k] += loc[6 + k];
+                f[b[3] + k] += loc[9 + k];
+            }
+        } else {
+            evm_pair_force(x + b[0], x + b[1], x + b[2], x + b[3], mon + 6 * i, meq + 2 * i, kvol[i], f + b[0],
+                           f + b[1], f + b[2], f + b[3]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* a10: BoundaryCylinderRepulsionExp (Boundary/BoundaryCylinderRepulsionExp.cpp:27-62, 65-110) */
 /* PlaneBoundaryElement::distance (Structure/BoundaryElementImpl.h:71-74)      */
 /* ------------------------------------------------------------------------- */
@@ -1003,6 +1102,26 @@ int64_t orc_ev_vectorize(const mdy_cylinders *cyl, const int32_t *pairs, int64_t
     return m;
 }
 
+/* CylinderVolumeMon::vectorize (Volume/CylinderVolumeMon.hpp:126-185): drop
+ * mother-daughter pairs in either order; kvol is the larger constant. */
+int64_t orc_evm_vectorize(const mdy_cylinders *cyl, const int32_t *pairs, int64_t npairs, int32_t *bs, int32_t *mon,
+                          double *meq, double *kvol) {
+    int64_t m = 0;
+    for (int64_t p = 0; p < npairs; p++) {
+        int32_t ci = pairs[2 * p], cn = pairs[2 * p + 1];
+        if (cyl->branch_child[ci] == cn || cyl->branch_child[cn] == ci) continue;
+        bs[4 * m] = cyl->bead1[ci]; bs[4 * m + 1] = cyl->bead2[ci];
+        bs[4 * m + 2] = cyl->bead1[cn]; bs[4 * m + 3] = cyl->bead2[cn];
+        int32_t *mm = mon + 6 * m;
+        mm[0] = cyl->mon_minus[ci]; mm[1] = cyl->mon_plus[ci]; mm[2] = cyl->mon_interval[ci];
+        mm[3] = cyl->mon_minus[cn]; mm[4] = cyl->mon_plus[cn]; mm[5] = cyl->mon_interval[cn];
+        meq[2 * m] = cyl->mon_size[ci]; meq[2 * m + 1] = cyl->mon_size[cn];
+        kvol[m] = smax(cyl->exvol_k[ci], cyl->exvol_k[cn]);
+        m++;
+    }
+    return m;
+}
+
 /* BoundaryCylinderNL::updateNeighbors (NeighborListImpl.cpp:738-751) with the
  * float cutoff (NeighborList.h:38-39), then BoundaryCylinderRepulsion::vectorize
  * (BoundaryCylinderRepulsion.cpp:33-88): the first bead of minus-end cylinders,
@@ -1052,7 +1171,12 @@ int orc_system_setup(orc_system *o, const mdy_system *sys, const double *x) {
         o->ev_bs = (int32_t *)malloc(sizeof(int32_t) * 4 * (size_t)(np + 1));
         o->ev_krep = (double *)malloc(sizeof(double) * (size_t)(np + 1));
         o->ev_eql = (double *)malloc(sizeof(double) * 2 * (size_t)(np + 1));
-        o->ev_n = orc_ev_vectorize(cyl, pairs, np, o->ev_bs, o->ev_krep, o->ev_eql);
+        if (sys->exvol_type == MDY_EXVOL_MONOMER) {
+            o->ev_mon = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)(np + 1));
+            o->ev_n = orc_evm_vectorize(cyl, pairs, np, o->ev_bs, o->ev_mon, o->ev_eql, o->ev_krep);
+        } else {
+            o->ev_n = orc_ev_vectorize(cyl, pairs, np, o->ev_bs, o->ev_krep, o->ev_eql);
+        }
         free(pairs);
     }
     if (sys->boundary_enabled && sys->bnd.n_planes > 0) {
@@ -1067,7 +1191,7 @@ int orc_system_setup(orc_system *o, const mdy_system *sys, const double *x) {
 }
 
 void orc_system_free(orc_system *o) {
-    free(o->ev_bs); free(o->ev_krep); free(o->ev_eql);
+    free(o->ev_bs); free(o->ev_krep); free(o->ev_eql); free(o->ev_mon);
     free(o->bd_nneigh); free(o->bd_bs); free(o->bd_krep); free(o->bd_slen);
     memset(o, 0, sizeof(*o));
 }
@@ -1082,7 +1206,9 @@ static void ffm_each(const orc_system *o, const double *x, double *e) {
     e[MDY_FF_BB] = orc_bb_energy(x, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0);
     e[MDY_FF_BDH] = orc_bdh_energy(x, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos);
     e[MDY_FF_BP] = orc_bp_energy(x, t->bp_n, t->bp_beadset, t->bp_k, t->bp_pos);
-    e[MDY_FF_EV] = o->sys.exvol_enabled ? orc_ev_energy(x, (int32_t)o->ev_n, o->ev_bs, o->ev_krep, o->ev_eql) : 0.0;
+    if (!o->sys.exvol_enabled) e[MDY_FF_EV] = 0.0;
+    else if (o->ev_mon) e[MDY_FF_EV] = orc_evm_energy(x, (int32_t)o->ev_n, o->ev_bs, o->ev_mon, o->ev_eql, o->ev_krep);
+    else e[MDY_FF_EV] = orc_ev_energy(x, (int32_t)o->ev_n, o->ev_bs, o->ev_krep, o->ev_eql);
     e[MDY_FF_BD] = (o->sys.boundary_enabled && o->sys.bnd.n_planes > 0)
                        ? orc_bd_energy(x, o->sys.bnd.n_planes, o->sys.bnd.plane, o->bd_nneigh, o->bd_bs, o->bd_krep,
                                        o->bd_slen)
@@ -1131,7 +1257,9 @@ int orc_ffm_forces(const orc_system *o, const double *x, double *f, int32_t nvar
     orc_bb_forces(x, f, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0, NULL);
     orc_bdh_forces(x, f, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos, NULL);
     orc_bp_forces(x, f, t->bp_n, t->bp_beadset, t->bp_k, t->bp_pos, NULL);
-    if (o->sys.exvol_enabled) {
+    if (o->sys.exvol_enabled && o->ev_mon) {
+        orc_evm_forces(x, f, (int32_t)o->ev_n, o->ev_bs, o->ev_mon, o->ev_eql, o->ev_krep);
+    } else if (o->sys.exvol_enabled) {
         int rc = orc_ev_forces(x, f, (int32_t)o->ev_n, o->ev_bs, o->ev_krep, o->ev_eql, o->coplanar_mode);
         if (rc) return rc;
     }

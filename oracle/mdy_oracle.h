@@ -17,6 +17,10 @@
  *       :125-163 consistency), brute-force 2-D quadrature, swap symmetry,
  *       central differences
  *   excluded volume coplanar branch   : perturbed-geometry invariant
+ *   excluded volume (monomer sum)     : the reference's energy-force
+ *       consistency fixtures (TestCylinderExclVolumeMon.cpp), its integral
+ *       vs monomer-sum check (TestCylinderExclVolume.cpp:48-100), the sample
+ *       count on point cylinders against a closed form, central differences
  *   excluded volume Simpson fallbacks : parity unpinned (only a reference run
  *       could pin them; checked against a high-order quadrature of the same
  *       integral within the Simpson error)
@@ -80,6 +84,12 @@ double orc_ev_pair_energy_simpson(const double *c1, const double *c2, const doub
 void   orc_ev_pair_force_simpson(const double *c1, const double *c2, const double *c3, const double *c4,
                                  double kr, int n1, int n2, double *f12);
 int    orc_are_in_plane(const double *c1, const double *c2, const double *c3, const double *c4);
+/* monomer-sum excluded volume (CylinderVolumeMon): per interaction mon[6] =
+ * (minus, plus, interval) of each cylinder, meq[2] monomer sizes, kvol */
+double orc_evm_energy(const double *x, int32_t n, const int32_t *bs, const int32_t *mon, const double *meq,
+                      const double *kvol);
+void   orc_evm_forces(const double *x, double *f, int32_t n, const int32_t *bs, const int32_t *mon, const double *meq,
+                      const double *kvol);
 double orc_bd_energy(const double *x, int32_t nplanes, const double *plane, const int32_t *nneigh,
                      const int32_t *bs, const double *krep, const double *slen);
 void   orc_bd_forces(const double *x, double *f, int32_t nplanes, const double *plane, const int32_t *nneigh,
@@ -93,6 +103,9 @@ int64_t orc_nl_pairs(const mdy_cylinders *cyl, const double *x, float rmax, floa
 /* CylinderExclVolume::vectorize over the pair list; returns interactions. */
 int64_t orc_ev_vectorize(const mdy_cylinders *cyl, const int32_t *pairs, int64_t npairs,
                          int32_t *bs, double *krep, double *eql);
+/* CylinderVolumeMon::vectorize over the pair list; returns interactions. */
+int64_t orc_evm_vectorize(const mdy_cylinders *cyl, const int32_t *pairs, int64_t npairs, int32_t *bs, int32_t *mon,
+                          double *meq, double *kvol);
 /* BoundaryCylinderNL::reset + BoundaryCylinderRepulsion::vectorize. */
 int64_t orc_bd_vectorize(const mdy_cylinders *cyl, const double *x, const mdy_boundary *bnd,
                          int32_t *nneigh, int32_t *bs, double *krep, double *slen, int64_t cap);
@@ -101,6 +114,7 @@ int64_t orc_bd_vectorize(const mdy_cylinders *cyl, const double *x, const mdy_bo
 typedef struct orc_system {
     mdy_system sys;              /* borrowed tables (bonded / cross-link) */
     int64_t ev_n;  int32_t *ev_bs;  double *ev_krep; double *ev_eql;
+    int32_t *ev_mon;             /* monomer-sum form: 6 per interaction (ev_krep = kvol, ev_eql = sizes) */
     int64_t bd_n;  int32_t *bd_nneigh; int32_t *bd_bs; double *bd_krep; double *bd_slen;
     int coplanar_mode;
 } orc_system;

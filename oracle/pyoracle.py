@@ -31,7 +31,7 @@ class OrcTrace(C.Structure):
 
 class OrcSystem(C.Structure):
     _fields_ = [("sys", _abi.System),
-                ("ev_n", C.c_int64), ("ev_bs", _i32p), ("ev_krep", _f64p), ("ev_eql", _f64p),
+                ("ev_n", C.c_int64), ("ev_bs", _i32p), ("ev_krep", _f64p), ("ev_eql", _f64p), ("ev_mon", _i32p),
                 ("bd_n", C.c_int64), ("bd_nneigh", _i32p), ("bd_bs", _i32p), ("bd_krep", _f64p),
                 ("bd_slen", _f64p), ("coplanar_mode", C.c_int)]
 
@@ -55,6 +55,8 @@ class Oracle:
         L.orc_ffm_energy.restype = C.c_double
         L.orc_ffm_energy_individual.restype = C.c_double
         L.orc_ev_energy.restype = C.c_double
+        L.orc_evm_energy.restype = C.c_double
+        L.orc_evm_vectorize.restype = C.c_int64
         L.orc_ev_pair_energy_analytic.restype = C.c_double
         L.orc_ev_pair_energy_simpson.restype = C.c_double
         for fn in ("orc_fs_energy", "orc_fb_energy", "orc_ls_energy", "orc_bs_energy", "orc_bb_energy",
@@ -161,6 +163,22 @@ class Oracle:
         rc = self.lib.orc_ev_forces(_p(x), _p(f), len(krep), _p(bs, C.c_int32), _p(krep), _p(eql), coplanar_mode)
         if rc:
             raise RuntimeError(_abi.ERRORS.get(rc, rc))
+        return f
+
+    def evm_energy(self, x, bs, mon, meq, kvol):
+        """Monomer-sum excluded volume: bs (n, 4) coordinate offsets, mon (n, 6)
+        minus/plus/interval of both cylinders, meq (n, 2) monomer sizes, kvol (n)."""
+        x, bs, mon, meq, kvol = (np.ascontiguousarray(x, np.float64), np.ascontiguousarray(bs, np.int32),
+                                 np.ascontiguousarray(mon, np.int32), np.ascontiguousarray(meq, np.float64),
+                                 np.ascontiguousarray(kvol, np.float64))
+        return self.lib.orc_evm_energy(_p(x), len(kvol), _p(bs, C.c_int32), _p(mon, C.c_int32), _p(meq), _p(kvol))
+
+    def evm_forces(self, x, bs, mon, meq, kvol):
+        x, bs, mon, meq, kvol = (np.ascontiguousarray(x, np.float64), np.ascontiguousarray(bs, np.int32),
+                                 np.ascontiguousarray(mon, np.int32), np.ascontiguousarray(meq, np.float64),
+                                 np.ascontiguousarray(kvol, np.float64))
+        f = np.zeros_like(x)
+        self.lib.orc_evm_forces(_p(x), _p(f), len(kvol), _p(bs, C.c_int32), _p(mon, C.c_int32), _p(meq), _p(kvol))
         return f
 
     def bd_energy(self, x, plane, nneigh, bs, krep, slen):

@@ -1,0 +1,137 @@
+"""Monomer-sum excluded volume (CylinderVolumeMon, cylinderVolumeExclusionFFType
+MONOMER) on the device against the oracle, whose restatement is pinned in
+test_oracle_evm.py: neighbor tables and pair coefficients, energies and
+forces bit for bit against the exact-mode oracle and within 1e-10 of the
+reference-order oracle, bit-identical minimization trajectories, the
+mixed-precision form, the batched launch and Hessian-vector products.
+Monomer intervals 1, 2 and 4 (1600, 400 and 100 monomer pairs per cylinder
+pair)."""
+import numpy as np
+import pytest
+import torch
+
+from medyan_amd import _abi, autodiff, mechanics, network
+from test_gpu_parity import bits_equal, ctx_for, exact  # noqa: F401  (exact is a fixture)
+
+pytestmark = pytest.mark.gpu
+NFF = _abi.MDY_NFF
+
+
+@pytest.fixture(scope="module")
+def mnets(lib):
+    return {
+        "net500_i4": network.generate(lib, "net500").with_monomer_exvol(4),
+        "branched_i1": network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8,
+                                        seed=2).with_monomer_exvol(1),
+        "dense_i2": network.generate(lib, "net500", n_filaments=60, cyl_per_filament=3, box=500.0, n_linkers=40,
+                                     n_motors=4, seed=3).with_monomer_exvol(2),
+    }
+
+
+CASES = ["net500_i4", "branched_i1", "dense_i2"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_tables_energies_forces_bitwise(exact, mnets, case):
+    net = mnets[case]
+    print(f"[gpu] monomer-sum excluded volume, {case}", flush=True)
+    ctx = ctx_for(net)
+    s = exact.system(net)
+    bs, kr = ctx.exvol_table()
+    obs, okvol, _ = s.ev_table
+    np.testing.assert_array_equal(bs, obs)
+    cyl_of = {int(b): i for i, b in enumerate(net.cyl_bead1)}
+    c1 = np.array([cyl_of[int(b)] for b in obs[:, 0]])
+    c2 = np.array([cyl_of[int(b)] for b in obs[:, 2]])
+    seg = net.cyl_mon_interval.astype(np.float64) * net.cyl_mon_size
+    assert bits_equal(kr, okvol * seg[c1] * seg[c2])          # kvol * segEqLen1 * segEqLen2
+    rng = np.random.default_rng(5)
+    for x in (net.coord, net.coord + rng.normal(0, 0.5, net.coord.shape)):
+        e, per = ctx.energy(x)
+        eo, pero = s.energy(x)
+        assert bits_equal([e], [eo])
+        np.testing.assert_allclose(per, pero, rtol=1e-13, atol=0)
+        assert per[_abi.MDY_FF_EV] > 0
+        assert bits_equal(ctx.forces(x), s.forces(x))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_reference_order_within_1e10(oracle, mnets, case):
+    net = mnets[case]
+    ctx = ctx_for(net)
+    s = oracle.system(net)
+    e, per = ctx.energy(net.coord)
+    eo, pero = s.energy(net.coord)
+    assert abs(e - eo) <= 1e-10 * abs(eo)
+    f, fo = ctx.forces(net.coord), s.forces(net.coord)
+    assert np.abs(f - fo).max() <= 1e-10 * np.abs(fo).max()
+
+
+@pytest.mark.parametrize("case", ["net500_i4", "branched_i1"])
+def test_minimization_trajectory_bitwise(exact, mnets, case):
+    net = mnets[case]
+    ctx = ctx_for(net)
+    s = exact.system(net)
+    p = _abi.default_cg_params(net.n_beads)
+    x, f, res = ctx.minimize(p, trace_cap=100000)
+    tr = ctx.trace()
+    xo, fo, reso, tro = s.minimize(p, trace_cap=100000)
+    assert res.status == 0 and reso.status == 0 and res.error_flags == 0
+    assert (res.num_iter, res.num_energy_call, res.num_force_call) == (
+        reso.num_iter, reso.num_energy_call, reso.num_force_call)
+    for key in ("lambda_", "energy", "maxf", "beta"):
+        assert bits_equal(tr[key], tro[key]), key
+    assert bits_equal(x, xo) and bits_equal(f, fo)
+    print(f"[gpu] {case}: {res.num_iter} iterations, max|F| {res.grad_inf_norm:.3f}")
+
+
+def test_mixed_precision_form(exact, mnets):
+    """The monomer sum runs in FP64 in both forms; the mixed form stores its
+    per-pair forces in FP32 slots."""
+    net = mnets["dense_i2"]
+    ctx = mechanics.Context(precision=_abi.MDY_PREC_MIXED)
+    ctx.set_network(net)
+    ctx.build_neighbors()
+    s = exact.system(net)
+    e, _ = ctx.energy(net.coord)
+    assert bits_equal([e], [s.energy(net.coord)[0]])
+    fo = s.forces(net.coord)
+    assert np.abs(ctx.forces(net.coord) - fo).max() <= 1e-6 * np.abs(fo).max()
+    x, f, res = ctx.minimize(_abi.default_cg_params(net.n_beads))
+    assert res.status == 0 and res.error_flags == 0 and res.grad_inf_norm < 1.0
+    assert np.abs(s.forces(x) - f).max() <= 1e-5 * max(1.0, np.abs(f).max())
+
+
+def test_batch_with_monomer_networks(mnets):
+    nets = [mnets["net500_i4"], mnets["branched_i1"], mnets["net500_i4"]]
+    singles = [ctx_for(n).minimize(_abi.default_cg_params(n.n_beads)) for n in nets]
+    ctxs = [ctx_for(n) for n in nets]
+    xs = [torch.tensor(n.coord, device="cuda") for n in nets]
+    res, _, _ = mechanics.minimize_batch(ctxs, [_abi.default_cg_params(n.n_beads) for n in nets],
+                                         [x.data_ptr() for x in xs], threads_per_block=128)
+    torch.cuda.synchronize()
+    for i, (x1, _, r1) in enumerate(singles):
+        assert res[i].status == 0 and res[i].num_iter == r1.num_iter
+        assert bits_equal(xs[i].cpu().numpy(), x1)
+
+
+def test_hvp_and_energy_jvp(oracle, mnets):
+    net = mnets["branched_i1"]
+    ctx = mechanics.Context(coplanar_mode=_abi.MDY_COPLANAR_EXACT)
+    ctx.set_network(net)
+    ctx.build_neighbors()
+    s = oracle.system(net)
+    rng = np.random.default_rng(7)
+    v = rng.normal(size=3 * net.n_beads)
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a, np.float64), device="cuda")
+    hv = autodiff.hvp(ctx, dev(net.coord), dev(v)).cpu().numpy()
+    h = 1e-3
+    fd = -(s.forces(net.coord + h * v) - s.forces(net.coord - h * v)) / (2 * h)
+    err = np.abs(hv - fd).max() / np.abs(fd).max()
+    print(f"[gpu] monomer-sum hvp: max |Hv - CD| / max|CD| = {err:.2e}")
+    assert err < 2e-6
+    w = rng.normal(size=3 * net.n_beads)
+    jv = autodiff.energy_jvp(ctx, dev(net.coord), dev(w)).cpu().numpy()
+    h = 1e-4
+    cd = (s.energy(net.coord + h * w, individual=True)[1] - s.energy(net.coord - h * w, individual=True)[1]) / (2 * h)
+    np.testing.assert_allclose(jv[:NFF], cd, rtol=0, atol=1e-6 * np.abs(cd).max())
