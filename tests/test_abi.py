@@ -70,3 +70,24 @@ def test_ctypes_layouts_match_header():
     sizes = dict(line.split() for line in out.strip().splitlines())
     for s in STRUCTS:
         assert int(sizes[s]) == C.sizeof(MIRROR[s]), s
+
+
+def test_ctypes_field_offsets_match_header():
+    """Every field of every mirrored struct at the C offset (equal sizes alone
+    would not catch two fields in swapped order)."""
+    prog = "#include <stdio.h>\n#include <stddef.h>\n#include \"mdy.h\"\nint main(void){\n"
+    expect = {}
+    for s in STRUCTS:
+        for name, _ in MIRROR[s]._fields_:
+            prog += f'printf("{s}.{name} %zu\\n", offsetof({s}, {name}));\n'
+            expect[f"{s}.{name}"] = getattr(MIRROR[s], name).offset
+    prog += "return 0;}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "off.c")
+        exe = os.path.join(d, "off")
+        open(c, "w").write(prog)
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    got = {k: int(v) for k, v in (line.split() for line in out.strip().splitlines())}
+    bad = {k: (got[k], expect[k]) for k in expect if got[k] != expect[k]}
+    assert not bad, bad
