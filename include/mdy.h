@@ -212,6 +212,17 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *sys);
 int mdy_build_neighbors(mdy_ctx *ctx, const double *coord);
 /* Same from device-resident coordinates. */
 int mdy_build_neighbors_dev(mdy_ctx *ctx, const double *d_coord);
+/* CGMethod::searchDir after the context's last minimization (3 * n_beads
+ * doubles to host memory): the direction after the last update, shifted by
+ * beta after the final accepted step and reset to the force when conjugacy is
+ * lost (CGMethod.hpp:1495-1507, CGMethod.cpp:356-360). MController::run passes
+ * it with CGMethod::LAMBDATOL (MDY_LAMBDATOL) to its line-search diagnosis
+ * after a line-search error (Controller/MController.h:142-152). Where a reset
+ * made it the force itself, components that are exactly zero may differ in
+ * sign from the reference's F + 0 * d. MDY_ERR_STATE before a minimization. */
+int mdy_get_search_dir(mdy_ctx *ctx, double *dir_out);
+#define MDY_LAMBDATOL 1e-8   /* CGMethod::LAMBDATOL (CGMethod.hpp:99) */
+
 /* Cylinder-cylinder neighbor pairs (index of c, index of neighbor n,
  * id(c) > id(n)), sorted by (c, n): the list HybridCylinderCylinderNL holds,
  * mother-daughter pairs included. */

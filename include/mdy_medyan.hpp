@@ -169,8 +169,13 @@ class CGMethod {
 public:
     // Minimizes coord[0, numDof) in place with the manager's force fields;
     // outForce (nullable) receives the final forces.
+    // CGMethod::searchDir and LAMBDATOL after minimize, which MController::run
+    // hands to its line-search diagnosis (MController.h:142-152)
+    std::vector<double> searchDir;
+    double LAMBDATOL = MDY_LAMBDATOL;
+
     MinimizationResult minimize(const ConjugateGradientParams &p, std::vector<double> &coord, int numDof,
-                                ForceFieldManager &ffm, std::vector<double> *outForce = nullptr) const {
+                                ForceFieldManager &ffm, std::vector<double> *outForce = nullptr) {
         if ((int)coord.size() < numDof || numDof != ffm.numDof())
             throw Error(MDY_ERR_ARG, "coordinate vector does not match the network");
         mdy_cg_params cp{};
@@ -205,6 +210,8 @@ public:
         m.energyRelChange = r.energy_rel_change;
         m.energyRelTol = r.energy_rel_tol;
         m.numIter = r.num_iter;
+        searchDir.assign(numDof, 0.0);
+        ffm.check(mdy_get_search_dir(ffm.ctx(), searchDir.data()));
         return m;
     }
 };

@@ -198,6 +198,7 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
     hipStream_t st = c.stream;
     c.have_sys = false;
     c.have_nl = false;
+    c.have_dir = false;
     c.nb = nb;
     c.n_fs = t.fs_n; c.n_fb = t.fb_n; c.n_ls = t.ls_n; c.n_ms = t.ms_n; c.n_bs = t.bs_n; c.n_bb = t.bb_n;
     c.n_bdh = t.bdh_n; c.n_bp = t.bp_n;
@@ -353,6 +354,16 @@ int mdy_build_neighbors(mdy_ctx *ctx, const double *coord) {
     if (hipMemcpyAsync(c.x.p, coord, sizeof(double) * 3 * c.nb, hipMemcpyHostToDevice, c.stream) != hipSuccess)
         return hipfail(ctx, "coord upload");
     return mdy_build_neighbors_dev(ctx, c.x.p);
+}
+
+int mdy_get_search_dir(mdy_ctx *ctx, double *dir_out) {
+    if (!ctx || !dir_out) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    const int rc = final_search_dir(c, dir_out);
+    if (rc == MDY_ERR_STATE) return fail(ctx, rc, "no completed minimization on this network");
+    if (rc != MDY_OK) return fail(ctx, rc, "search direction readback failed");
+    return MDY_OK;
 }
 
 int mdy_get_pairs(mdy_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n) {

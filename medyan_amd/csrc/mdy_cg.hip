@@ -58,6 +58,7 @@ enum {
     R_FLAGS = R_EA + kNFF + 1,
     R_NEC, R_NFC, R_NIT, R_GIN, R_ERC, R_NEE, R_NSM, R_NREC, R_STATUS, R_TRN, R_LAST, R_FOUT, R_NPH,
     R_T0, R_T1,            // 100 MHz wall clock at the leader's start and end
+    R_DIR, R_PEND, R_BETA, R_DPREV,   // search direction at the end (buffer ids 0 F0, 1 F1, 2 S0, 3 S1)
     R_SIZE
 };
 
@@ -808,6 +809,12 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         R[R_FLAGS] = flags; R[R_NEC] = nEC; R[R_NFC] = nFC; R[R_NIT] = nIter; R[R_GIN] = gradInf; R[R_ERC] = relE;
         R[R_NEE] = nEE; R[R_NSM] = nSM; R[R_NREC] = nRec; R[R_STATUS] = K.status; R[R_TRN] = (double)trn;
         R[R_LAST] = after ? 1.0 : 0.0; R[R_FOUT] = (double)cur; R[R_NPH] = (double)K.ph;
+        // the reference shifts the direction right after an accepted step; here
+        // that shift is pending until the next trial
+        auto bufid = [&](const double *q) {
+            return q == A.F0 ? 0.0 : q == A.F1 ? 1.0 : q == A.S0 ? 2.0 : q == A.S1 ? 3.0 : -1.0;
+        };
+        R[R_DIR] = bufid(dvec); R[R_PEND] = pendingSpec ? 1.0 : 0.0; R[R_BETA] = beta; R[R_DPREV] = bufid(dprev);
         R[R_T1] = (double)wall_clock64();
     }
     return;
@@ -920,6 +927,52 @@ static void collect(Ctx &c, const double *R, const mdy_cg_params *p, const CgArg
     res->status = (int32_t)R[R_STATUS];
     c.trace_n = (long long)R[R_TRN];
     c.prof_n = (long long)R[R_NPH];
+    c.have_dir = res->status == MDY_OK;
+    c.dir_buf = (int)R[R_DIR]; c.dir_pend = R[R_PEND] != 0.0; c.dir_beta = R[R_BETA];
+    c.dir_prev = (int)R[R_DPREV]; c.dir_cur = (int)R[R_FOUT];
+}
+
+// s = F + beta d and the exact s.F, in one block (cgShiftSearchDir and
+// resetCGParams, CGMethod.cpp:325-360: the same expression and the same exact
+// sum as the kernel's direction update)
+__global__ void __launch_bounds__(1024) k_final_dir(const double *__restrict__ F, const double *__restrict__ d,
+                                                     double beta, int n, double *__restrict__ s, double *sf) {
+    __shared__ long long lds[136];
+    mdy_xsum x;
+    mdy_xs_zero(&x);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v = F[i] + beta * d[i];
+        s[i] = v;
+        mdy_xs_add(&x, v * F[i]);
+    }
+    block_reduce_xsum(x, lds);
+    if (threadIdx.x == 0) sf[0] = mdy_xs_value(&x);
+}
+
+int final_search_dir(Ctx &c, double *h_out) {
+    if (!c.have_dir) return MDY_ERR_STATE;
+    hipStream_t st = c.stream;
+    const size_t n = 3 * (size_t)c.nb;
+    auto buf = [&](int id) -> const double * {
+        return id == 0 ? c.F0.p : id == 1 ? c.F1.p : id == 2 ? c.S0.p : id == 3 ? c.S1.p : nullptr;
+    };
+    const double *src = buf(c.dir_buf);
+    if (c.dir_pend) {
+        const double *F = buf(c.dir_cur), *d = buf(c.dir_prev);
+        if (!F || !d) return MDY_ERR_STATE;
+        MDY_TRY(ensure(c.xt, n));
+        MDY_TRY(ensure(c.out, 8));
+        hipLaunchKernelGGL(k_final_dir, dim3(1), dim3(1024), 0, st, F, d, c.dir_beta, (int)n, c.xt.p, c.out.p);
+        MDY_HIP(hipGetLastError());
+        double sf = 0;
+        MDY_HIP(hipMemcpyAsync(&sf, c.out.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        MDY_HIP(hipStreamSynchronize(st));
+        src = (sf <= 0) ? F : c.xt.p;   // resetCGParams when s.F <= 0
+    }
+    if (!src) return MDY_ERR_STATE;
+    MDY_HIP(hipMemcpyAsync(h_out, src, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    MDY_HIP(hipStreamSynchronize(st));
+    return MDY_OK;
 }
 
 static int copy_force_out(Ctx &c, const double *R, double *d_force_out, hipStream_t st) {

@@ -116,6 +116,12 @@ struct Ctx {
     DBuf<unsigned char> batch_args;  // batched launch: per-network kernel arguments
     DBuf<int2> batch_map;            // batched launch: (network, block within its cluster)
     double last_ms = 0;
+    // the last minimization's search direction (CGMethod::searchDir): the CG
+    // buffer that holds it (0 F0, 1 F1, 2 S0, 3 S1) or, when its final update
+    // s = F + beta d is still pending, the current force buffer, d and beta
+    bool have_dir = false, dir_pend = false;
+    int dir_buf = -1, dir_cur = 0, dir_prev = -1;
+    double dir_beta = 0;
     int coop_blocks = 0;
     Dev dev() const;
 };

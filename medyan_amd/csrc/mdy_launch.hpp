@@ -12,6 +12,8 @@ int build_neighbors(Ctx &c, const double *d_x);
 int exclusive_scan(Ctx &c, const int *in, int *out, int n, hipStream_t st);  // out has n+1 entries
 // mdy_cg.hip
 int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res);
+// CGMethod::searchDir after the context's last minimization, to host memory (3 n_beads doubles)
+int final_search_dir(Ctx &c, double *h_out);
 int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d_x, double *const *d_force_out,
                        mdy_min_result *res, double *net_ms, int threads_per_block, const double *weights,
                        int first_pass_calls, hipStream_t st, double *total_ms);

@@ -301,6 +301,12 @@ class Context:
         self._check(self.lib.mdy_energy_jvp_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_w), C.c_void_p(d_out),
                                                 C.c_void_p(stream) if stream else None))
 
+    def search_dir(self):
+        """CGMethod::searchDir after the last minimization (see mdy_get_search_dir)."""
+        d = np.zeros(3 * self.net.n_beads)
+        self._check(self.lib.mdy_get_search_dir(self.h, _f64(d)))
+        return d
+
     def trace(self):
         n = C.c_int64()
         self._check(self.lib.mdy_get_trace(self.h, None, None, None, None, None, 0, C.byref(n)))

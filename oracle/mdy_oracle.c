@@ -1433,6 +1433,14 @@ static void reset_cg(cg_state *s, int method, double *lastFDotF) {
     if (method == MDY_CG_POLAK_RIBIERE) memcpy(s->forcePrev, s->force, sizeof(double) * (size_t)s->nvar);
 }
 
+/* CGMethod::searchDir as the last orc_minimize left it */
+static double *g_last_dir = NULL;
+static int32_t g_last_dir_n = 0;
+int32_t orc_last_search_dir(double *out, int32_t cap) {
+    if (out) memcpy(out, g_last_dir, sizeof(double) * (size_t)(cap < g_last_dir_n ? cap : g_last_dir_n));
+    return g_last_dir_n;
+}
+
 static void trace_push(orc_trace *t, double lam, double e, double mf, double beta, int safe) {
     if (!t || t->n >= t->cap) return;
     t->lambda[t->n] = lam; t->energy[t->n] = e; t->maxf[t->n] = mf; t->beta[t->n] = beta; t->safe[t->n] = safe;
@@ -1561,6 +1569,10 @@ done:
     r->num_iter = numIter;
 done_noafter:
     r->status = status;
+    free(g_last_dir);
+    g_last_dir = (double *)malloc(sizeof(double) * ((size_t)ndof + 1));
+    memcpy(g_last_dir, s.searchDir, sizeof(double) * (size_t)ndof);
+    g_last_dir_n = ndof;
     free(s.force); free(s.forcePrev); free(s.searchDir); free(s.coordLS); free(s.coordMinE);
     return status;
 }

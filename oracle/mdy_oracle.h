@@ -142,6 +142,9 @@ int orc_minimize(const mdy_cg_params *p, double *x, int32_t nvar, int32_t ndof,
 /* whole-network minimization with the system's own energy/force/recovery */
 int orc_minimize_system(const orc_system *o, const mdy_cg_params *p, double *x, double *out_force,
                         mdy_min_result *res, orc_trace *trace, int32_t iter_cap);
+/* CGMethod::searchDir left by the last orc_minimize call (copies up to cap
+ * values, returns its length) */
+int32_t orc_last_search_dir(double *out, int32_t cap);
 /* the reference recovery move (MController::moveAlongDirectionLimitingBeadMove_) */
 int orc_recover_move(void *user, double *x, const double *dir, int32_t ndof, double maxdist);
 

@@ -57,6 +57,7 @@ class Oracle:
         L.orc_ev_energy.restype = C.c_double
         L.orc_evm_energy.restype = C.c_double
         L.orc_evm_vectorize.restype = C.c_int64
+        L.orc_last_search_dir.restype = C.c_int32
         L.orc_ev_pair_energy_analytic.restype = C.c_double
         L.orc_ev_pair_energy_simpson.restype = C.c_double
         for fn in ("orc_fs_energy", "orc_fb_energy", "orc_ls_energy", "orc_bs_energy", "orc_bb_energy",
@@ -164,6 +165,13 @@ class Oracle:
         if rc:
             raise RuntimeError(_abi.ERRORS.get(rc, rc))
         return f
+
+    def last_search_dir(self):
+        """CGMethod::searchDir as the last minimization left it."""
+        n = self.lib.orc_last_search_dir(None, 0)
+        out = np.zeros(n)
+        self.lib.orc_last_search_dir(_p(out), n)
+        return out
 
     def evm_energy(self, x, bs, mon, meq, kvol):
         """Monomer-sum excluded volume: bs (n, 4) coordinate offsets, mon (n, 6)

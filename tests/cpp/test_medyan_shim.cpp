@@ -88,8 +88,15 @@ static int run() {
     // the quadratic line search from the same start
     std::vector<double> cq = start;
     p.lineSearchAlgorithm = "QUADRATIC";
-    MinimizationResult rq = CGMethod{}.minimize(p, cq, ndof, ffm);
+    CGMethod cg;
+    MinimizationResult rq = cg.minimize(p, cq, ndof, ffm);
     REQUIRE(rq.success());
+    REQUIRE((int)cg.searchDir.size() == ndof && cg.LAMBDATOL == 1e-8);
+    double sdotf = 0;
+    std::vector<double> fq(ndof);
+    ffm.computeForces(cq.data(), fq.data(), ndof);
+    for (int i = 0; i < ndof; i++) sdotf += cg.searchDir[i] * fq[i];
+    REQUIRE(sdotf > 0);   // a descent direction at the returned coordinates
     REQUIRE(rq.converged());
     REQUIRE(rq.energiesAfter.total < rq.energiesBefore.total);
 

@@ -25,7 +25,8 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for must in ("mdy_create", "mdy_destroy", "mdy_set_network", "mdy_build_neighbors", "mdy_get_pairs",
-                 "mdy_energy", "mdy_forces", "mdy_minimize", "mdy_minimize_dev", "mdy_hvp_dev", "mdy_energy_jvp_dev", "mdy_minimize_batch", "mdy_stretch_forces", "mdy_load_forces"):
+                 "mdy_energy", "mdy_forces", "mdy_minimize", "mdy_minimize_dev", "mdy_hvp_dev", "mdy_energy_jvp_dev", "mdy_minimize_batch", "mdy_stretch_forces", "mdy_load_forces",
+                 "mdy_get_search_dir"):
         assert must in names
 
 

@@ -258,6 +258,27 @@ def test_call_limit_and_line_search_flags(exact, nets):
     assert bits_equal(x2, xo2)
 
 
+def test_search_direction_after_minimization(exact, nets):
+    """mdy_get_search_dir: CGMethod::searchDir after a converged run (the
+    final shift still pending on the device), a call-limit stop and a
+    line-search error, against the oracle's (-0 and +0 compare equal)."""
+    net = nets["net500"]
+    ctx = ctx_for(net)
+    s = exact.system(net)
+    limit = _abi.default_cg_params(net.n_beads, force_call_limit=40, energy_rel_tol=0.0)
+    limit.n_beads_limit = 1
+    cases = {"converged": _abi.default_cg_params(net.n_beads), "call_limit": limit,
+             "line_search_error": _abi.default_cg_params(net.n_beads, try_to_recover=0, lambda_max=1e-9)}
+    for name, p in cases.items():
+        _, _, res = ctx.minimize(p)
+        d = ctx.search_dir()
+        _, _, reso, _ = s.minimize(p)
+        do = exact.last_search_dir()
+        assert res.num_iter == reso.num_iter, name
+        assert np.array_equal(d, do), name
+        print(f"[gpu] search direction after {name}: {res.num_iter} iterations, max|d| {np.abs(d).max():.3g}")
+
+
 def test_parallel_pair_quadrature_fallback(exact):
     """Exactly parallel cylinders: the analytic excluded-volume energy is 0/0
     and the Simpson rule takes over (energy path); the force path perturbs

@@ -106,6 +106,26 @@ def test_line_search_error_without_recovery(oracle):
     assert tr["lambda_"][0] == 0.0
 
 
+def test_search_direction_after_line_search_error(oracle):
+    """CGMethod::searchDir as MController reads it after a line-search error:
+    the first line search runs along the reset direction F(x0) + 0 * 0."""
+    p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0)
+    x0 = INITS["input_2"]
+    _, _, res, _ = oracle.minimize(p, x0, 10, quad_energy, uphill_force)
+    assert res.error_flags & _abi.MDY_ERROR_LINESEARCH
+    f0 = np.zeros(10)
+    uphill_force(x0, f0)
+    np.testing.assert_array_equal(oracle.last_search_dir(), f0)
+
+
+def test_search_direction_after_convergence_is_a_descent_direction(oracle):
+    p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0)
+    x, f, res, _ = oracle.minimize(p, INITS["input_2"], 10, quad_energy, quad_force)
+    assert res.status == 0 and res.num_iter > 0
+    d = oracle.last_search_dir()
+    assert float(d @ f) > 0          # shifted by beta, reset when conjugacy is lost
+
+
 def test_safe_mode_then_recovery(oracle):
     p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, energy_rel_tol=0.0, try_to_recover=1, force_call_limit=4,
                     n_beads_limit=1)
