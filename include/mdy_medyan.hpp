@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -104,9 +105,21 @@ public:
     // Neighbor lists only (the tables are unchanged).
     void updateNeighborLists(const double *coord) { check(mdy_build_neighbors(ctx_, coord)); }
 
-    double computeEnergy(const double *coord) const {
-        double e = 0;
-        check(mdy_energy(ctx_, coord, &e, nullptr));
+    // verbose prints each force field's energy; an invalid energy (-1 from a
+    // force field) makes the total +inf and names the culprit, as
+    // ForceFieldManager::computeEnergy does (ForceFieldManager.cpp:137-162)
+    double computeEnergy(const double *coord, bool verbose = false) const {
+        double e = 0, per[MDY_NFF];
+        check(mdy_energy(ctx_, coord, &e, per));
+        for (int k = 0; k < MDY_NFF; k++) {
+            if (verbose) std::cout << kForceFieldNames[k] << " energy = " << per[k] << std::endl;
+            if (per[k] <= -1) {
+                std::cout << "Energy = " << per[k] << std::endl;
+                std::cout << "Energy of system became infinite. Try adjusting minimization parameters." << std::endl;
+                std::cout << "The culprit was ... " << kForceFieldNames[k] << std::endl;
+                break;
+            }
+        }
         return e;
     }
     EnergyReport computeIndividualEnergies(const double *coord) const {
