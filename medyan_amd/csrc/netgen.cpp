@@ -1,6 +1,6 @@
 // netgen.cpp -- seeded synthetic actin networks for the benchmark workloads.
 //
-// Geometry rules (stated in DESIGN.md and identical for CPU and GPU runs):
+// Geometry rules (the same network for CPU and GPU runs of a seed):
 //   * a cubic box of edge L with MEDYAN's six boundary planes inset 25 nm
 //     (BoundaryCubic, src/Structure/BoundaryImpl.cpp:25-52);
 //   * straight filaments of `cyl_per_filament` full-length cylinders; start
