@@ -426,9 +426,11 @@ def bench_successive(a, lib):
         x = x_init.clone()
         gen = torch.Generator(device="cuda")
         gen.manual_seed(0x4D454459 + 4)
-        nl_ms, cg_ms, res = [], [], []
+        nl_ms, cg_ms, res, starts = [], [], [], []
         for m in range(a.warmup + a.steps):
             x.add_(torch.randn(x.shape, generator=gen, device="cuda", dtype=torch.float64), alpha=5.0)
+            if m >= a.warmup:
+                starts.append(x.clone())
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             ctx.build_neighbors_dev(x.data_ptr())
@@ -444,10 +446,28 @@ def bench_successive(a, lib):
                 nl_ms.append(1e3 * (t1 - t0))
                 cg_ms.append(ctx.last_ms())
                 res.append(r)
-        return ctx, nl_ms, cg_ms, res
+        return ctx, nl_ms, cg_ms, res, starts
 
-    mctx, m_nl, m_cg, m_res = chain(_abi.MDY_PREC_MIXED)
-    fctx, f_nl, f_cg, f_res = chain(_abi.MDY_PREC_FP64)
+    mctx, m_nl, m_cg, m_res, _ = chain(_abi.MDY_PREC_MIXED)
+    fctx, f_nl, f_cg, f_res, f_starts = chain(_abi.MDY_PREC_FP64)
+    # paired: the mixed form from each FP64 step's own start, final energies
+    # compared; control: the FP64 form from the same start displaced by
+    # N(0, 1e-9 nm), the resolution of the 1 pN stopping rule itself
+    nff = _abi.MDY_NFF
+    paired, control = [], []
+    cgen = torch.Generator(device="cuda")
+    cgen.manual_seed(7)
+    for xs, rf in zip(f_starts, f_res):
+        ef = rf.energies_after[nff]
+        x = xs.clone()
+        mctx.build_neighbors_dev(x.data_ptr())
+        r = mctx.minimize_dev(params, x.data_ptr())
+        paired.append(abs(r.energies_after[nff] - ef) / abs(ef))
+        x = xs.clone()
+        x.add_(torch.randn(x.shape, generator=cgen, device="cuda", dtype=torch.float64), alpha=1e-9)
+        fctx.build_neighbors_dev(x.data_ptr())
+        r = fctx.minimize_dev(params, x.data_ptr())
+        control.append(abs(r.energies_after[nff] - ef) / abs(ef))
     n_ev = len(mctx.exvol_table()[1])
     n_bd = len(mctx.boundary_table()[1])
     m_bytes = [algorithmic_bytes(net, n_ev, n_bd, r) for r in m_res]
@@ -483,6 +503,8 @@ def bench_successive(a, lib):
             "fp64_ms_per_step": float(np.mean(f_step)),
             "fp64_minimizations_per_s": 1e3 / float(np.mean(f_step)),
             "mixed_vs_fp64_speedup": float(np.mean(f_step) / np.mean(m_step)),
+            "paired_final_energy_rel_diff_mixed_vs_fp64": paired,
+            "control_final_energy_rel_diff_fp64_start_moved_1e-9nm": control,
             "kernel_GBps": {
                 "k_cg mixed": achieved,
                 "k_cg fp64": float(np.mean(f_gbs)),
