@@ -324,7 +324,7 @@ def bench_ensemble(a, lib):
 
     tune = {}
     for tpb in (64, 128):
-        for first in (500, 1000):
+        for first in (250, 500, 1000, 2000):
             tune[(tpb, first)] = run(tpb, first)[2]
             print(f"[ensemble256] autotune {tpb} threads, first pass {first} force calls: "
                   f"{tune[(tpb, first)]:.1f} ms", file=sys.stderr, flush=True)

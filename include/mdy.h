@@ -299,11 +299,11 @@ int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *params, double *d_coord,
  * default the per-evaluation work, interactions + 2 x beads), at least one
  * block each; each cluster runs its own minimization with its own barrier.
  * Networks beyond the resident block count run in further launches.
- * first_pass_calls > 0 runs the batch in passes: pass k stops each
- * minimization after first_pass_calls * 4^(k-1) force calls, and a network
- * stopped by that cap (below its own limit, not converged) restarts from its
- * initial coordinates in the next pass, whose blocks are shared among the
- * networks still running (uncapped once each gets 64 blocks or more).
+ * first_pass_calls > 0 runs the batch in passes: pass k suspends each
+ * minimization at the top of an iteration after first_pass_calls * 4^(k-1)
+ * force calls, and the next pass resumes the suspended ones where they
+ * stopped, on the blocks the finished networks released (uncapped once each
+ * gets 64 blocks or more).
  * Results equal those of one uncapped pass (a
  * trajectory does not depend on its cluster size); 0 = one pass.
  * params, d_coord (device, updated in place), results: n entries each;

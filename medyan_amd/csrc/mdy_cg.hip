@@ -49,6 +49,9 @@ struct CgArgs {
     double max_dist, lambda_max, grad_tol, energy_rel_tol;
     long long spin_limit;
     int grid;             // blocks of this minimization's cluster (batched launch)
+    // batched passes: suspend at the top of an iteration once this many force
+    // calls are done (0: never), and resume from the state a suspension saved
+    int suspend_calls, resume;
 };
 
 // result record layout (doubles)
@@ -59,7 +62,9 @@ enum {
     R_NEC, R_NFC, R_NIT, R_GIN, R_ERC, R_NEE, R_NSM, R_NREC, R_STATUS, R_TRN, R_LAST, R_FOUT, R_NPH,
     R_T0, R_T1,            // 100 MHz wall clock at the leader's start and end
     R_DIR, R_PEND, R_BETA, R_DPREV,   // search direction at the end (buffer ids 0 F0, 1 F1, 2 S0, 3 S1)
-    R_SIZE
+    R_SUSP,                // 1: the run suspended at the top of an iteration (batched passes)
+    R_RS,                  // the state a suspension saves (26 words)
+    R_SIZE = R_RS + 26
 };
 
 __device__ __forceinline__ int ld_i(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -519,6 +524,27 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
                (A.force_call_limit != 0 && nFC > A.force_call_limit);
     };
     auto converged = [&]() { return gradInf < A.grad_tol || relE < A.energy_rel_tol; };
+    // Suspension: everything the iterations carry from one to the next; the
+    // vectors (x, forces, directions, minimum-energy backup, trace) stay in
+    // the network's buffers. The phase counter restarts at 0 with cleared
+    // accumulators, which changes no value.
+    auto bufid = [&](const double *q) {
+        return q == A.F0 ? 0.0 : q == A.F1 ? 1.0 : q == A.S0 ? 2.0 : q == A.S1 ? 3.0 : -1.0;
+    };
+    auto bufptr = [&](double id) -> double * {
+        return id == 0.0 ? A.F0 : id == 1.0 ? A.F1 : id == 2.0 ? A.S0 : id == 3.0 ? A.S1 : nullptr;
+    };
+    if (leader) R[R_SUSP] = 0.0;
+    if (A.resume) {
+        const double *S = R + R_RS;
+        flags = (unsigned)S[0]; nEC = (int)S[1]; nFC = (int)S[2]; nIter = (int)S[3]; nEE = (int)S[4];
+        nSM = (int)S[5]; nRec = (int)S[6]; trn = (long long)S[7]; gradInf = S[8]; relE = S[9];
+        cur = (int)S[10]; sidx = (int)S[11]; dvec = bufptr(S[12]); maxF = S[13]; FFcur = S[14]; E0 = S[15];
+        prevEnergy = S[16]; lastFDotF = S[17]; slope = S[18]; safeMode = (int)S[19]; minimumE = S[20];
+        haveMin = S[21] != 0.0; beta = S[22]; pendingSpec = S[23] != 0.0; dprev = bufptr(S[24]);
+        Snew = bufptr(S[25]);
+        op = OP_NONE; stage = ST_ITER;
+    }
     auto trace = [&](double l, double e, double b) {
         if (leader && trn < A.tr_cap) {
             A.tr_lam[trn] = l; A.tr_e[trn] = e; A.tr_mf[trn] = gradInf; A.tr_beta[trn] = b;
@@ -598,6 +624,20 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             if (callLimit() || converged()) {
                 if (callLimit() && haveMin) op = OP_RESTORE;
                 stage = ST_FINAL;
+                break;
+            }
+            if (A.suspend_calls > 0 && nFC >= A.suspend_calls) {   // this pass ends; the next resumes here
+                if (leader) {
+                    double *S = R + R_RS;
+                    S[0] = flags; S[1] = nEC; S[2] = nFC; S[3] = nIter; S[4] = nEE; S[5] = nSM; S[6] = nRec;
+                    S[7] = (double)trn; S[8] = gradInf; S[9] = relE; S[10] = cur; S[11] = sidx; S[12] = bufid(dvec);
+                    S[13] = maxF; S[14] = FFcur; S[15] = E0; S[16] = prevEnergy; S[17] = lastFDotF; S[18] = slope;
+                    S[19] = safeMode; S[20] = minimumE; S[21] = haveMin ? 1.0 : 0.0; S[22] = beta;
+                    S[23] = pendingSpec ? 1.0 : 0.0; S[24] = bufid(dprev); S[25] = bufid(Snew);
+                    R[R_SUSP] = 1.0;
+                }
+                after = false;
+                stage = ST_FINISH;
                 break;
             }
             nIter++;
@@ -811,9 +851,6 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         R[R_LAST] = after ? 1.0 : 0.0; R[R_FOUT] = (double)cur; R[R_NPH] = (double)K.ph;
         // the reference shifts the direction right after an accepted step; here
         // that shift is pending until the next trial
-        auto bufid = [&](const double *q) {
-            return q == A.F0 ? 0.0 : q == A.F1 ? 1.0 : q == A.S0 ? 2.0 : q == A.S1 ? 3.0 : -1.0;
-        };
         R[R_DIR] = bufid(dvec); R[R_PEND] = pendingSpec ? 1.0 : 0.0; R[R_BETA] = beta; R[R_DPREV] = bufid(dprev);
         R[R_T1] = (double)wall_clock64();
     }
@@ -904,6 +941,8 @@ static int prepare_cg(Ctx &c, const mdy_cg_params *p, double *d_x, int G, hipStr
     A.energy_rel_tol = p->energy_rel_tol;
     A.spin_limit = 20000000LL;   // polls (about 10 s) before the watchdog fires
     A.grid = G;
+    A.suspend_calls = 0;
+    A.resume = 0;
     return MDY_OK;
 }
 
@@ -1026,14 +1065,14 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
 // default its per-evaluation work, interactions + 2 x beads), at least one
 // block each; networks beyond the resident block count run in further
 // launches. With first_pass_calls > 0 the minimizations run in passes: pass
-// k stops every minimization after first_pass_calls * 4^(k-1) force calls;
-// one stopped by that cap (below its own limit, not converged) restarts from
-// its initial coordinates in the next pass, which shares all the blocks
-// among the networks still running; a pass in which every network gets at
-// least kUncappedShare blocks runs uncapped. A minimization's trajectory does not
-// depend on its cluster size, so the results equal those of one uncapped
-// pass; the passes only move the blocks of finished networks to the slow
-// ones. cs[i] owns network i's tables, neighbor lists and CG buffers; cs[0]
+// k suspends every minimization at the top of its first iteration after
+// first_pass_calls * 4^(k-1) force calls (one below its own limit and not
+// converged), saving its state; the next pass resumes it exactly there, with
+// all the blocks shared among the networks still running; a pass in which
+// every network gets at least kUncappedShare blocks runs uncapped. A
+// minimization's trajectory does not depend on its cluster size, so the
+// results equal those of one uncapped pass; the passes only move the blocks
+// of finished networks to the slow ones. cs[i] owns network i's tables, neighbor lists and CG buffers; cs[0]
 // the launch arrays.
 constexpr int kUncappedShare = 64;   // blocks per network at which a pass runs uncapped
 int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d_x, double *const *d_force_out,
@@ -1049,14 +1088,7 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
     MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, tpb, 0));
     if (per_cu < 1) return MDY_ERR_LAUNCH;
     const int Gmax = per_cu * c0.num_cu;
-    if (first_pass_calls > 0) {   // initial coordinates, for restarts
-        for (int i = 0; i < n; i++) {
-            Ctx &ci = *cs[i];
-            MDY_TRY(ensure(ci.x0, 3 * (size_t)ci.nb));
-            MDY_HIP(hipMemcpyAsync(ci.x0.p, d_x[i], sizeof(double) * 3 * (size_t)ci.nb, hipMemcpyDeviceToDevice,
-                                   st));
-        }
-    }
+    std::vector<char> resume(n, 0);   // suspended in an earlier pass
     hipEvent_t e0, e1;
     MDY_HIP(hipEventCreate(&e0));
     MDY_HIP(hipEventCreate(&e1));
@@ -1104,12 +1136,11 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
             for (int k = 0; k < spare - used && k < m; k++) gsz[frac[k].second]++;
             args.assign(m, CgArgs{});
             map.clear();
-            std::vector<int> limit(m);
             for (int i = 0; i < m && rc == MDY_OK; i++) {
                 const int k = todo[lo + i];
                 rc = prepare_cg(*cs[k], &p[k], d_x[k], gsz[i], st, args[i]);
-                limit[i] = args[i].force_call_limit;
-                if (cap > 0 && cap < (long long)args[i].force_call_limit) args[i].force_call_limit = (int)cap;
+                args[i].resume = resume[k];
+                if (cap > 0 && cap < (long long)args[i].force_call_limit) args[i].suspend_calls = (int)cap;
                 for (int b = 0; b < gsz[i]; b++) map.push_back(make_int2(i, b));
             }
             if (rc != MDY_OK) break;
@@ -1147,23 +1178,13 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
                     break;
                 }
                 if (t_origin < 0 || R[R_T0] < t_origin) t_origin = R[R_T0];
-                collect(ci, R, &p[k], args[i], &res[k]);
-                res[k].force_call_limit = limit[i];
                 ci.coop_blocks = gsz[i];
-                const bool converged = res[k].grad_inf_norm < p[k].grad_tol ||
-                                       res[k].energy_rel_change < p[k].energy_rel_tol;
-                const bool stopped_by_cap = args[i].force_call_limit < limit[i] &&
-                                            res[k].num_force_call > args[i].force_call_limit && !converged &&
-                                            res[k].status == MDY_OK;
-                if (stopped_by_cap) {
-                    if (hipMemcpyAsync(d_x[k], ci.x0.p, sizeof(double) * 3 * (size_t)ci.nb,
-                                       hipMemcpyDeviceToDevice, st) != hipSuccess) {
-                        rc = MDY_ERR_HIP;
-                        break;
-                    }
+                if (R[R_SUSP] != 0.0 && R[R_STATUS] == MDY_OK) {   // resumes in the next pass
+                    resume[k] = 1;
                     next.push_back(k);
                     continue;
                 }
+                collect(ci, R, &p[k], args[i], &res[k]);
                 if (net_ms) net_ms[k] = (R[R_T1] - t_origin) * 1e-5;   // 100 MHz ticks
                 rc = copy_force_out(ci, R, d_force_out ? d_force_out[k] : nullptr, st);
             }

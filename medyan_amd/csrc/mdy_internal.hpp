@@ -112,7 +112,6 @@ struct Ctx {
     DBuf<double> tr_lam, tr_e, tr_mf, tr_beta; DBuf<int> tr_safe;
     long long trace_cap = 0, trace_n = 0;
     DBuf<long long> prof; DBuf<int> prof_kind; long long prof_cap = 0, prof_n = 0;
-    DBuf<double> x0;                 // batched passes: initial coordinates for restarts
     DBuf<unsigned char> batch_args;  // batched launch: per-network kernel arguments
     DBuf<int2> batch_map;            // batched launch: (network, block within its cluster)
     double last_ms = 0;

@@ -356,8 +356,8 @@ def minimize_batch(ctxs, params, d_coords, d_forces=None, threads_per_block=64, 
     its neighbor lists; d_coords[i] its device coordinates (updated in place).
     params: one CGParams / ConjugateGradientParams for all, or one per
     network. weights (optional): block shares. first_pass_calls > 0: passes
-    with force-call caps growing 4x, restarting capped networks on more
-    blocks. Returns (results, per-network completion ms, total ms)."""
+    with force-call caps growing 4x; a capped network is suspended and resumed
+    on more blocks. Returns (results, per-network completion ms, total ms)."""
     n = len(ctxs)
     lib = load_library()
     if not isinstance(params, (list, tuple)):

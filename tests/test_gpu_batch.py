@@ -83,7 +83,7 @@ def test_batch_equals_single_minimizations(lib, precision, tpb, passes, ls):
 @pytest.mark.parametrize("passes", [0, 3])
 def test_batch_larger_than_resident_blocks(lib, passes):
     """More networks than resident blocks: the call runs several launches;
-    with passes, capped networks restart (caps 3, 12, 48, ... force calls)."""
+    with passes, capped networks are suspended and resumed (caps 3, 12, 48, ... force calls)."""
     base = [network.generate(lib, None, seed=100 + i, n_filaments=3, cyl_per_filament=3, box=700.0, n_linkers=2,
                              n_motors=0, n_branches=0) for i in range(4)]
     n = 256 + 8   # above one 256-thread block per CU on 256 CUs
