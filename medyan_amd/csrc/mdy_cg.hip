@@ -1022,6 +1022,15 @@ static int copy_force_out(Ctx &c, const double *R, double *d_force_out, hipStrea
     return MDY_OK;
 }
 
+// two timing events, destroyed on every return path
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_out, mdy_min_result *res) {
     hipStream_t st = c.stream;
     // persistent grid: all blocks resident
@@ -1035,25 +1044,19 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     c.coop_blocks = G;
     CgArgs A;
     MDY_TRY(prepare_cg(c, p, d_x, G, st, A));
-    hipEvent_t e0, e1;
-    MDY_HIP(hipEventCreate(&e0));
-    MDY_HIP(hipEventCreate(&e1));
-    MDY_HIP(hipEventRecord(e0, st));
+    EventPair ev;
+    MDY_HIP(hipEventCreate(&ev.a));
+    MDY_HIP(hipEventCreate(&ev.b));
+    MDY_HIP(hipEventRecord(ev.a, st));
     void *args[] = {&A};
-    hipError_t le = hipLaunchCooperativeKernel(kern, dim3(G), dim3(kBlock), args, 0, st);
-    if (le != hipSuccess) {
-        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-        return MDY_ERR_LAUNCH;
-    }
-    MDY_HIP(hipEventRecord(e1, st));
+    if (hipLaunchCooperativeKernel(kern, dim3(G), dim3(kBlock), args, 0, st) != hipSuccess) return MDY_ERR_LAUNCH;
+    MDY_HIP(hipEventRecord(ev.b, st));
     double R[R_SIZE];
     MDY_HIP(hipMemcpyAsync(R, c.cg_out.p, sizeof(R), hipMemcpyDeviceToHost, st));
     MDY_HIP(hipStreamSynchronize(st));
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventElapsedTime(&ms, ev.a, ev.b);
     c.last_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     collect(c, R, p, A, res);
     MDY_TRY(copy_force_out(c, R, d_force_out, st));
     MDY_HIP(hipStreamSynchronize(st));
@@ -1089,9 +1092,10 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
     if (per_cu < 1) return MDY_ERR_LAUNCH;
     const int Gmax = per_cu * c0.num_cu;
     std::vector<char> resume(n, 0);   // suspended in an earlier pass
-    hipEvent_t e0, e1;
-    MDY_HIP(hipEventCreate(&e0));
-    MDY_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    MDY_HIP(hipEventCreate(&ev.a));
+    MDY_HIP(hipEventCreate(&ev.b));
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     double kernel_ms = 0;
     double t_origin = -1.0;   // device clock at the first launch's start
     std::vector<CgArgs> args;
@@ -1194,8 +1198,6 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
     }
     (void)hipStreamSynchronize(st);
     if (total_ms) *total_ms = kernel_ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return rc;
 }
 
