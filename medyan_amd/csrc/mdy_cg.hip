@@ -1078,6 +1078,7 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
 // of finished networks to the slow ones. cs[i] owns network i's tables, neighbor lists and CG buffers; cs[0]
 // the launch arrays.
 constexpr int kUncappedShare = 64;   // blocks per network at which a pass runs uncapped
+constexpr int kCapGrowth = 4;        // force-call cap growth from one pass to the next (x2 measured slower)
 int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d_x, double *const *d_force_out,
                        mdy_min_result *res, double *net_ms, int threads_per_block, const double *weights,
                        int first_pass_calls, hipStream_t st, double *total_ms) {
@@ -1194,7 +1195,7 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
             }
         }
         todo.swap(next);
-        if (cap > 0) cap *= 4;
+        if (cap > 0) cap *= kCapGrowth;
     }
     (void)hipStreamSynchronize(st);
     if (total_ms) *total_ms = kernel_ms;
