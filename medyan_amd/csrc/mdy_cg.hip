@@ -140,7 +140,7 @@ struct Cg {
             long long spins = 0;
             while (__hip_atomic_load(flags + (size_t)b * 16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
                 __builtin_amdgcn_s_sleep(1);
-                if ((++spins & 255) == 0) {
+                if (__builtin_expect((++spins & 255) == 0, 0)) {
                     if (__hip_atomic_load(A.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ab = 1; break; }
                     if (spins > A.spin_limit) {
                         __hip_atomic_store(A.bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -282,11 +282,11 @@ struct Cg {
         const int nqe = (int)accl[17];
         ev.nqf = (int)accl[18];
         ev.qb = qb;
-        if (nqe > kQueueCap || ev.nqf > kQueueCap) {
+        if (__builtin_expect(nqe > kQueueCap || ev.nqf > kQueueCap, 0)) {
             status = MDY_ERR_CAPACITY;
             return true;
         }
-        if (nqe > 0 && !energy_fallback(pos, d, nqe, qb)) return false;
+        if (__builtin_expect(nqe > 0, 0) && !energy_fallback(pos, d, nqe, qb)) return false;
         double e[kNFF + 2];
         energy_finish(X, nullptr, nqe, d.qe, qeu(qb), e);
         ev.E = e[kNFF];
@@ -318,7 +318,7 @@ struct Cg {
             status = MDY_ERR_CAPACITY;
             return true;
         }
-        if (nqe > 0 && !energy_fallback(pos, d, nqe, qb)) return false;
+        if (__builtin_expect(nqe > 0, 0) && !energy_fallback(pos, d, nqe, qb)) return false;
         double v[kNFF], e[kNFF + 2];
         grid_partials_reduce<kNFF>(r, G, v, lds);
         energy_finish(X, v, nqe, d.qe, qeu(qb), e);
@@ -574,7 +574,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             break;
         }
         case OP_GATHER:
-            if (ev.nqf) ok = K.force_fallback(P, ev);
+            if (__builtin_expect(ev.nqf != 0, 0)) ok = K.force_fallback(P, ev);
             if (ok && !K.status) ok = K.gather(gOut, gPrev, gMove, P.d, P.lam, maxF, FF, FFp);
             break;
         case OP_MINSTEP: {
@@ -598,9 +598,9 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         default:
             break;
         }
-        if (!ok) goto aborted;
+        if (__builtin_expect(!ok, 0)) goto aborted;
         op = OP_NONE;
-        if (K.status && stage != ST_FINISH) { after = false; stage = ST_FINISH; }
+        if (__builtin_expect(K.status != 0, 0) && stage != ST_FINISH) { after = false; stage = ST_FINISH; }
 
         // ---- control
         switch (stage) {
@@ -615,7 +615,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         case ST_INIT_G:
             nFC++;
             gradInf = maxF;
-            if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
+            if (__builtin_expect(!isfinite(gradInf), 0)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
             FFcur = FF;
             dvec = FB(cur); lastFDotF = FF; slope = FF;   // resetCGParams
             stage = ST_ITER;
@@ -797,7 +797,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         case ST_REC_GATHER:
             nFC++; nRec++;
             gradInf = maxF;
-            if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
+            if (__builtin_expect(!isfinite(gradInf), 0)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
             cur = 1 - cur;
             FFcur = FF;
             dvec = FB(cur); lastFDotF = FF; slope = FF;
@@ -809,7 +809,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         case ST_ACCEPT: {
             nFC++;
             gradInf = maxF;
-            if (!isfinite(gradInf)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
+            if (__builtin_expect(!isfinite(gradInf), 0)) { K.status = MDY_ERR_NONFINITE; after = false; stage = ST_FINISH; break; }
             E0 = quadLS ? Eacc : newE;   // E(x) of the next line search
             relE = fabs(newE - prevEnergy) / fabs(prevEnergy);
             prevEnergy = newE;
