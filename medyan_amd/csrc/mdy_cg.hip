@@ -421,12 +421,9 @@ struct Cg {
         block_reduce<1>(m, lds, 0x1u);
         unsigned long long *a = accs(p1);
         if (tid == 0) atomicMax(a + kAccMax, (unsigned long long)__double_as_longlong(m[0]));
-#pragma clang loop unroll(disable)
-        for (int k = 0; k < 2; k++) {   // F.F, then F.Fprev (one inlined copy of the reduction)
-            mdy_xsum &t = k == 0 ? sff : sfp;
-            block_reduce_xsum(t, ldsl());
-            acc_push(a + (k == 0 ? kAcc0 : kAcc1), t);
-        }
+        block_reduce_xsum2(sff, sfp, ldsl());   // F.F and F.Fprev together
+        acc_push(a + kAcc0, sff);
+        acc_push(a + kAcc1, sfp);
         if (!sync(PK_GATHER)) return false;
         acc_fetch(a, nullptr, nullptr);
         maxF = __longlong_as_double((long long)accl[kAccMax]);

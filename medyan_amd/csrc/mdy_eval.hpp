@@ -466,6 +466,35 @@ __device__ __forceinline__ void block_reduce_xsum(mdy_xsum &s, long long *lds) {
     mdy_xs_norm(&s);
     __syncthreads();
 }
+// Two block sums in one pass (one set of barriers, interleaved shuffles).
+// lds: >= 17 * 16 int64.
+__device__ __forceinline__ void block_reduce_xsum2(mdy_xsum &s, mdy_xsum &t, long long *lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    mdy_xs_norm(&s);
+    mdy_xs_norm(&t);
+#pragma unroll
+    for (int k = 0; k < MDY_XS_N; k++) {
+        const long long r = wave_sum_i64(s.d[k]), q = wave_sum_i64(t.d[k]);
+        if (lane == 0) { lds[wave * 16 + k] = r; lds[wave * 16 + 8 + k] = q; }
+    }
+    const int f = wave_or(s.flags), g = wave_or(t.flags);
+    if (lane == 0) { lds[wave * 16 + 7] = f; lds[wave * 16 + 15] = g; }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        const int k = threadIdx.x;
+        long long r = lds[k];
+        for (int w = 1; w < nw; w++) r = ((k & 7) == 7) ? (r | lds[w * 16 + k]) : r + lds[w * 16 + k];
+        lds[256 + k] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MDY_XS_N; k++) { s.d[k] = lds[256 + k]; t.d[k] = lds[256 + 8 + k]; }
+    s.flags = (int)lds[256 + 7];
+    t.flags = (int)lds[256 + 15];
+    mdy_xs_norm(&s);
+    mdy_xs_norm(&t);
+    __syncthreads();
+}
 // per-block partial records: [blk * kRed + off .. off+7] hold a normalized accumulator
 __device__ __forceinline__ void store_xsum(double *red, int blk, int off, const mdy_xsum &s) {
     long long *r = reinterpret_cast<long long *>(red) + (size_t)blk * kRed + off;
