@@ -258,10 +258,11 @@ struct Cg {
     }
 
     // ---- hot loop: all interactions at `pos`, energy (exact total) and force
-    // slots. with_sd: the caller has pushed an s.F block total into this
-    // phase's second accumulator; its value lands in ev.xa. Returns false on
-    // abort; a queue overflow sets `status`.
-    __attribute__((always_inline)) __device__ bool eval(const PosAny &pos, Eval &ev, bool with_sd) {
+    // slots. sd: this thread's share of s.F (zero without a new direction),
+    // reduced together with the energy into the phase's second accumulator;
+    // with_sd: its value lands in ev.xa. Returns false on abort; a queue
+    // overflow sets `status`.
+    __attribute__((always_inline)) __device__ bool eval(const PosAny &pos, Eval &ev, bool with_sd, mdy_xsum &sd) {
         const int p0 = ph, qb = p0 % 3;
         Dev d = dv(qb);
         mdy_xsum xs;
@@ -270,8 +271,9 @@ struct Cg {
         mark(0);
         sweep<MIXED>(d, pos, gtid, gs, nullptr, xs, slots(), false, true);
         mark(1);
-        block_reduce_xsum(xs, ldsl());
+        block_reduce_xsum2(xs, sd, ldsl());
         acc_push(accs(p0) + kAcc0, xs);
+        acc_push(accs(p0) + kAcc1, sd);
         if (!sync(PK_EVAL)) return false;
         acc_fetch(accs(p0), d.qe_n, d.qf_n);
         mdy_xsum X = acc_local(kAcc0);
@@ -439,13 +441,11 @@ struct Cg {
     }
 };
 
-// s = F + beta * d for every bead into Sn, and this block's share of s.F,
-// pushed into the current phase's second accumulator
+// s = F + beta * d for every bead into Sn, and this thread's share of s.F
+// into sd (reduced with the phase's energy, or on its own without a trial)
 template <bool MIXED>
 __device__ __attribute__((always_inline)) void spec_direction(Cg<MIXED> &K, const double *Fn, const double *dprev,
-                                                              double beta, double *Sn) {
-    mdy_xsum sd;
-    mdy_xs_zero(&sd);
+                                                              double beta, double *Sn, mdy_xsum &sd) {
     for (int b = K.gtid; b < K.A.D.nb; b += K.gs) {
         const int o = 3 * b;
         for (int k = 0; k < 3; k++) {
@@ -454,8 +454,6 @@ __device__ __attribute__((always_inline)) void spec_direction(Cg<MIXED> &K, cons
             mdy_xs_add(&sd, s * Fn[o + k]);
         }
     }
-    block_reduce_xsum(sd, K.ldsl());
-    K.acc_push(K.accs(K.ph) + kAcc1, sd);
 }
 
 // The minimizer as a state machine: every loop turn performs at most one kind
@@ -560,11 +558,15 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         case OP_EVAL:
         case OP_SDONLY: {
             const int p1 = K.ph;
-            if (P.mode == PosAny::SPEC) spec_direction(K, P.F, P.d, P.beta, Snew);
+            mdy_xsum sd;
+            mdy_xs_zero(&sd);
+            if (P.mode == PosAny::SPEC) spec_direction(K, P.F, P.d, P.beta, Snew, sd);
             if (op == OP_EVAL) {
-                ok = K.eval(P, ev, P.mode == PosAny::SPEC);
+                ok = K.eval(P, ev, P.mode == PosAny::SPEC, sd);
                 nEE++;
             } else {   // s.F only (max|F| = 0: no trial)
+                block_reduce_xsum(sd, K.ldsl());
+                K.acc_push(K.accs(p1) + kAcc1, sd);
                 ok = K.sync(PK_REDUCE);
                 if (ok) sdF = K.acc_value(K.accs(p1) + kAcc1);
             }
