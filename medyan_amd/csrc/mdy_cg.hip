@@ -602,6 +602,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         if (__builtin_expect(K.status != 0, 0) && stage != ST_FINISH) { after = false; stage = ST_FINISH; }
 
         // ---- control
+    control:
         switch (stage) {
         case ST_INIT_FF:   // energiesBefore = computeIndividualEnergies(x); E(x) for the first line search
             nEC++; nEE++;
@@ -715,7 +716,13 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             if (lam <= 0.0 || lam <= 1e-8) { done = true; lam = 0.0; Et = E0; }
             if (!done) { op = OP_EVAL; P = PosAny::line(A.x, dvec, lam); break; }
             newE = Et;
-            stage = ST_LS_DONE;
+            if (lam != 0.0) {   // accepted: gather at once (ST_LS_DONE's accepting branch)
+                op = OP_GATHER; P = PosAny::line(A.x, dvec, lam);
+                gOut = FB(1 - cur); gPrev = FB(cur); gMove = true;
+                stage = ST_ACCEPT;
+            } else {
+                stage = ST_LS_DONE;
+            }
             break;
         }
         case ST_QUAD: {   // quadraticoptimization, CGMethod.hpp:663-765 (at most 200 refinements)
@@ -829,7 +836,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             if (A.method != MDY_CG_STEEPEST) lastFDotF = newGrad;
             trace(lam, newE, beta);
             stage = ST_ITER;
-            break;
+            goto control;   // no device work in between
         }
         case ST_FINAL:
             if (after) { op = OP_EVAL_FF; P = PosAny::at(A.x); ffout = R + R_EA; stage = ST_FINAL_DONE; }
