@@ -127,7 +127,13 @@ typedef struct mdy_system {
     int32_t exvol_enabled;
     int32_t boundary_enabled;
     int32_t exvol_type;        /* MDY_EXVOL_*: cylinderVolumeExclusionFFType */
+    int32_t br_dihedral_type;  /* MDY_BRDIH_*: BRDIHEDRALFFTYPE */
 } mdy_system;
+/* Branching dihedral form (BranchingFF.h:64-81): COSINE,
+ * kdih (1 - cos theta) (BranchingDihedralCosine.cpp:211-661), or QUADRATIC,
+ * kdih theta^2 (BranchingDihedralQuadratic.cpp:26-398). Both occupy the
+ * MDY_FF_BDH slot and the same table (BranchingDihedral::vectorize). */
+enum { MDY_BRDIH_COSINE = 0, MDY_BRDIH_QUADRATIC = 1 };
 /* Cylinder excluded volume form (MController::initializeFF,
  * Controller/MController.cpp:128-153): the analytic integral
  * (CylinderExclVolume<CylinderExclVolRepulsion>) or the monomer-pair sum

@@ -27,6 +27,7 @@ MDY_ERR_TIMEOUT = -10
 MDY_ERR_QUADRATIC = -11
 MDY_LS_BACKTRACKING, MDY_LS_QUADRATIC = 0, 1
 MDY_EXVOL_INTEGRAL, MDY_EXVOL_MONOMER = 0, 1
+MDY_BRDIH_COSINE, MDY_BRDIH_QUADRATIC = 0, 1
 
 FF_NAMES = ["FilamentStretching", "FilamentBending", "LinkerStretching", "MotorGhostStretching",
             "BranchingStretching", "BranchingBending", "BranchingDihedral", "BranchingPosition",
@@ -79,7 +80,8 @@ class Boundary(C.Structure):
 class System(C.Structure):
     _fields_ = [("n_beads", C.c_int32), ("cyl", Cylinders), ("ff", FFTables), ("bnd", Boundary),
                 ("exvol_cutoff", C.c_float), ("exvol_enabled", C.c_int32),
-                ("boundary_enabled", C.c_int32), ("exvol_type", C.c_int32)]
+                ("boundary_enabled", C.c_int32), ("exvol_type", C.c_int32),
+                ("br_dihedral_type", C.c_int32)]
 
 
 class CGParams(C.Structure):

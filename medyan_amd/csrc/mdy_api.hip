@@ -21,6 +21,7 @@ Dev Ctx::dev() const {
     D.n_bd = have_nl ? n_bd : 0;
     D.ev_bs = ev_bs.p; D.ev_kr = ev_kr.p;
     D.ev_mon = ev_mon; D.ev_cyl = ev_cyl.p; D.cyl_mon = c_mon.p;
+    D.bdh_quad = bdh_quad;
     D.fs_bs = fs_bs.p; D.fs_k = fs_k.p; D.fs_l0 = fs_l0.p;
     D.fb_bs = fb_bs.p; D.fb_k = fb_k.p; D.fb_th = fb_th.p; D.fb_ct = fb_ct.p; D.fb_st = fb_st.p;
     D.lm_bs = lm_bs.p; D.lm_k = lm_k.p; D.lm_l0 = lm_l0.p; D.lm_p1 = lm_p1.p; D.lm_p2 = lm_p2.p;
@@ -187,6 +188,8 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
         return fail(ctx, MDY_ERR_ARG, "incomplete boundary table");
     if (s->exvol_type != MDY_EXVOL_INTEGRAL && s->exvol_type != MDY_EXVOL_MONOMER)
         return fail(ctx, MDY_ERR_ARG, "exvol_type");
+    if (s->br_dihedral_type != MDY_BRDIH_COSINE && s->br_dihedral_type != MDY_BRDIH_QUADRATIC)
+        return fail(ctx, MDY_ERR_ARG, "br_dihedral_type");
     const bool mon = s->exvol_enabled && s->exvol_type == MDY_EXVOL_MONOMER && cy.n > 0;
     if (mon) {
         if (!cy.mon_minus || !cy.mon_plus || !cy.mon_size || !cy.mon_interval)
@@ -305,6 +308,7 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
         for (const auto &kv : last) plus[kv.second] = 1;
         rc |= upload(c.c_plus, plus.data(), cy.n, st);
         c.ev_mon = mon ? 1 : 0;
+        c.bdh_quad = s->br_dihedral_type == MDY_BRDIH_QUADRATIC ? 1 : 0;
         if (mon) {
             // segEqLen = monInterval * monEqLen (CylinderVolumeMon.hpp:41-43)
             std::vector<int4> m(cy.n);

@@ -49,8 +49,12 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(Dev D, const double *
             if (j < D.n_bdh) {
                 const int4 b = D.bdh_bs[j];
                 d3 f1, f2, f3, f4;
-                bdh_force(ld3(x + b.x), ld3(x + b.y), ld3(x + b.z), ld3(x + b.w), D.bdh_k[j], D.bdh_pos[j], f1, f2, f3,
-                          f4);
+                if (D.bdh_quad)
+                    bdhq_force(ld3(x + b.x), ld3(x + b.y), ld3(x + b.z), ld3(x + b.w), D.bdh_k[j], D.bdh_pos[j], f1, f2,
+                               f3, f4);
+                else
+                    bdh_force(ld3(x + b.x), ld3(x + b.y), ld3(x + b.z), ld3(x + b.w), D.bdh_k[j], D.bdh_pos[j], f1, f2,
+                              f3, f4);
                 acc = d3{acc.x + f3.x, acc.y + f3.y, acc.z + f3.z};
             }
             if (j < D.n_bp) {

@@ -71,6 +71,12 @@ __device__ __forceinline__ double xfabs(double a) { return fabs(a); }
 __device__ __forceinline__ Dual xfabs(Dual a) { return a.v < 0 ? -a : a; }
 __device__ __forceinline__ double xatan(double a) { return mdy_atan(a); }
 __device__ __forceinline__ Dual xatan(Dual a) { return Dual(mdy_atan(a.v), a.d / (1 + a.v * a.v)); }
+__device__ __forceinline__ double xacos(double a) { return mdy_acos(a); }
+// d acos(v) = -dv / sqrt(1 - v^2); a constant argument (a clamped cosine) has
+// no derivative, also at v = +-1
+__device__ __forceinline__ Dual xacos(Dual a) {
+    return Dual(mdy_acos(a.v), a.d == 0.0 ? 0.0 : -a.d / sqrt((1 - a.v) * (1 + a.v)));
+}
 __device__ __forceinline__ double xexp(double a) { return mdy_exp(a); }
 __device__ __forceinline__ Dual xexp(Dual a) {
     const double e = mdy_exp(a.v);
@@ -396,6 +402,96 @@ __device__ __forceinline__ void bdh_force(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4
         const V3<T> lmp{s2.x - l2.x, s2.y - l2.y, s2.z - l2.z};
         f1 = V3<T>{(1 - position) * lmp.x, (1 - position) * lmp.y, (1 - position) * lmp.z};
         f2 = V3<T>{l2.x + (position)*lmp.x, l2.y + (position)*lmp.y, l2.z + (position)*lmp.z};
+    }
+}
+
+// BranchingDihedralQuadratic (BranchingDihedralQuadratic.cpp:26-86, 88-398):
+// E = kdih theta^2 with theta the dihedral between the planes (c2, mp, c3)
+// and (mp, c3, c4). The energy takes cos(theta) from the plane normals
+// b1 x b2 and b3 x b2; the forces take it from the bond angles (sines held
+// at >= 1e-3) and differentiate E through it: dE = eFac d(cos theta),
+// eFac = -2 kdih theta / max(sin theta, 1e-5). sin(theta) is formed as
+// sqrt((1 - c)(1 + c)), c = cos(theta) clamped to [-1, 1].
+template <class T>
+__device__ __forceinline__ T bdhq_energy(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double kdih, double pos) {
+    double position = pos;
+    if (are_equal1(position)) {
+        position = 0.5;
+        c2 = branch_plus_end(c1, c2, position);
+    }
+    const V3<T> mp = midpt(c1, c2, position);
+    const V3<T> b1 = sub(c2, mp), b2 = sub(c3, mp), b3 = sub(c4, c3);
+    const V3<T> n1 = cross3(b1, b2), n2 = cross3(b3, b2);
+    const T n1mag = xsqrt(n1.x * n1.x + n1.y * n1.y + n1.z * n1.z);
+    const T n2mag = xsqrt(n2.x * n2.x + n2.y * n2.y + n2.z * n2.z);
+    T ct = (n1.x * n2.x + n1.y * n2.y + n1.z * n2.z) / (n1mag * n2mag);
+    if (ct < -1.0) ct = T(-1.0);
+    if (ct > 1.0) ct = T(1.0);
+    const T th = xacos(ct);
+    return kdih * th * th;
+}
+template <class T>
+__device__ __forceinline__ void bdhq_force(V3<T> c1, V3<T> c2, V3<T> c3, V3<T> c4, double kdih, double pos, V3<T> &f1,
+                                           V3<T> &f2, V3<T> &f3, V3<T> &f4) {
+    double p = pos;
+    if (are_equal1(p)) {
+        p = 0.5;
+        c2 = branch_plus_end(c1, c2, p);
+    }
+    const V3<T> mp = midpt(c1, c2, p);
+    const V3<T> b1 = sub(c2, mp), b2 = sub(c3, mp), b3 = sub(c4, c3);
+    const T l1sq = b1.x * b1.x + b1.y * b1.y + b1.z * b1.z;
+    const T l2sq = b2.x * b2.x + b2.y * b2.y + b2.z * b2.z;
+    const T l3sq = b3.x * b3.x + b3.y * b3.y + b3.z * b3.z;
+    const T l1 = xsqrt(l1sq), l2 = xsqrt(l2sq), l3 = xsqrt(l3sq);
+    const T i1sq = 1.0 / l1sq, i2sq = 1.0 / l2sq, i3sq = 1.0 / l3sq;
+    const T i13 = 1.0 / (l1 * l3), i12 = 1.0 / (l1 * l2), i32 = 1.0 / (l3 * l2);
+    const T d13 = b1.x * b3.x + b1.y * b3.y + b1.z * b3.z;
+    const T d12 = b1.x * b2.x + b1.y * b2.y + b1.z * b2.z;
+    const T d32 = b3.x * b2.x + b3.y * b2.y + b3.z * b2.z;
+    // bond angles <b1, b2> and <b3, b2>
+    const T c12 = d12 * i12;
+    const T s12 = smax(xsqrt(smax(T(1 - c12 * c12), T(0.0))), T(0.001));
+    const T is12 = 1.0 / s12;
+    const T is12sq = is12 * is12;
+    const T c32 = d32 * i32;
+    const T s32 = smax(xsqrt(smax(T(1 - c32 * c32), T(0.0))), T(0.001));
+    const T is32 = 1.0 / s32;
+    const T is32sq = is32 * is32;
+    const T c13 = d13 * i13;
+    // cos(theta) = (cos<b1,b3> - cos<b1,b2> cos<b3,b2>) / (sin<b1,b2> sin<b3,b2>)
+    const T num = c13 - c12 * c32;
+    const T iden = is12 * is32;
+    T ct = num * iden;
+    if (ct < -1.0) ct = T(-1.0);
+    if (ct > 1.0) ct = T(1.0);
+    const T th = xacos(ct);
+    const T st = smax(xsqrt((1 - ct) * (1 + ct)), T(1e-5));
+    const T eFac = -2 * kdih * th * (1.0 / st);
+    // dE/db_i = sum_j e_ij b_j (e symmetric)
+    const T e11 = -eFac * ct * is12sq * i1sq;
+    const T e12 = eFac * i12 * (-c32 * iden + c12 * ct * is12sq);
+    const T e13 = eFac * i13 * iden;
+    const T e22 = eFac * i2sq * (2 * c13 * iden - ct * (is12sq + is32sq));
+    const T e23 = eFac * i32 * (-c12 * iden + c32 * ct * is32sq);
+    const T e33 = -eFac * ct * is32sq * i3sq;
+    // b1 = c2 - mp, b2 = c3 - mp, b3 = c4 - c3, mp = (1-p) c1 + p c2
+    const T a1 = (1 - p) * (e11 + e12), a2 = (1 - p) * (e12 + e22), a3 = (1 - p) * (e13 + e23);
+    const T g1 = (p - 1) * e11 + p * e12, g2 = (p - 1) * e12 + p * e22, g3 = (p - 1) * e13 + p * e23;
+    const T h1 = -e12 + e13, h2 = -e22 + e23, h3 = -e23 + e33;
+    auto comb = [&](T u, T v, T w) {
+        return V3<T>{u * b1.x + v * b2.x + w * b3.x, u * b1.y + v * b2.y + w * b3.y, u * b1.z + v * b2.z + w * b3.z};
+    };
+    const V3<T> fa = comb(a1, a2, a3), fg = comb(g1, g2, g3);
+    f3 = comb(h1, h2, h3);
+    f4 = comb(-e13, -e23, -e33);
+    if (are_equal1(pos)) {   // the energy is a function of (c1, c2', c3, c4), c2' the extended plus end
+        const double factor1 = (p - 1) / p, factor2 = 1 / p;
+        f1 = V3<T>{fa.x + fg.x * factor1, fa.y + fg.y * factor1, fa.z + fg.z * factor1};
+        f2 = V3<T>{fg.x * factor2, fg.y * factor2, fg.z * factor2};
+    } else {
+        f1 = fa;
+        f2 = fg;
     }
 }
 

@@ -279,10 +279,11 @@ __device__ __forceinline__ void item_compute(const Dev &D, const ItemRec &r, con
         }
     } else if (r.type == IT_BDH) {
         ff = MDY_FF_BDH;
-        U = bdh_energy(c[0], c[1], c[2], c[3], r.p0, r.p1);
+        U = D.bdh_quad ? bdhq_energy(c[0], c[1], c[2], c[3], r.p0, r.p1) : bdh_energy(c[0], c[1], c[2], c[3], r.p0, r.p1);
         if (FORCES) {
             d3 f1, f2, f3, f4;
-            bdh_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f1, f2, f3, f4);
+            if (D.bdh_quad) bdhq_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f1, f2, f3, f4);
+            else bdh_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f1, f2, f3, f4);
             int s = D.slot_bdh + 4 * j;
             put3(contrib, s, f1); put3(contrib, s + 1, f2); put3(contrib, s + 2, f3); put3(contrib, s + 3, f4);
         }

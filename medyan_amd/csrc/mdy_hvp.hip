@@ -42,7 +42,8 @@ __device__ __forceinline__ int item_forces(const Dev &D, const ItemRec &r, const
         bb_force(c[0], c[1], c[2], c[3], r.p0, r.p1, r.p2, r.p3, f[0], f[1], f[2], f[3]);
         return MDY_FF_BB;
     } else if (r.type == IT_BDH) {
-        bdh_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f[0], f[1], f[2], f[3]);
+        if (D.bdh_quad) bdhq_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f[0], f[1], f[2], f[3]);
+        else bdh_force(c[0], c[1], c[2], c[3], r.p0, r.p1, f[0], f[1], f[2], f[3]);
         return MDY_FF_BDH;
     } else if (r.type == IT_BP) {
         bp_force(c[0], c[1], c[2], r.p0, r.p1, D.cq, D.sq, f[0], f[1], f[2]);

@@ -26,6 +26,7 @@ struct Dev {
     int n_ev, n_fs, n_fb, n_lm, n_ls, n_bs, n_bb, n_bdh, n_bp, n_bd;
     const int4 *ev_bs; const double *ev_kr;   // monomer-sum form: ev_kr holds kvol * seg1 * seg2
     int ev_mon;                               // excluded volume as the monomer-pair sum
+    int bdh_quad;                             // branching dihedral E = k theta^2 (else k (1 - cos theta))
     const int2 *ev_cyl;                       // monomer-sum form: the pair's cylinders
     const int4 *cyl_mon;                      // per cylinder: minus serial, plus serial, interval
     const int2 *fs_bs; const double *fs_k, *fs_l0;
@@ -80,6 +81,7 @@ struct Ctx {
     DBuf<int> c_b1, c_b2, c_fil, c_pos, c_child; DBuf<long long> c_id; DBuf<short> c_type;
     DBuf<double> c_eq, c_ek; DBuf<unsigned char> c_minus, c_plus;   // c_plus: last cylinder of its filament
     int ev_mon = 0;                // monomer-sum excluded volume
+    int bdh_quad = 0;              // quadratic branching dihedral
     DBuf<int4> c_mon;              // minus serial, plus serial, interval, 0
     DBuf<double> c_seg;            // interval * monomer size
     float ev_cut = 0, bd_cut = 0;

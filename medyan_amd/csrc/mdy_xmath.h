@@ -8,6 +8,7 @@
  *     gen_xmath_tables.py in 100-digit decimal arithmetic. Built from IEEE
  *     +,-,*,/ only (no fused multiply-add), so host and device agree exactly.
  *     Accuracy ~1 ulp (pinned against the C library in the tests).
+ *     mdy_acos is built on mdy_atan (half-angle identity).
  *
  *   mdy_xsum : order-independent exact summation of doubles. Every term is
  *     placed on a fixed-point grid of unit 2^-144 (terms are truncated toward
@@ -91,6 +92,15 @@ MDY_HD double mdy_atan(double x) {
     r = inv ? ri : r;
     r = x < 0 ? -r : r;
     return (x != x) ? x : r;
+}
+
+/* acos from the half-angle identity acos(x) = 2 atan(sqrt((1 - x) / (1 + x))):
+ * 1 - x (x >= 1/2) and 1 + x (x <= -1/2) are exact, so the quotient carries
+ * at most ~1.5 ulp, halved by the square root, into an atan argument of the
+ * same relative error; x = -1 gives 2 atan(inf) = pi. Accuracy ~2 ulp
+ * (pinned against the C library in the tests). */
+MDY_HD double mdy_acos(double x) {
+    return 2.0 * mdy_atan(sqrt((1.0 - x) / (1.0 + x)));
 }
 
 /* ------------------------------------------------------------------------ */

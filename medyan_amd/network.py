@@ -88,6 +88,8 @@ class Network:
     # monomer-sum excluded volume (exvol_type MDY_EXVOL_MONOMER): per-cylinder
     # monomer serials of its beads, monomer size and sampling interval
     exvol_type: int = _abi.MDY_EXVOL_INTEGRAL
+    # branching dihedral form (BRDIHEDRALFFTYPE): MDY_BRDIH_COSINE or MDY_BRDIH_QUADRATIC
+    br_dihedral_type: int = _abi.MDY_BRDIH_COSINE
     cyl_mon_minus: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     cyl_mon_plus: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     cyl_mon_size: np.ndarray = field(default_factory=lambda: np.zeros(0))
@@ -159,6 +161,7 @@ class Network:
         s.exvol_enabled = int(self.exvol_enabled)
         s.boundary_enabled = int(self.boundary_enabled)
         s.exvol_type = int(self.exvol_type)
+        s.br_dihedral_type = int(self.br_dihedral_type)
         self._keep = keep
         return s
 

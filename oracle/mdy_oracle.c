@@ -35,6 +35,10 @@ void orc_set_exact(int on) { g_exact = on; }
 int orc_get_exact(void) { return g_exact; }
 static inline double o_atan(double x) { return g_exact ? mdy_atan(x) : atan(x); }
 static inline double o_exp(double x) { return g_exact ? mdy_exp(x) : exp(x); }
+static inline double o_acos(double x) { return g_exact ? mdy_acos(x) : acos(x); }
+/* sin(acos(c)) for c in [-1, 1]: the C library's sin of the angle in
+ * reference mode (as the reference), sqrt((1 - c)(1 + c)) in exact mode */
+static inline double o_sin_of(double c, double theta) { return g_exact ? sqrt((1 - c) * (1 + c)) : sin(theta); }
 
 /* sum helper: sequential in reference mode, exact in exact mode. In exact
  * mode every interaction energy is also added to g_total, so the manager's
@@ -511,6 +515,109 @@ void orc_bdh_forces(const double *x, double *f, int32_t n, const int32_t *bs, co
         for (int d = 0; d < 3; d++) {
             f1[d] += L1[d]; f2[d] += L2[d]; f3[d] += L3[d]; f4[d] += L4[d];
             if (stretch3) stretch3[3 * i + d] = L3[d];
+        }
+    }
+}
+
+/* BranchingDihedralQuadratic (BranchingDihedralQuadratic.cpp:26-86 energy,
+ * 88-398 forces): E = kdih theta^2. The energy takes cos(theta) from the
+ * normals n1 = b1 x b2, n2 = b3 x b2 (b1 = c2 - mp, b2 = c3 - mp,
+ * b3 = c4 - c3); the forces take it from the bond angles,
+ * (cos<b1,b3> - cos<b1,b2> cos<b3,b2>) / (sin<b1,b2> sin<b3,b2>) with the
+ * sines held at >= 1e-3, and use dE = eFac d(cos theta),
+ * eFac = -2 kdih theta / max(sin theta, 1e-5). */
+double orc_bdhq_energy(const double *x, int32_t n, const int32_t *bs, const double *kdih, const double *pos) {
+    osum U;
+    os_zero(&U);
+    for (int32_t i = 0; i < n; i++) {
+        const double *c1 = x + bs[4 * i], *c2 = x + bs[4 * i + 1], *c3 = x + bs[4 * i + 2], *c4 = x + bs[4 * i + 3];
+        double c2p[3], mp[3], b1[3], b2[3], b3[3], n1[3], n2[3];
+        double p = pos[i];
+        if (are_equal1(p)) {
+            p = 0.5;
+            extend_plus(c2p, c1, c2, p);
+            c2 = c2p;
+        }
+        midpt(mp, c1, c2, p);
+        for (int d = 0; d < 3; d++) { b1[d] = c2[d] - mp[d]; b2[d] = c3[d] - mp[d]; b3[d] = c4[d] - c3[d]; }
+        cross3(n1, b1, b2);
+        cross3(n2, b3, b2);
+        double n1mag = sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+        double n2mag = sqrt(n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2]);
+        double ct = (n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2]) / (n1mag * n2mag);
+        if (ct < -1.0) ct = -1.0;
+        if (ct > 1.0) ct = 1.0;
+        double th = o_acos(ct);
+        os_add(&U, kdih[i] * th * th);
+    }
+    return os_val(&U);
+}
+static inline double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
+void orc_bdhq_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kdih, const double *pos,
+                     double *stretch3) {
+    for (int32_t i = 0; i < n; i++) {
+        double c1[3], c2[3], c3[3], c4[3], mp[3], b1[3], b2[3], b3[3];
+        for (int d = 0; d < 3; d++) {
+            c1[d] = x[bs[4 * i] + d]; c2[d] = x[bs[4 * i + 1] + d];
+            c3[d] = x[bs[4 * i + 2] + d]; c4[d] = x[bs[4 * i + 3] + d];
+        }
+        double p = pos[i];
+        if (are_equal1(p)) {
+            p = 0.5;
+            for (int d = 0; d < 3; d++) c2[d] = (1 / p) * (c2[d] - (1 - p) * c1[d]);
+        }
+        midpt(mp, c1, c2, p);
+        for (int d = 0; d < 3; d++) { b1[d] = c2[d] - mp[d]; b2[d] = c3[d] - mp[d]; b3[d] = c4[d] - c3[d]; }
+        const double l1sq = b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2];
+        const double l2sq = b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2];
+        const double l3sq = b3[0] * b3[0] + b3[1] * b3[1] + b3[2] * b3[2];
+        const double l1 = sqrt(l1sq), l2 = sqrt(l2sq), l3 = sqrt(l3sq);
+        const double i1sq = 1.0 / l1sq, i2sq = 1.0 / l2sq, i3sq = 1.0 / l3sq;
+        const double i13 = 1.0 / (l1 * l3), i12 = 1.0 / (l1 * l2), i32 = 1.0 / (l3 * l2);
+        const double d13 = b1[0] * b3[0] + b1[1] * b3[1] + b1[2] * b3[2];
+        const double d12 = b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2];
+        const double d32 = b3[0] * b2[0] + b3[1] * b2[1] + b3[2] * b2[2];
+        const double c12 = d12 * i12;
+        const double s12 = dmax(sqrt(dmax(1 - c12 * c12, 0.0)), 0.001);
+        const double is12 = 1.0 / s12, is12sq = is12 * is12;
+        const double c32 = d32 * i32;
+        const double s32 = dmax(sqrt(dmax(1 - c32 * c32, 0.0)), 0.001);
+        const double is32 = 1.0 / s32, is32sq = is32 * is32;
+        const double c13 = d13 * i13;
+        const double num = c13 - c12 * c32, iden = is12 * is32;
+        double ct = num * iden;
+        if (ct < -1.0) ct = -1.0;
+        if (ct > 1.0) ct = 1.0;
+        const double th = o_acos(ct);
+        const double st = dmax(o_sin_of(ct, th), 1e-5);
+        const double eFac = -2 * kdih[i] * th * (1.0 / st);
+        /* dE/db_i = sum_j e_ij b_j */
+        const double e11 = -eFac * ct * is12sq * i1sq;
+        const double e12 = eFac * i12 * (-c32 * iden + c12 * ct * is12sq);
+        const double e13 = eFac * i13 * iden;
+        const double e22 = eFac * i2sq * (2 * c13 * iden - ct * (is12sq + is32sq));
+        const double e23 = eFac * i32 * (-c12 * iden + c32 * ct * is32sq);
+        const double e33 = -eFac * ct * is32sq * i3sq;
+        /* chain rule through b1 = c2 - mp, b2 = c3 - mp, b3 = c4 - c3, mp = (1-p) c1 + p c2 */
+        const double w[4][3] = {
+            {(1 - p) * (e11 + e12), (1 - p) * (e12 + e22), (1 - p) * (e13 + e23)},
+            {(p - 1) * e11 + p * e12, (p - 1) * e12 + p * e22, (p - 1) * e13 + p * e23},
+            {-e12 + e13, -e22 + e23, -e23 + e33},
+            {-e13, -e23, -e33}};
+        double L[4][3];
+        for (int k = 0; k < 4; k++)
+            for (int d = 0; d < 3; d++) L[k][d] = w[k][0] * b1[d] + w[k][1] * b2[d] + w[k][2] * b3[d];
+        if (are_equal1(pos[i])) {   /* the energy is a function of (c1, c2', c3, c4) */
+            const double factor1 = (p - 1) / p, factor2 = 1 / p;
+            for (int d = 0; d < 3; d++) {
+                L[0][d] = L[0][d] + L[1][d] * factor1;
+                L[1][d] = L[1][d] * factor2;
+            }
+        }
+        double *F[4] = {f + bs[4 * i], f + bs[4 * i + 1], f + bs[4 * i + 2], f + bs[4 * i + 3]};
+        for (int d = 0; d < 3; d++) {
+            for (int k = 0; k < 4; k++) F[k][d] += L[k][d];
+            if (stretch3) stretch3[3 * i + d] = L[2][d];
         }
     }
 }
@@ -1204,7 +1311,8 @@ static void ffm_each(const orc_system *o, const double *x, double *e) {
     e[MDY_FF_MS] = orc_ls_energy(x, t->ms_n, t->ms_beadset, t->ms_k, t->ms_l0, t->ms_pos1, t->ms_pos2);
     e[MDY_FF_BS] = orc_bs_energy(x, t->bs_n, t->bs_beadset, t->bs_k, t->bs_l0, t->bs_pos);
     e[MDY_FF_BB] = orc_bb_energy(x, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0);
-    e[MDY_FF_BDH] = orc_bdh_energy(x, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos);
+    e[MDY_FF_BDH] = (o->sys.br_dihedral_type == MDY_BRDIH_QUADRATIC ? orc_bdhq_energy : orc_bdh_energy)(
+        x, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos);
     e[MDY_FF_BP] = orc_bp_energy(x, t->bp_n, t->bp_beadset, t->bp_k, t->bp_pos);
     if (!o->sys.exvol_enabled) e[MDY_FF_EV] = 0.0;
     else if (o->ev_mon) e[MDY_FF_EV] = orc_evm_energy(x, (int32_t)o->ev_n, o->ev_bs, o->ev_mon, o->ev_eql, o->ev_krep);
@@ -1255,7 +1363,8 @@ int orc_ffm_forces(const orc_system *o, const double *x, double *f, int32_t nvar
     orc_ls_forces(x, f, t->ms_n, t->ms_beadset, t->ms_k, t->ms_l0, t->ms_pos1, t->ms_pos2, NULL);
     orc_bs_forces(x, f, t->bs_n, t->bs_beadset, t->bs_k, t->bs_l0, t->bs_pos, NULL);
     orc_bb_forces(x, f, t->bb_n, t->bb_beadset, t->bb_k, t->bb_theta0, NULL);
-    orc_bdh_forces(x, f, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos, NULL);
+    (o->sys.br_dihedral_type == MDY_BRDIH_QUADRATIC ? orc_bdhq_forces : orc_bdh_forces)(
+        x, f, t->bdh_n, t->bdh_beadset, t->bdh_k, t->bdh_pos, NULL);
     orc_bp_forces(x, f, t->bp_n, t->bp_beadset, t->bp_k, t->bp_pos, NULL);
     if (o->sys.exvol_enabled && o->ev_mon) {
         orc_evm_forces(x, f, (int32_t)o->ev_n, o->ev_bs, o->ev_mon, o->ev_eql, o->ev_krep);
@@ -1610,6 +1719,9 @@ void orc_xmath_atan(const double *x, double *y, int64_t n) {
 }
 void orc_xmath_exp(const double *x, double *y, int64_t n) {
     for (int64_t i = 0; i < n; i++) y[i] = mdy_exp(x[i]);
+}
+void orc_xmath_acos(const double *x, double *y, int64_t n) {
+    for (int64_t i = 0; i < n; i++) y[i] = mdy_acos(x[i]);
 }
 /* exact sum of x[0..n) added in the given order, or (stride > 1) in a
  * strided interleaving merged afterwards: the result must not depend on it */

@@ -63,6 +63,9 @@ double orc_bp_energy(const double *x, int32_t n, const int32_t *bs, const double
 void   orc_bp_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kpos, const double *pos,
                      double *stretch3);
 double orc_bdh_energy(const double *x, int32_t n, const int32_t *bs, const double *kdih, const double *pos);
+double orc_bdhq_energy(const double *x, int32_t n, const int32_t *bs, const double *kdih, const double *pos);
+void   orc_bdhq_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kdih, const double *pos,
+                       double *stretch3);
 void   orc_bdh_forces(const double *x, double *f, int32_t n, const int32_t *bs, const double *kdih, const double *pos,
                       double *stretch3);
 /* boundary load forces on filament ends (ForceFieldManager::computeLoadForces) */
@@ -151,6 +154,7 @@ int orc_recover_move(void *user, double *x, const double *dir, int32_t ndof, dou
 /* shared numerics of medyan_amd/csrc/mdy_xmath.h, exported for their tests */
 void   orc_xmath_atan(const double *x, double *y, int64_t n);
 void   orc_xmath_exp(const double *x, double *y, int64_t n);
+void   orc_xmath_acos(const double *x, double *y, int64_t n);
 double orc_xsum(const double *x, int64_t n, int64_t stride);
 
 #ifdef __cplusplus

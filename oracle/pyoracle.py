@@ -61,7 +61,7 @@ class Oracle:
         L.orc_ev_pair_energy_analytic.restype = C.c_double
         L.orc_ev_pair_energy_simpson.restype = C.c_double
         for fn in ("orc_fs_energy", "orc_fb_energy", "orc_ls_energy", "orc_bs_energy", "orc_bb_energy",
-                   "orc_bd_energy", "orc_bdh_energy", "orc_bp_energy"):
+                   "orc_bd_energy", "orc_bdh_energy", "orc_bdhq_energy", "orc_bp_energy"):
             getattr(L, fn).restype = C.c_double
         L.orc_nl_pairs.restype = C.c_int64
         L.orc_ev_vectorize.restype = C.c_int64
@@ -75,6 +75,7 @@ class Oracle:
         L.orc_xsum.argtypes = [_f64p, C.c_int64, C.c_int64]
         L.orc_xmath_atan.argtypes = [_f64p, _f64p, C.c_int64]
         L.orc_xmath_exp.argtypes = [_f64p, _f64p, C.c_int64]
+        L.orc_xmath_acos.argtypes = [_f64p, _f64p, C.c_int64]
 
     # -- numerics mode ------------------------------------------------------
     def set_exact(self, on):
@@ -95,6 +96,12 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)
         self.lib.orc_xmath_exp(_p(x), _p(y), x.size)
+        return y
+
+    def xmath_acos(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.lib.orc_xmath_acos(_p(x), _p(y), x.size)
         return y
 
     def xsum(self, x, stride=1):
@@ -128,8 +135,8 @@ class Oracle:
     # bonded / cross-link force fields: name in fs, fb, ls, bs, bb; params in
     # the C order (fs: k, l0; fb: k, theta0; ls: k, l0, pos1, pos2;
     # bs: k, l0, pos; bb: k, theta0); bead offsets already x3
-    _NPAR = {"fs": 2, "fb": 2, "ls": 4, "bs": 3, "bb": 2, "bdh": 2, "bp": 2}
-    _FSTRETCH = {"ls": 1, "bs": 3, "bb": 3, "bdh": 3, "bp": 3}
+    _NPAR = {"fs": 2, "fb": 2, "ls": 4, "bs": 3, "bb": 2, "bdh": 2, "bdhq": 2, "bp": 2}
+    _FSTRETCH = {"ls": 1, "bs": 3, "bb": 3, "bdh": 3, "bdhq": 3, "bp": 3}
 
     def _ff_args(self, name, x, bs, params):
         if len(params) != self._NPAR[name]:
@@ -316,7 +323,9 @@ class OracleSystem:
             self.o.lib.orc_bs_forces(_p(x), _p(f), len(n.bs_k), _p(arr[0], C.c_int32), _p(arr[1]), _p(arr[2]),
                                      _p(arr[3]), _p(part))
             br[: part.size] += part
-        for fn, bs, k, pos in ((self.o.lib.orc_bdh_forces, n.bdh_beadset, n.bdh_k, n.bdh_pos),
+        bdh_fn = (self.o.lib.orc_bdhq_forces if n.br_dihedral_type == _abi.MDY_BRDIH_QUADRATIC
+                  else self.o.lib.orc_bdh_forces)
+        for fn, bs, k, pos in ((bdh_fn, n.bdh_beadset, n.bdh_k, n.bdh_pos),
                                (self.o.lib.orc_bp_forces, n.bp_beadset, n.bp_k, n.bp_pos)):
             if len(k):
                 part = np.zeros(3 * len(k))
