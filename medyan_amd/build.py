@@ -12,6 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmdy_hip.so")
+TRAJ_OUT = os.path.join(HERE, "libmdy_traj.so")
+# the HDF5 C library for the traj.h5 writer (a separate library, optional)
+HDF5_ROOT = os.environ.get("MDY_HDF5_ROOT", "/opt/conda")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["mdy_api.hip", "mdy_eval.hip", "mdy_nl.hip", "mdy_cg.hip", "mdy_hvp.hip", "mdy_aux.hip"]
@@ -49,7 +52,21 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, OUT)
+    build_traj()
     return OUT
+
+
+def build_traj():
+    """libmdy_traj.so (traj.h5 writer) when the HDF5 headers are present;
+    returns its path or None."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        return None
+    tmp = TRAJ_OUT + ".tmp"
+    _run(["g++", *COMMON, "-shared", "-I", inc, os.path.join(CSRC, "traj_h5.cpp"), "-o", tmp,
+          "-L", lib, "-lhdf5", f"-Wl,-rpath,{lib}"])
+    os.replace(tmp, TRAJ_OUT)
+    return TRAJ_OUT
 
 
 if __name__ == "__main__":

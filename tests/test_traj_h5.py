@@ -1,0 +1,149 @@
+"""traj.h5 snapshot output (SnapshotOutput, src/Output.h:504-571) written by
+libmdy_traj.so. The reference ships no traj.h5 file, so the layout is checked
+against its writers as read in the source (group and dataset names, element
+types, matrix orientation: src/Structure/Output/O*.hpp, src/Util/Io/H5.hpp:106-118)
+with the HDF5 library's own h5dump, and the values by reading them back."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from medyan_amd import _abi, network
+
+HDF5_ROOT = os.environ.get("MDY_HDF5_ROOT", "/opt/conda")
+H5DUMP = os.path.join(HDF5_ROOT, "bin", "h5dump")
+pytestmark = pytest.mark.skipif(not os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")),
+                                reason="no HDF5 C library in this image")
+
+
+def h5dump(*args):
+    r = subprocess.run([H5DUMP, *args], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    return r.stdout
+
+
+def values(path, dset):
+    out = h5dump("-m", "%.17g", "-y", "-w", "0", "-d", dset, path)
+    body = out[out.index("DATA {") + 6: out.rindex("}")]
+    return np.array([float(v) for v in re.findall(r"[-+0-9.eE]+(?:inf|nan)?", body.split("}")[0])])
+
+
+@pytest.fixture(scope="module")
+def net(lib):
+    return network.generate(lib, "net500", n_filaments=20, cyl_per_filament=4, n_linkers=15, n_motors=3,
+                            n_branches=4, seed=9)
+
+
+def test_write_and_read_back(net, tmp_path):
+    from medyan_amd import traj
+    path = str(tmp_path / "traj.h5")
+    out = traj.SnapshotOutput(path)
+    rng = np.random.default_rng(1)
+    xs = [net.coord, net.coord + rng.normal(0, 1.0, net.coord.shape)]
+    snaps = []
+    for k, x in enumerate(xs):
+        e = rng.uniform(0, 100, _abi.MDY_NFF)
+        s = traj.snapshot_of(net, x, np.concatenate([[e.sum()], e]), time=0.5 * k)
+        out.append(s)
+        snaps.append(s)
+        assert out.count() == k + 1
+    finished = lambda: h5dump("-d", "/header/finished", path).split("DATA {")[1]
+    assert "FALSE" in finished()
+    out.finish()
+    assert "TRUE" in finished()
+
+    fils = traj.filament_beads(net)
+    assert sum(len(f) for f in fils) == net.n_beads            # every bead in exactly one filament
+    assert np.array_equal(np.sort(np.concatenate(fils)), np.arange(net.n_beads))
+    for k, (x, s) in enumerate(zip(xs, snaps)):
+        back = out.read_filaments(k, net.n_beads)
+        want = np.concatenate([x.reshape(-1, 3)[f] for f in fils]).ravel()
+        assert np.array_equal(back.view(np.int64), want.view(np.int64))
+        np.testing.assert_array_equal(values(path, f"/snapshots/{k}/energies"), s["energies"])
+        np.testing.assert_array_equal(values(path, f"/snapshots/{k}/time"), [0.5 * k])
+        np.testing.assert_array_equal(values(path, f"/snapshots/{k}/linkers/0/coords"), s["lk_coord"][0])
+    assert values(path, "/header/count")[0] == 2
+    n_lk = len(net.ls_k) + len(net.ms_k) + len(net.bs_k)
+    assert values(path, "/snapshots/1/linkers/count")[0] == n_lk
+    assert values(path, "/snapshots/1/filaments/count")[0] == len(fils)
+
+
+def test_layout_and_types(net, tmp_path):
+    from medyan_amd import traj
+    path = str(tmp_path / "traj.h5")
+    out = traj.SnapshotOutput(path, version="v-test")
+    out.append(traj.snapshot_of(net, net.coord, np.zeros(_abi.MDY_NFF + 1)))
+    out.finish()
+    d = h5dump("-H", path)
+    # header: count (int64), finished (bool enum), version attribute, meta
+    assert re.search(r'GROUP "header" \{\s*ATTRIBUTE "version" \{\s*DATATYPE\s+H5T_STRING', d)
+    assert re.search(r'DATASET "count" \{\s*DATATYPE\s+H5T_STD_I64LE\s*DATASPACE\s+SCALAR', d)
+    assert re.search(r'DATASET "finished" \{\s*DATATYPE\s+H5T_ENUM \{\s*H5T_STD_I8LE;\s*"FALSE"\s+0;\s*"TRUE"\s+1;', d)
+    for name in ("vertexColumnNamesFloat64", "vertexColumnNamesInt64", "globalFilamentModel", "system", "chem",
+                 "energyNames", "diffusingSpeciesNames", "globalSpeciesNames"):
+        assert f'DATASET "{name}"' in d, name
+    # snapshot 0: time, energies, filaments (numBeads x 3), linkers, bubbles, membranes, chemistry
+    fils = traj.filament_beads(net)
+    assert re.search(r'DATASET "0" \{\s*DATATYPE\s+H5T_IEEE_F64LE\s*DATASPACE\s+SIMPLE \{ \( %d, 3 \)' % len(fils[0]), d)
+    assert re.search(r'DATASET "coords" \{\s*DATATYPE\s+H5T_IEEE_F64LE\s*DATASPACE\s+SIMPLE \{ \( 2, 3 \)', d)
+    for g in ("filaments", "linkers", "bubbles", "membranes"):
+        assert f'GROUP "{g}"' in d
+    for name in ("time", "energies", "diffusingSpeciesCopyNumbers", "globalSpeciesCopyNumbers", "subtype", "id"):
+        assert f'DATASET "{name}"' in d, name
+    names = h5dump("-d", "/header/meta/energyNames", path)
+    assert all(f'"{n}"' in names for n in ["total", *_abi.FF_NAMES])
+    assert '"bead-cylinder"' in h5dump("-d", "/header/meta/globalFilamentModel", path)
+    assert '"v-test"' in h5dump("-a", "/header/version", path)
+    kinds = {h5dump("-d", f"/snapshots/0/linkers/{i}/type", path).split('"')[-2]
+             for i in range(len(net.ls_k) + len(net.ms_k) + len(net.bs_k))}
+    assert kinds == {"linker", "motor", "brancher"}
+
+
+def test_branch_point_coordinates(net):
+    from medyan_amd import traj
+    s = traj.snapshot_of(net, net.coord, [0.0])
+    x = net.coord.reshape(-1, 3)
+    b = net.bs_beadset.reshape(-1, 3) // 3
+    br = s["lk_coord"][s["lk_kind"] == traj.MDY_TRAJ_BRANCHER]
+    p = net.bs_pos[:, None]
+    np.testing.assert_allclose(br[:, :3], (1 - p) * x[b[:, 0]] + p * x[b[:, 1]], rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(br[:, 3:], x[b[:, 2]])
+
+
+def test_errors(net, tmp_path):
+    from medyan_amd import traj
+    s = traj.snapshot_of(net, net.coord, [0.0])
+    o = traj.SnapshotOutput.__new__(traj.SnapshotOutput)   # a handle on a file that was never created
+    o.lib, o.filename = traj.load_library(), os.fsencode(str(tmp_path / "missing.h5"))
+    with pytest.raises(RuntimeError, match="MDY_ERR_IO"):
+        o.append(s)
+    out = traj.SnapshotOutput(str(tmp_path / "t.h5"))
+    bad = dict(s, lk_kind=np.full_like(s["lk_kind"], 7))
+    with pytest.raises(RuntimeError, match="MDY_ERR_ARG"):
+        out.append(bad)
+    assert out.count() == 0
+
+
+def test_abi_symbols_and_struct_layout(tmp_path):
+    """libmdy_traj.so exports every function include/mdy_traj.h declares, and
+    the ctypes mirror of mdy_traj_snapshot has the C layout."""
+    import ctypes as C
+    from medyan_amd import traj
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "mdy_traj.h")).read()
+    names = re.findall(r"^(?:int|int64_t|const char \*)\s*(mdy_traj_\w+)\(", hdr, re.M)
+    assert len(names) == 6
+    lib = traj.load_library()
+    for n in names:
+        assert hasattr(lib, n), n
+    fields = [f for f, _ in traj.Snapshot._fields_]
+    prog = ('#include <stdio.h>\n#include <stddef.h>\n#include "mdy_traj.h"\nint main(void){\n'
+            + "".join(f'printf("%zu\\n", offsetof(mdy_traj_snapshot, {f}));\n' for f in fields)
+            + 'printf("%zu\\n", sizeof(mdy_traj_snapshot)); return 0;}\n')
+    c, exe = tmp_path / "l.c", tmp_path / "l"
+    c.write_text(prog)
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(root, "include"), str(c), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split()]
+    assert got == [getattr(traj.Snapshot, f).offset for f in fields] + [C.sizeof(traj.Snapshot)]
