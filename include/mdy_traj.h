@@ -22,8 +22,10 @@
  * (src/Util/Io/H5.hpp:106-118).
  *
  * Each call opens and closes the file (as SnapshotOutput::append does), so a
- * file is complete after every call. Returns MDY_OK (0) or MDY_ERR_ARG (bad
- * argument) / MDY_ERR_IO (the HDF5 library reported an error).
+ * file is complete after every call. Returns MDY_OK (0) or MDY_ERR_ARG (-1,
+ * bad argument), MDY_ERR_CAPACITY (-5, readback buffer too small) or
+ * MDY_ERR_IO (the HDF5 library reported an error, or the file's contents
+ * are not what this layout expects).
  */
 #ifndef MDY_TRAJ_H
 #define MDY_TRAJ_H

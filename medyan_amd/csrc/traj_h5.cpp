@@ -46,20 +46,21 @@ struct Silence {   // the library's own error stack printing, off for one call
     ~Silence() { H5Eset_auto2(H5E_DEFAULT, fn, data); }
 };
 
-struct Err {};   // thrown inside this file only, turned into MDY_ERR_IO at the boundary
+// thrown inside this file only, turned into its code at the boundary
+struct Err {
+    int code;
+};
+[[noreturn]] void raise(const std::string &msg, int code = MDY_ERR_IO) {
+    g_err = msg;
+    throw Err{code};
+}
 
 hid_t check(hid_t id, const char *what) {
-    if (id < 0) {
-        g_err = std::string("HDF5: ") + what;
-        throw Err{};
-    }
+    if (id < 0) raise(std::string("HDF5: ") + what);
     return id;
 }
 void check(herr_t rc, const char *what, int) {
-    if (rc < 0) {
-        g_err = std::string("HDF5: ") + what;
-        throw Err{};
-    }
+    if (rc < 0) raise(std::string("HDF5: ") + what);
 }
 
 H vlen_string_type() {
@@ -145,10 +146,7 @@ void write_u64s_empty(hid_t loc, const char *name, int rank, const hsize_t *dims
 }
 
 H open_rw(const char *path) {
-    if (!path) {
-        g_err = "path is null";
-        throw Err{};
-    }
+    if (!path) raise("path is null", -1);
     return H(check(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), "cannot open file"), H5Fclose);
 }
 
@@ -163,8 +161,8 @@ template <class F> int guarded(F &&f) {
     Silence quiet;
     try {
         return f();
-    } catch (const Err &) {
-        return MDY_ERR_IO;
+    } catch (const Err &e) {
+        return e.code;
     } catch (...) {
         g_err = "unexpected exception";
         return MDY_ERR_IO;
@@ -200,7 +198,7 @@ int mdy_traj_create(const char *path, const char *version, int32_t n_energy_name
         write_str(meta, "chem", chem_input);
         std::vector<const char *> names;
         for (int32_t i = 0; i < n_energy_names; i++) {
-            if (!energy_names[i]) throw (g_err = "energy name is null", Err{});
+            if (!energy_names[i]) raise("energy name is null", -1);
             names.push_back(energy_names[i]);
         }
         write_strs(meta, "energyNames", names);
@@ -226,7 +224,7 @@ int mdy_traj_append(const char *path, const mdy_traj_snapshot *s) {
         const int64_t count = read_i64(hdr, "count");
         H snaps = group(f, "snapshots");
         const std::string key = std::to_string(count);
-        if (exists(snaps, key.c_str())) throw (g_err = "snapshot exists", Err{});
+        if (exists(snaps, key.c_str())) raise("snapshot " + key + " exists");
         H snap = group(snaps, key.c_str());
         write_f64(snap, "time", s->time);
         {
@@ -304,10 +302,10 @@ int64_t mdy_traj_read_filaments(const char *path, int64_t k, double *coord, int6
             H d(check(H5Dopen2(fils, std::to_string(i).c_str(), H5P_DEFAULT), "filament"), H5Dclose);
             H sp(check(H5Dget_space(d), "filament space"), H5Sclose);
             hsize_t dims[2] = {0, 0};
-            if (H5Sget_simple_extent_ndims(sp) != 2) throw (g_err = "filament rank", Err{});
+            if (H5Sget_simple_extent_ndims(sp) != 2) raise("filament rank");
             H5Sget_simple_extent_dims(sp, dims, nullptr);
             const int64_t m = (int64_t)(dims[0] * dims[1]);
-            if (total + m > cap) throw (g_err = "capacity", Err{});
+            if (total + m > cap) raise("capacity", -5);   // MDY_ERR_CAPACITY
             check(H5Dread(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, coord + total), "filament read", 0);
             total += m;
         }

@@ -124,6 +124,9 @@ def test_errors(net, tmp_path):
     with pytest.raises(RuntimeError, match="MDY_ERR_ARG"):
         out.append(bad)
     assert out.count() == 0
+    out.append(s)
+    with pytest.raises(RuntimeError, match="MDY_ERR_CAPACITY"):
+        out.read_filaments(0, net.n_beads - 1)
 
 
 def test_abi_symbols_and_struct_layout(tmp_path):
