@@ -132,3 +132,40 @@ def test_hvp(oracle, qnets):
     err = np.abs(hv[0] - hv[1] - ref).max() / np.abs(ref).max()
     print(f"[gpu] quadratic dihedral hvp: max |dHv - CD| / max|CD| = {err:.2e}")
     assert err < 1e-7
+
+
+def test_batch_mixes_dihedral_forms(qnets):
+    """One batched launch over networks of both dihedral forms: each equals
+    its own single-network minimization bit for bit."""
+    q = qnets["many_branches"]
+    c = q.copy()
+    c.br_dihedral_type = _abi.MDY_BRDIH_COSINE
+    nets = [q, c, qnets["branched_full"]]
+    singles = [ctx_for(n).minimize(_abi.default_cg_params(n.n_beads)) for n in nets]
+    ctxs = [ctx_for(n) for n in nets]
+    xs = [torch.tensor(n.coord, device="cuda") for n in nets]
+    res, _, _ = mechanics.minimize_batch(ctxs, [_abi.default_cg_params(n.n_beads) for n in nets],
+                                         [x.data_ptr() for x in xs], threads_per_block=128)
+    torch.cuda.synchronize()
+    for i, (x1, _, r1) in enumerate(singles):
+        assert res[i].status == 0 and res[i].num_iter == r1.num_iter
+        assert bits_equal(xs[i].cpu().numpy(), x1)
+    assert singles[0][2].num_iter != singles[1][2].num_iter or not bits_equal(singles[0][0], singles[1][0])
+
+
+def test_energy_jvp(oracle, qnets):
+    """d E_k / dx . w per force field; the dihedral's entry against a central
+    difference of the oracle's quadratic-form energy."""
+    net = qnets["branched_full"]
+    ctx = mechanics.Context(coplanar_mode=_abi.MDY_COPLANAR_EXACT)
+    ctx.set_network(net)
+    ctx.build_neighbors()
+    rng = np.random.default_rng(9)
+    w = rng.normal(size=3 * net.n_beads)
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a, np.float64), device="cuda")
+    jv = autodiff.energy_jvp(ctx, dev(net.coord), dev(w)).cpu().numpy()
+    args = (net.bdh_beadset, net.bdh_k, net.bdh_pos)
+    E = lambda x: oracle.ff_energy("bdhq", x, *args)
+    cd = lambda h: (E(net.coord + h * w) - E(net.coord - h * w)) / (2 * h)
+    ref = (4 * cd(5e-5) - cd(1e-4)) / 3
+    assert abs(jv[_abi.MDY_FF_BDH] - ref) <= 1e-7 * max(1.0, abs(ref))
