@@ -169,3 +169,16 @@ def test_energy_jvp(oracle, qnets):
     cd = lambda h: (E(net.coord + h * w) - E(net.coord - h * w)) / (2 * h)
     ref = (4 * cd(5e-5) - cd(1e-4)) / 3
     assert abs(jv[_abi.MDY_FF_BDH] - ref) <= 1e-7 * max(1.0, abs(ref))
+
+
+def test_unknown_form_rejected(qnets):
+    """BranchingFF.h:84-88 throws for an unrecognized dihedral type; the
+    ABI returns MDY_ERR_ARG and keeps no network."""
+    n = qnets["branched_full"].copy()
+    n.br_dihedral_type = 2
+    ctx = mechanics.Context(0)
+    with pytest.raises(mechanics.MdyError, match="br_dihedral_type") as e:
+        ctx.set_network(n)
+    assert e.value.code == _abi.MDY_ERR_ARG
+    with pytest.raises(mechanics.MdyError):
+        ctx.energy(n.coord)
