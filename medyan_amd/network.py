@@ -165,6 +165,20 @@ class Network:
         self._keep = keep
         return s
 
+    def with_branching_dihedral(self, ff_type):
+        """A copy whose branching dihedral is BRDIHEDRALFFTYPE `ff_type`
+        (BranchingFF.h:64-88): "COSINE" or "QUADRATIC". The reference's
+        COSINEV2 / QUADRATICV2 forms are not provided (NotImplementedError);
+        any other name is rejected as the reference rejects it (ValueError)."""
+        forms = {"COSINE": _abi.MDY_BRDIH_COSINE, "QUADRATIC": _abi.MDY_BRDIH_QUADRATIC}
+        if ff_type in ("COSINEV2", "QUADRATICV2"):
+            raise NotImplementedError(f"branching dihedral {ff_type} is not provided")
+        if ff_type not in forms:
+            raise ValueError(f"Branching dihedral FF {ff_type} not recognized.")
+        n = self.copy()
+        n.br_dihedral_type = forms[ff_type]
+        return n
+
     def with_monomer_exvol(self, interval, mon_size=2.7):
         """A copy whose excluded volume is the monomer-pair sum
         (CylinderVolumeMon, cylinderVolumeExclusionFFType MONOMER) sampling

@@ -16,9 +16,7 @@ pytestmark = pytest.mark.gpu
 
 
 def quadratic(net):
-    n = net.copy()
-    n.br_dihedral_type = _abi.MDY_BRDIH_QUADRATIC
-    return n
+    return net.with_branching_dihedral("QUADRATIC")
 
 
 @pytest.fixture(scope="module")

@@ -120,9 +120,21 @@ def test_modes_differ_by_rounding_only(oracle):
 
 
 def quadratic(net):
-    n = net.copy()
-    n.br_dihedral_type = _abi.MDY_BRDIH_QUADRATIC
-    return n
+    return net.with_branching_dihedral("QUADRATIC")
+
+
+def test_dihedral_type_names(lib):
+    net = network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5, n_branches=8, seed=4)
+    assert net.br_dihedral_type == _abi.MDY_BRDIH_COSINE
+    q = net.with_branching_dihedral("QUADRATIC")
+    assert q.br_dihedral_type == _abi.MDY_BRDIH_QUADRATIC and net.br_dihedral_type == _abi.MDY_BRDIH_COSINE
+    assert q.with_branching_dihedral("COSINE").br_dihedral_type == _abi.MDY_BRDIH_COSINE
+    assert q.system().br_dihedral_type == _abi.MDY_BRDIH_QUADRATIC
+    for v2 in ("COSINEV2", "QUADRATICV2"):
+        with pytest.raises(NotImplementedError):
+            net.with_branching_dihedral(v2)
+    with pytest.raises(ValueError, match="not recognized"):
+        net.with_branching_dihedral("HARMONIC")
 
 
 def test_system_uses_the_selected_form(oracle, lib):
