@@ -22,7 +22,8 @@
  * (src/Util/Io/H5.hpp:106-118).
  *
  * Each call opens and closes the file (as SnapshotOutput::append does), so a
- * file is complete after every call. Returns MDY_OK (0) or MDY_ERR_ARG (-1,
+ * file is complete after every call. Calls come from one thread at a time
+ * (the HDF5 library may be built without thread safety). Returns MDY_OK (0) or MDY_ERR_ARG (-1,
  * bad argument), MDY_ERR_CAPACITY (-5, readback buffer too small) or
  * MDY_ERR_IO (the HDF5 library reported an error, or the file's contents
  * are not what this layout expects).
