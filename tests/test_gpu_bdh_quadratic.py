@@ -26,10 +26,14 @@ def qnets(lib):
                                                     seed=4, k_br_dihedral=10.0, k_br_position=20.0)),
         "many_branches": quadratic(network.generate(lib, "net500", n_filaments=60, cyl_per_filament=4,
                                                     n_branches=20, seed=6, k_br_dihedral=10.0, k_br_position=20.0)),
+        # with the monomer-sum excluded volume as well (every optional form at once)
+        "with_monomer_exvol": quadratic(network.generate(lib, "net500", n_filaments=40, cyl_per_filament=5,
+                                                         n_branches=8, seed=4, k_br_dihedral=10.0,
+                                                         k_br_position=20.0).with_monomer_exvol(2)),
     }
 
 
-CASES = ["branched_full", "many_branches"]
+CASES = ["branched_full", "many_branches", "with_monomer_exvol"]
 
 
 @pytest.mark.parametrize("case", CASES)
