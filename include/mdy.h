@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MDY_ABI_VERSION 2
+#define MDY_ABI_VERSION 3
 
 enum {
     MDY_OK = 0,
@@ -154,6 +154,12 @@ typedef struct mdy_cg_params {
     int32_t n_beads_limit;        /* Bead::numBeads() used by the call limit           */
     int32_t try_to_recover;       /* tryToRecoverInLineSearchError                     */
     int32_t line_search;          /* MDY_LS_*: lineSearchAlgorithm (CGMethod.hpp:51)   */
+    int32_t report_on_call_limit; /* reportOnCallLimit (CGMethod.hpp:59; the reference's
+                                     default is true, set 1): when the force-call limit
+                                     ends the minimization, restore the lowest-energy
+                                     coordinates backed up near convergence
+                                     (CGMethod.hpp:1589-1608). MController sets it
+                                     false for interrupted segments (MController.h:171). */
 } mdy_cg_params;
 /* BACKTRACKING (backtrackingLineSearch, the default) or QUADRATIC
  * (quadraticLineSearchV2 + quadraticoptimization outside safe mode,

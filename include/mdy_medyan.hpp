@@ -43,6 +43,9 @@ struct ConjugateGradientParams {
     // "BACKTRACKING" or "QUADRATIC"; CGMethod::minimize (CGMethod.hpp:1319-1331)
     // takes any other name as QUADRATIC
     std::string lineSearchAlgorithm = "BACKTRACKING";
+    // restore the lowest-energy backup when the call limit ends the run
+    // (CGMethod.hpp:59, 1589-1608)
+    bool reportOnCallLimit = true;
 };
 
 struct EnergyReport {
@@ -202,6 +205,7 @@ public:
         cp.n_beads_limit = numDof / 3;
         cp.try_to_recover = p.tryToRecoverInLineSearchError ? 1 : 0;
         cp.line_search = p.lineSearchAlgorithm == "BACKTRACKING" ? MDY_LS_BACKTRACKING : MDY_LS_QUADRATIC;
+        cp.report_on_call_limit = p.reportOnCallLimit ? 1 : 0;
         if (outForce) outForce->assign(numDof, 0.0);
         mdy_min_result r{};
         ffm.check(mdy_minimize(ffm.ctx(), &cp, coord.data(), outForce ? outForce->data() : nullptr, &r));

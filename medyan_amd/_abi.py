@@ -6,7 +6,7 @@ medyan_amd.mechanics.
 """
 import ctypes as C
 
-MDY_ABI_VERSION = 2
+MDY_ABI_VERSION = 3
 
 MDY_OK = 0
 ERRORS = {
@@ -88,7 +88,8 @@ class CGParams(C.Structure):
     _fields_ = [("method", C.c_int32), ("max_dist", C.c_double), ("lambda_max", C.c_double),
                 ("grad_tol", C.c_double), ("energy_rel_tol", C.c_double),
                 ("energy_call_limit", C.c_int32), ("force_call_limit", C.c_int32),
-                ("n_beads_limit", C.c_int32), ("try_to_recover", C.c_int32), ("line_search", C.c_int32)]
+                ("n_beads_limit", C.c_int32), ("try_to_recover", C.c_int32), ("line_search", C.c_int32),
+                ("report_on_call_limit", C.c_int32)]
 
 
 class MinResult(C.Structure):
@@ -136,7 +137,7 @@ class Network(C.Structure):
 def default_cg_params(n_beads, grad_tol=1.0, max_dist=1.0, lambda_max=0.01,
                       force_call_limit=100000, method=MDY_CG_POLAK_RIBIERE,
                       energy_rel_tol=1e-9, try_to_recover=1, energy_call_limit=0,
-                      line_search=MDY_LS_BACKTRACKING):
+                      line_search=MDY_LS_BACKTRACKING, report_on_call_limit=1):
     """CG parameters of the benchmark workloads: GRADIENTTOLERANCE 1, MAXDISTANCE 1,
     LAMBDAMAX 0.01 (the shipped example values), energy-change tolerance 1e-9."""
     p = CGParams()
@@ -150,4 +151,5 @@ def default_cg_params(n_beads, grad_tol=1.0, max_dist=1.0, lambda_max=0.01,
     p.n_beads_limit = n_beads
     p.try_to_recover = try_to_recover
     p.line_search = line_search
+    p.report_on_call_limit = report_on_call_limit
     return p

@@ -62,6 +62,19 @@ static bool bad_offsets(const int32_t *bs, size_t n, int nb) {
     return false;
 }
 
+// a force field with interactions needs every one of its parameter arrays
+static bool missing_params(const mdy_ff_tables &t) {
+    if (t.fs_n > 0 && (!t.fs_k || !t.fs_l0)) return true;
+    if (t.fb_n > 0 && (!t.fb_k || !t.fb_theta0)) return true;
+    if (t.ls_n > 0 && (!t.ls_k || !t.ls_l0 || !t.ls_pos1 || !t.ls_pos2)) return true;
+    if (t.ms_n > 0 && (!t.ms_k || !t.ms_l0 || !t.ms_pos1 || !t.ms_pos2)) return true;
+    if (t.bs_n > 0 && (!t.bs_k || !t.bs_l0 || !t.bs_pos)) return true;
+    if (t.bb_n > 0 && (!t.bb_k || !t.bb_theta0)) return true;
+    if (t.bdh_n > 0 && (!t.bdh_k || !t.bdh_pos)) return true;
+    if (t.bp_n > 0 && (!t.bp_k || !t.bp_pos)) return true;
+    return false;
+}
+
 }  // namespace mdy
 
 using namespace mdy;
@@ -166,6 +179,11 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
     if (!ctx || !s) return MDY_ERR_ARG;
     Ctx &c = ctx->c;
     (void)hipSetDevice(c.device);
+    // the previous network is dropped before validation: a rejected call
+    // leaves the context with no network, never with the old tables
+    c.have_sys = false;
+    c.have_nl = false;
+    c.have_dir = false;
     const int nb = s->n_beads;
     const mdy_ff_tables &t = s->ff;
     const mdy_cylinders &cy = s->cyl;
@@ -179,6 +197,8 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
         bad_offsets(t.bdh_beadset, 4 * (size_t)t.bdh_n, nb) || bad_offsets(t.bp_beadset, 3 * (size_t)t.bp_n, nb) ||
         bad_offsets(cy.bead1, cy.n, nb) || bad_offsets(cy.bead2, cy.n, nb))
         return fail(ctx, MDY_ERR_ARG, "bead offset out of range (must be 3*index < 3*n_beads)");
+    if (missing_params(t))
+        return fail(ctx, MDY_ERR_ARG, "null parameter array for a force field with interactions");
     for (int i = 0; i < cy.n; i++)
         if (cy.branch_child && (cy.branch_child[i] < -1 || cy.branch_child[i] >= cy.n))
             return fail(ctx, MDY_ERR_ARG, "branch_child out of range");
@@ -199,9 +219,6 @@ int mdy_set_network(mdy_ctx *ctx, const mdy_system *s) {
                 return fail(ctx, MDY_ERR_ARG, "monomer table: interval >= 1 and plus >= minus serial required");
     }
     hipStream_t st = c.stream;
-    c.have_sys = false;
-    c.have_nl = false;
-    c.have_dir = false;
     c.nb = nb;
     c.n_fs = t.fs_n; c.n_fb = t.fb_n; c.n_ls = t.ls_n; c.n_ms = t.ms_n; c.n_bs = t.bs_n; c.n_bb = t.bb_n;
     c.n_bdh = t.bdh_n; c.n_bp = t.bp_n;

@@ -45,7 +45,7 @@ struct CgArgs {
     int *tr_safe;
     long long tr_cap;
     // parameters
-    int method, line_search, try_recover, energy_call_limit, force_call_limit;
+    int method, line_search, try_recover, energy_call_limit, force_call_limit, report_call_limit;
     double max_dist, lambda_max, grad_tol, energy_rel_tol;
     long long spin_limit;
     int grid;             // blocks of this minimization's cluster (batched launch)
@@ -622,7 +622,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
             break;
         case ST_ITER:
             if (callLimit() || converged()) {
-                if (callLimit() && haveMin) op = OP_RESTORE;
+                if (callLimit() && haveMin && A.report_call_limit) op = OP_RESTORE;
                 stage = ST_FINAL;
                 break;
             }
@@ -945,6 +945,7 @@ static int prepare_cg(Ctx &c, const mdy_cg_params *p, double *d_x, int G, hipStr
     A.force_call_limit = p->force_call_limit > 3 * p->n_beads_limit ? p->force_call_limit : 3 * p->n_beads_limit;
     A.max_dist = p->max_dist; A.lambda_max = p->lambda_max; A.grad_tol = p->grad_tol;
     A.energy_rel_tol = p->energy_rel_tol;
+    A.report_call_limit = p->report_on_call_limit;
     A.spin_limit = 20000000LL;   // polls (about 10 s) before the watchdog fires
     A.grid = G;
     A.suspend_calls = 0;

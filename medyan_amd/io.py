@@ -13,6 +13,8 @@
 * ``network_from_filament_file``: a Network of those filaments
   (network.from_filaments) for the minimizer.
 """
+import math
+
 import numpy as np
 
 from . import network
@@ -47,7 +49,8 @@ def filament_beads(points, cyl_size=108.0, projection="STRAIGHT"):
         raise ValueError("Filament data specified contains insufficient coordinates.")
     if len(pts) == 2:
         d = float(np.sqrt(((pts[1] - pts[0]) ** 2).sum()))
-        nseg = max(1, int(np.round(d / cyl_size)))
+        # std::round: halves round away from zero (Controller.cpp:592-595)
+        nseg = max(1, int(math.floor(d / cyl_size + 0.5)))
     else:
         nseg = len(pts) - 1
     nb = nseg + 1

@@ -117,6 +117,9 @@ class ConjugateGradientParams:
     # "BACKTRACKING" or "QUADRATIC"; CGMethod::minimize (CGMethod.hpp:1319-1331)
     # takes any other name as QUADRATIC
     lineSearchAlgorithm: str = "BACKTRACKING"
+    # restore the lowest-energy backup when the call limit ends the run
+    # (CGMethod.hpp:59, 1589-1608)
+    reportOnCallLimit: bool = True
 
     def to_c(self, n_beads) -> _abi.CGParams:
         p = _abi.CGParams()
@@ -131,6 +134,7 @@ class ConjugateGradientParams:
         p.try_to_recover = 1 if self.tryToRecoverInLineSearchError else 0
         p.line_search = (_abi.MDY_LS_BACKTRACKING if self.lineSearchAlgorithm == "BACKTRACKING"
                          else _abi.MDY_LS_QUADRATIC)
+        p.report_on_call_limit = 1 if self.reportOnCallLimit else 0
         return p
 
 
@@ -176,12 +180,27 @@ class Context:
 
     # -- system ----------------------------------------------------------------
     def set_network(self, net: Network):
+        """Uploads a network. On failure the context holds no network (the
+        library drops the previous one before validating)."""
+        self.net = None
+        self._sys = None
+        sys_ = net.system()
+        self._check(self.lib.mdy_set_network(self.h, C.byref(sys_)))
         self.net = net
-        self._sys = net.system()
-        self._check(self.lib.mdy_set_network(self.h, C.byref(self._sys)))
+        self._sys = sys_
+
+    def _coord(self, coord, copy=False):
+        """A contiguous float64 view (or copy) of coord, checked against the
+        network's 3 * n_beads: the library reads and writes that many."""
+        if self.net is None:
+            raise MdyError(_abi.MDY_ERR_STATE, "no network set")
+        x = np.ascontiguousarray(self.net.coord if coord is None else coord, dtype=np.float64)
+        if x.size != 3 * self.net.n_beads:
+            raise ValueError(f"coordinates have {x.size} entries, the network needs 3 * {self.net.n_beads}")
+        return x.copy() if copy else x
 
     def build_neighbors(self, coord=None):
-        x = np.ascontiguousarray(self.net.coord if coord is None else coord, dtype=np.float64)
+        x = self._coord(coord)
         self._check(self.lib.mdy_build_neighbors(self.h, _f64(x)))
 
     def pairs(self):
@@ -214,15 +233,15 @@ class Context:
 
     # -- ForceFieldManager -------------------------------------------------------
     def energy(self, coord):
-        x = np.ascontiguousarray(coord, dtype=np.float64)
+        x = self._coord(coord)
         tot = C.c_double()
         per = np.zeros(_abi.MDY_NFF)
         self._check(self.lib.mdy_energy(self.h, _f64(x), C.byref(tot), _f64(per)))
         return tot.value, per
 
     def forces(self, coord, accum=None):
-        x = np.ascontiguousarray(coord, dtype=np.float64)
-        f = np.zeros_like(x) if accum is None else np.ascontiguousarray(accum, dtype=np.float64).copy()
+        x = self._coord(coord)
+        f = np.zeros_like(x) if accum is None else self._coord(accum, copy=True)
         self._check(self.lib.mdy_forces(self.h, _f64(x), _f64(f)))
         return f
 
@@ -233,7 +252,7 @@ class Context:
         reference turns into exceptions."""
         if isinstance(params, ConjugateGradientParams):
             params = params.to_c(self.net.n_beads)
-        x = np.ascontiguousarray(self.net.coord if coord is None else coord, dtype=np.float64).copy()
+        x = self._coord(coord, copy=True)
         f = np.zeros_like(x)
         self._check(self.lib.mdy_set_trace(self.h, int(trace_cap)))
         res = _abi.MinResult()
@@ -271,7 +290,7 @@ class Context:
 
     def stretch_forces(self, coord):
         """assignallforcemags outputs at coord: (linker, motor, branch [n x 3])."""
-        x = np.ascontiguousarray(coord, dtype=np.float64)
+        x = self._coord(coord)
         n = self.net
         nbr = max(len(n.bs_k), len(n.bdh_k), len(n.bp_k))
         lk, mt, br = np.zeros(max(len(n.ls_k), 1)), np.zeros(max(len(n.ms_k), 1)), np.zeros(max(3 * nbr, 1))
@@ -281,7 +300,7 @@ class Context:
     def load_forces(self, coord, mon_size=(2.7,), n_mon=(40,), max_mon=None):
         """computeLoadForces at coord (boundary neighbor list of the last
         build): (plus, minus), each n_beads x max_mon."""
-        x = np.ascontiguousarray(coord, dtype=np.float64)
+        x = self._coord(coord)
         ms = np.ascontiguousarray(mon_size, dtype=np.float64)
         nm = np.ascontiguousarray(n_mon, dtype=np.int32)
         mm = int(max_mon if max_mon is not None else nm.max())

@@ -110,9 +110,40 @@ class Network:
         return Network(**kw)
 
     # -- C ABI view -------------------------------------------------------
+    # (prefix, beads per interaction, per-interaction parameter arrays)
+    _TABLES = (("fs", 2, ("k", "l0")), ("fb", 3, ("k", "theta0")), ("ls", 4, ("k", "l0", "pos1", "pos2")),
+               ("ms", 4, ("k", "l0", "pos1", "pos2")), ("bs", 3, ("k", "l0", "pos")), ("bb", 4, ("k", "theta0")),
+               ("bdh", 4, ("k", "pos")), ("bp", 3, ("k", "pos")))
+
+    def check(self):
+        """Raises ValueError unless every table is consistent: a beadset of
+        width x count entries and one value per interaction in each parameter
+        array; one coordinate triple per bead; one entry per cylinder in each
+        cylinder array."""
+        if np.asarray(self.coord).size != 3 * int(self.n_beads):
+            raise ValueError(f"coord has {np.asarray(self.coord).size} entries, expected 3 * {self.n_beads}")
+        for pre, width, params in self._TABLES:
+            n = len(getattr(self, f"{pre}_k"))
+            bs = np.asarray(getattr(self, f"{pre}_beadset"))
+            if bs.size != width * n:
+                raise ValueError(f"{pre}_beadset has {bs.size} entries, expected {width} x {n}")
+            for q in params:
+                if len(getattr(self, f"{pre}_{q}")) != n:
+                    raise ValueError(f"{pre}_{q} has {len(getattr(self, f'{pre}_{q}'))} entries, expected {n}")
+        nc = self.n_cyl
+        for q in ("bead2", "id", "filament", "position", "type", "eq_length", "exvol_k", "branch_child",
+                  "minus_end"):
+            if len(getattr(self, f"cyl_{q}")) != nc:
+                raise ValueError(f"cyl_{q} has {len(getattr(self, f'cyl_{q}'))} entries, expected {nc}")
+        npl = len(self.plane_k)
+        if np.asarray(self.plane).size != 4 * npl or len(self.plane_screen) != npl:
+            raise ValueError("plane tables: 4 coefficients and one screening length per plane")
+
     def system(self) -> _abi.System:
         """Build an mdy_system pointing into this object's arrays (kept alive
-        as long as this Network lives)."""
+        as long as this Network lives). Raises ValueError on inconsistent
+        tables (see check())."""
+        self.check()
         keep = []
 
         def ptr(a, ctype, dtype):
@@ -167,16 +198,23 @@ class Network:
 
     def with_branching_dihedral(self, ff_type):
         """A copy whose branching dihedral is BRDIHEDRALFFTYPE `ff_type`
-        (BranchingFF.h:64-88): "COSINE" or "QUADRATIC". The reference's
-        COSINEV2 / QUADRATICV2 forms are not provided (NotImplementedError);
-        any other name is rejected as the reference rejects it (ValueError)."""
+        (BranchingFF.h:64-88): "COSINE" or "QUADRATIC", or "" for no
+        branching dihedral force field (the copy's dihedral table is empty).
+        The reference's COSINEV2 / QUADRATICV2 forms are not provided
+        (NotImplementedError); any other name is rejected as the reference
+        rejects it (ValueError)."""
         forms = {"COSINE": _abi.MDY_BRDIH_COSINE, "QUADRATIC": _abi.MDY_BRDIH_QUADRATIC}
         if ff_type in ("COSINEV2", "QUADRATICV2"):
             raise NotImplementedError(f"branching dihedral {ff_type} is not provided")
-        if ff_type not in forms:
+        if ff_type != "" and ff_type not in forms:
             raise ValueError(f"Branching dihedral FF {ff_type} not recognized.")
         n = self.copy()
-        n.br_dihedral_type = forms[ff_type]
+        if ff_type == "":
+            n.bdh_beadset = np.zeros(0, np.int32)
+            n.bdh_k = np.zeros(0)
+            n.bdh_pos = np.zeros(0)
+        else:
+            n.br_dihedral_type = forms[ff_type]
         return n
 
     def with_monomer_exvol(self, interval, mon_size=2.7):

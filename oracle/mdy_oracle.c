@@ -1667,7 +1667,7 @@ int orc_minimize(const mdy_cg_params *p, double *x, int32_t nvar, int32_t ndof, 
         }
     }
     r->num_iter = numIter;
-    if (call_limit_exceeded(r)) {
+    if (p->report_on_call_limit && call_limit_exceeded(r)) {   /* CGMethod.hpp:1589-1608 */
         if (s.haveMinE) { memcpy(x, s.coordMinE, sizeof(double) * (size_t)nvar); s.haveMinE = 0; }
     }
     r->energies_after[MDY_NFF] = efn(user, x, r->energies_after, 1);

@@ -5,6 +5,8 @@ forces bit for bit against the exact-mode oracle and within 1e-10 of the
 reference-order oracle, bit-identical minimization trajectories, the
 mixed-precision form, branch-point stretch forces and Hessian-vector
 products."""
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -184,3 +186,35 @@ def test_unknown_form_rejected(qnets):
     assert e.value.code == _abi.MDY_ERR_ARG
     with pytest.raises(mechanics.MdyError):
         ctx.energy(n.coord)
+
+
+def test_rejected_network_drops_the_previous_one(qnets):
+    """A context that held a network and then gets a rejected one holds no
+    network at all: the library answers MDY_ERR_STATE, never the old tables
+    with the new coordinates."""
+    good = qnets["branched_full"]
+    bad = good.copy()
+    bad.br_dihedral_type = 2
+    ctx = mechanics.Context(0)
+    ctx.set_network(good)
+    ctx.build_neighbors()
+    e_ok, _ = ctx.energy(good.coord)
+    assert np.isfinite(e_ok)
+    with pytest.raises(mechanics.MdyError, match="br_dihedral_type"):
+        ctx.set_network(bad)
+    # straight through the C ABI, with a buffer of the old network's size
+    x = np.ascontiguousarray(good.coord, dtype=np.float64)
+    tot = C.c_double()
+    rc = ctx.lib.mdy_energy(ctx.h, x.ctypes.data_as(C.POINTER(C.c_double)), C.byref(tot), None)
+    assert rc == _abi.MDY_ERR_STATE
+    p = _abi.default_cg_params(good.n_beads)
+    res = _abi.MinResult()
+    rc = ctx.lib.mdy_minimize(ctx.h, C.byref(p), x.ctypes.data_as(C.POINTER(C.c_double)), None, C.byref(res))
+    assert rc == _abi.MDY_ERR_STATE
+    with pytest.raises(mechanics.MdyError):
+        ctx.energy(good.coord)
+    # a valid network afterwards works again
+    ctx.set_network(good)
+    ctx.build_neighbors()
+    assert ctx.energy(good.coord)[0] == e_ok
+    ctx.close()

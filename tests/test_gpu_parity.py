@@ -258,6 +258,24 @@ def test_call_limit_and_line_search_flags(exact, nets):
     assert bits_equal(x2, xo2)
 
 
+@pytest.mark.parametrize("report", [1, 0])
+def test_call_limit_report_flag(exact, nets, report):
+    """report_on_call_limit reaches the device: a run stopped by the call
+    limit after taking lowest-energy backups ends with the oracle's
+    coordinates, forces and energies for either value of the flag."""
+    net = nets["net500"]
+    ctx = ctx_for(net)
+    s = exact.system(net)
+    p = _abi.default_cg_params(net.n_beads, force_call_limit=400, energy_rel_tol=0.0, grad_tol=0.05,
+                               report_on_call_limit=report)
+    p.n_beads_limit = 1
+    x, f, res = ctx.minimize(p)
+    xo, fo, reso, _ = s.minimize(p)
+    assert res.num_force_call == reso.num_force_call == 401
+    assert bits_equal(x, xo) and bits_equal(f, fo)
+    assert bits_equal(res.energies_after, reso.energies_after)
+
+
 def test_search_direction_after_minimization(exact, nets):
     """mdy_get_search_dir: CGMethod::searchDir after a converged run (the
     final shift still pending on the device), a call-limit stop and a

@@ -42,3 +42,12 @@ def test_network_from_file_minimizable_shape(lib):
     net = io.network_from_filament_file(os.path.join(GOLDEN, "filament_input_2filaments.txt"), 2000.0)
     assert net.n_cyl == 18 and net.n_beads == 20
     assert len(net.fs_k) == 18 and len(net.fb_k) == 16
+
+
+def test_half_cylinder_rounds_away_from_zero():
+    # two points 2.5 cylinders apart: std::round(2.5) = 3 cylinders (Controller.cpp:592-595),
+    # where round-half-to-even would give 2
+    b = io.filament_beads([[0, 0, 0], [0, 0, 270.0]], cyl_size=108.0)
+    assert len(b) == 4
+    assert len(io.filament_beads([[0, 0, 0], [0, 0, 162.0]], cyl_size=108.0)) == 3   # 1.5 -> 2
+    assert len(io.filament_beads([[0, 0, 0], [0, 0, 161.9]], cyl_size=108.0)) == 2

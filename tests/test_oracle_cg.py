@@ -257,3 +257,46 @@ def test_quadratic_search_on_network(oracle, lib):
     assert res.grad_inf_norm < p.grad_tol or res.energy_rel_change < p.energy_rel_tol
     assert refined_steps(tr, p) > 0
     np.testing.assert_array_equal(f, s.forces(x))
+
+
+@pytest.mark.parametrize("report", [1, 0])
+def test_report_on_call_limit_restores_the_backup(oracle, report):
+    """reportOnCallLimit (CGMethod.hpp:59, 1589-1608): when the force-call
+    limit ends a run that took a lowest-energy backup (forces within 10 x
+    gradTol after half the call budget, :1530-1533), true restores the
+    backup; false (MController.h:171, interrupted segments) keeps the last
+    coordinates. Here the energy refuses every trial once the budget is
+    nearly spent, so two line searches fail and a recovery move (not an
+    accepted step, so not backed up) precedes the limit."""
+    budget = 230
+    x0 = INITS["input_2"]
+    st = dict(nf=0, cur=None, fresh=True)
+
+    def force(x, f):
+        st["nf"] += 1
+        st["fresh"] = True
+        quad_force(x, f)
+
+    def energy(x):
+        # the line search's current point: the first call after a force call,
+        # or the same point again after a failed search
+        if st["fresh"] or (st["cur"] is not None and np.array_equal(x, st["cur"])):
+            st["cur"], st["fresh"] = np.array(x), False
+            return quad_energy(x)
+        return 1e300 if st["nf"] >= budget else quad_energy(x)
+
+    moved = {}
+
+    def recover(x, d, maxdist):
+        with np.errstate(divide="ignore"):
+            step = np.min(maxdist / np.abs(d))
+        moved["before"] = x.copy()
+        x[:] += step * d
+        moved["after"] = x.copy()
+
+    p = quad_params(_abi.MDY_CG_POLAK_RIBIERE, grad_tol=10.0, energy_rel_tol=0.0, try_to_recover=1,
+                    force_call_limit=budget, n_beads_limit=1, report_on_call_limit=report)
+    x, f, res, _ = oracle.minimize(p, x0, 10, energy, force, recover=recover)
+    assert res.num_force_call == budget + 1 and res.num_recovery == 1
+    assert not np.array_equal(moved["before"], moved["after"])
+    np.testing.assert_array_equal(x, moved["before"] if report else moved["after"])
