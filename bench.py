@@ -14,8 +14,11 @@ Prints one JSON line. The roofline block is for the persistent CG kernel
 counts the minimization reports, over the launch's duration measured with HIP
 events on the stream it runs on. The CPU baseline is the oracle (a C
 restatement of the reference minimizer, built with the reference's release
-flags) timed single-threaded on a bounded number of CG iterations and scaled
-to minimizations/s with the GPU run's iteration count per minimization.
+flags), single-threaded: its seconds per energy evaluation, per force
+evaluation and per CG iteration for the rest are measured (the last on a
+bounded sample of the same minimization) and applied to the energy calls,
+force calls and iterations of the GPU's minimization. `--cpu-validate` checks
+that model against one full CPU minimization (no GPU needed).
 
 Other workloads (--workload): `ensemble256`, 256 independent networks
 minimized by one batched launch (networks/s, per-network p50/p99); `net200k`,
@@ -48,6 +51,8 @@ def parse():
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget (0: skip)")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false", help="time the FP64 form only")
+    ap.add_argument("--cpu-validate", action="store_true",
+                    help="CPU only: one full oracle minimization of --workload against the CPU cost model")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -70,6 +75,15 @@ def algorithmic_bytes(net, n_ev, n_bd, res):
             + 8 * ndof * (7 * res.num_iter + 3 * trials))
 
 
+def generator_rule(lib):
+    """The seeded generator's placement rule, stated with the workload."""
+    from medyan_amd import network
+    g = network.gen_params(lib)
+    return (f"generator: filament start uniform in the box, direction uniform on the sphere, a placement "
+            f"rejected if a segment comes within {g.min_sep:g} nm of a placed one (avoids singular "
+            f"excluded-volume overlaps), then N(0, {g.jitter:g} nm) bead jitter")
+
+
 def traffic_from_profiles(workload):
     """HBM bytes per k_cg launch from the committed counter summary, if any."""
     for path in sorted(glob.glob(os.path.join(HERE, "profiles", f"k_cg_{workload}_traffic*.md"))):
@@ -80,9 +94,24 @@ def traffic_from_profiles(workload):
     return None
 
 
-def cpu_baseline(net, gpu_iters, budget_s, name=WORKLOAD):
-    """The oracle minimizer (reference release flags), one thread, on the same
-    network for a bounded number of iterations."""
+def _timed(fn, budget_s):
+    """Seconds per call of fn, repeated for about budget_s."""
+    fn()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            return dt / n
+
+
+def cpu_cost_model(net, budget_s):
+    """The oracle minimizer (reference release flags, reference-order
+    arithmetic), one thread: seconds per energy evaluation and per force
+    evaluation timed at the start coordinates, and the rest of a bounded
+    sample minimization (line-search vector passes, CG updates) per CG
+    iteration. Returns (model, sample result, sample seconds)."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle
     from medyan_amd import _abi
@@ -90,21 +119,77 @@ def cpu_baseline(net, gpu_iters, budget_s, name=WORKLOAD):
     o = pyoracle.Oracle(fast=True)
     o.set_exact(0)
     s = o.system(net)
+    t_e = _timed(lambda: s.energy(), min(1.0, 0.1 * budget_s))
+    t_f = _timed(lambda: s.forces(), min(1.0, 0.1 * budget_s))
     p = _abi.default_cg_params(net.n_beads, energy_rel_tol=0.0)
     t0 = time.perf_counter()
-    _, _, r, _ = s.minimize(p, iter_cap=10)
-    dt = time.perf_counter() - t0
-    per_iter = dt / max(r.num_iter, 1)
-    n = max(10, int(budget_s / per_iter))
+    _, _, r, _ = s.minimize(p, iter_cap=5)
+    per_iter = (time.perf_counter() - t0) / max(r.num_iter, 1)
+    n = max(10, int(0.8 * budget_s / per_iter))
     t0 = time.perf_counter()
     _, _, r, _ = s.minimize(p, iter_cap=n)
     dt = time.perf_counter() - t0
-    per_iter = dt / max(r.num_iter, 1)
-    return dict(value=1.0 / (per_iter * gpu_iters), unit="minimizations/s", cores=1, kind="port",
-                sample=(f"{name}: first {r.num_iter} CG iterations of the oracle minimizer "
-                        f"(-O2 -ffast-math, 1 thread) in {dt:.1f} s = {per_iter * 1e3:.2f} ms/iteration, "
-                        f"scaled by the GPU's {gpu_iters} iterations per minimization; "
+    t_v = max(0.0, dt - r.num_energy_call * t_e - r.num_force_call * t_f) / max(r.num_iter, 1)
+    return dict(t_energy=t_e, t_force=t_f, t_iter_rest=t_v), r, dt
+
+
+def cpu_model_seconds(m, n_energy_call, n_force_call, n_iter):
+    return n_energy_call * m["t_energy"] + n_force_call * m["t_force"] + n_iter * m["t_iter_rest"]
+
+
+def cpu_baseline(net, gpu_res, budget_s, name=WORKLOAD):
+    """CPU minimizations/s for the GPU's minimization: the cost model of
+    cpu_cost_model applied to the energy calls, force calls and iterations the
+    reference's algorithm makes on that path (the GPU's counts in the
+    reference's convention: every line search recomputes E(x))."""
+    m, r, dt = cpu_cost_model(net, budget_s)
+    secs = cpu_model_seconds(m, gpu_res.num_energy_call, gpu_res.num_force_call, gpu_res.num_iter)
+    return dict(value=1.0 / secs, unit="minimizations/s", cores=1, kind="port",
+                sample=(f"{name}: oracle minimizer (-O2 -ffast-math, 1 thread): energy evaluation "
+                        f"{m['t_energy'] * 1e3:.2f} ms, force evaluation {m['t_force'] * 1e3:.2f} ms, "
+                        f"{m['t_iter_rest'] * 1e3:.2f} ms other work per CG iteration (from the first "
+                        f"{r.num_iter} iterations, {dt:.1f} s); applied to the GPU run's "
+                        f"{gpu_res.num_energy_call} energy calls, {gpu_res.num_force_call} force calls, "
+                        f"{gpu_res.num_iter} iterations = {secs:.0f} s per minimization; "
                         f"host {_cpu_model()}, 1 of {os.cpu_count()} cores"))
+
+
+def _mean_counts(results):
+    """Mean energy calls, force calls and iterations of several minimizations."""
+    from types import SimpleNamespace
+    import numpy as np
+    return SimpleNamespace(**{k: int(round(np.mean([getattr(r, k) for r in results])))
+                              for k in ("num_energy_call", "num_force_call", "num_iter")})
+
+
+def cpu_validate(workload, budget_s):
+    """A full CPU minimization (oracle, reference release flags, one thread)
+    against the cost model's estimate from a bounded sample of the same
+    minimization, with the CPU run's own counts. No GPU needed."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle
+    from medyan_amd import _abi, mechanics, network
+
+    lib = mechanics.load_library()
+    net = network.generate(lib, workload)
+    m, r_s, dt_s = cpu_cost_model(net, budget_s)
+    o = pyoracle.Oracle(fast=True)
+    o.set_exact(0)
+    s = o.system(net)
+    p = _abi.default_cg_params(net.n_beads, energy_rel_tol=0.0)
+    t0 = time.perf_counter()
+    _, _, r, _ = s.minimize(p)
+    full = time.perf_counter() - t0
+    est = cpu_model_seconds(m, r.num_energy_call, r.num_force_call, r.num_iter)
+    naive = dt_s / max(r_s.num_iter, 1) * r.num_iter
+    print(json.dumps({
+        "workload": workload, "host": _cpu_model(), "threads": 1,
+        "full_seconds": full, "model_seconds": est, "model_over_full": est / full,
+        "per_iteration_scaling_seconds": naive, "per_iteration_over_full": naive / full,
+        "num_iter": int(r.num_iter), "num_energy_call": int(r.num_energy_call),
+        "num_force_call": int(r.num_force_call), "grad_inf_norm": float(r.grad_inf_norm),
+        "status": int(r.status), "model": m, "sample_iterations": int(r_s.num_iter), "sample_seconds": dt_s,
+    }))
 
 
 def cpu_baseline_ensemble(nets, iters, budget_s):
@@ -160,6 +245,8 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}): the single-network CG path "
               "runs on one device; multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if a.cpu_validate:
+        return cpu_validate(a.workload, a.cpu_seconds)
     import numpy as np
     import torch
     from medyan_amd import _abi, mechanics, network
@@ -259,7 +346,7 @@ def main():
             "workload": (f"{a.workload}: {net.n_cyl} cylinders, {net.n_beads} beads, "
                          f"{n_ev} excluded-volume pairs, {n_bd} boundary terms; PR-CG + backtracking, "
                          "gradTol 1 pN, maxDist 1, lambdaMax 0.01, energy criterion off; "
-                         "neighbor-list rebuild + minimization per step"),
+                         "neighbor-list rebuild + minimization per step; " + generator_rule(lib)),
             "precision": "fp64",
             "cg_iterations": int(r.num_iter),
             "us_per_cg_iteration": 1e3 * k_ms / max(int(r.num_iter), 1),
@@ -283,7 +370,7 @@ def main():
         "cpu_baseline": None,
     }
     if a.cpu_seconds > 0:
-        out["cpu_baseline"] = cpu_baseline(net, int(r.num_iter), a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(net, r, a.cpu_seconds)
     print(json.dumps(out))
 
 
@@ -376,7 +463,7 @@ def bench_ensemble(a, lib):
             "workload": (f"ensemble256: 256 networks, {int(ncyl.sum())} cylinders in total "
                          f"({int(ncyl.min())}..{int(ncyl.max())} per network, log-uniform), 60 cylinders per "
                          "um^3, 0.5 linkers and 0.05 motors per cylinder; PR-CG + backtracking to 1 pN, "
-                         "energy criterion off; tables and neighbor lists resident"),
+                         "energy criterion off; tables and neighbor lists resident; " + generator_rule(lib)),
             "threads_per_block": tpb,
             "first_pass_force_calls": first,
             "autotune_ms": {f"{k[0]} threads, first pass {k[1]}": round(v, 1) for k, v in tune.items()},
@@ -496,7 +583,7 @@ def bench_successive(a, lib):
             "workload": (f"net200k: {net.n_cyl} cylinders, {net.n_beads} beads, {len(net.ls_k)} linkers, "
                          f"{len(net.ms_k)} motors, {len(net.bs_k)} branch points, {n_ev} excluded-volume pairs "
                          "at the last rebuild; each step: +N(0, 5 nm) bead displacement of the previous minimum, "
-                         "neighbor-list rebuild, PR-CG to 1 pN"),
+                         "neighbor-list rebuild, PR-CG to 1 pN; " + generator_rule(lib)),
             "mixed_cg_iterations": [int(r.num_iter) for r in m_res],
             "fp64_cg_iterations": [int(r.num_iter) for r in f_res],
             "mixed_ms_per_step": float(np.mean(m_step)),
@@ -523,7 +610,7 @@ def bench_successive(a, lib):
             "algorithmic_bytes_per_launch": float(np.mean(m_bytes)),
             "launch_ms": float(np.mean(m_cg)),
         },
-        "cpu_baseline": (cpu_baseline(net, int(np.mean([r.num_iter for r in f_res])), a.cpu_seconds, "net200k")
+        "cpu_baseline": (cpu_baseline(net, _mean_counts(f_res), a.cpu_seconds, "net200k")
                          if a.cpu_seconds > 0 else None),
     }
     print(json.dumps(out))

@@ -320,27 +320,42 @@ extern "C" int mdy_generate(const mdy_gen_params *p, mdy_network **out) {
         mkey(i, c);
         mgrid[SegGrid::key(c[0], c[1], c[2])].push_back(i);
     }
+    // the cylinders of the 27 cells around each occupied cell, in scan order,
+    // built once per cell (the candidates of a first cylinder are this list
+    // without its own filament's cylinders)
+    std::unordered_map<int64_t, int> uid;
+    std::vector<int> u_off(1, 0), u_list, cyl_u(ncyl_tot);
+    for (int i = 0; i < ncyl_tot; i++) {
+        int64_t c[3];
+        mkey(i, c);
+        const int64_t exact = ((c[0] + (1 << 20)) << 42) | ((c[1] + (1 << 20)) << 21) | (c[2] + (1 << 20));
+        auto found = uid.find(exact);
+        if (found != uid.end()) { cyl_u[i] = found->second; continue; }
+        for (int64_t a = -1; a <= 1; a++)
+            for (int64_t b = -1; b <= 1; b++)
+                for (int64_t d = -1; d <= 1; d++) {
+                    auto it = mgrid.find(SegGrid::key(c[0] + a, c[1] + b, c[2] + d));
+                    if (it != mgrid.end()) u_list.insert(u_list.end(), it->second.begin(), it->second.end());
+                }
+        u_off.push_back((int)u_list.size());
+        cyl_u[i] = uid[exact] = (int)u_off.size() - 2;
+    }
     auto make_links = [&](int count, double kval, std::vector<int32_t> &bs, std::vector<double> &kk,
                           std::vector<double> &l0, std::vector<double> &p1v, std::vector<double> &p2v) -> bool {
-        std::vector<int> cand;
         for (int l = 0; l < count; l++) {
             long tries = 0;
             for (;;) {
                 if (++tries > max_tries) return false;
                 int c1 = rng.below(ncyl_tot);
-                int64_t c[3];
-                mkey(c1, c);
-                cand.clear();
-                for (int64_t i = -1; i <= 1; i++)
-                    for (int64_t j = -1; j <= 1; j++)
-                        for (int64_t k = -1; k <= 1; k++) {
-                            auto it = mgrid.find(SegGrid::key(c[0] + i, c[1] + j, c[2] + k));
-                            if (it == mgrid.end()) continue;
-                            for (int id : it->second)
-                                if (st->cfil[id] != st->cfil[c1]) cand.push_back(id);
-                        }
-                if (cand.empty()) continue;
-                int c2 = cand[rng.below((int)cand.size())];
+                const int *U = u_list.data() + u_off[cyl_u[c1]];
+                const int nU = u_off[cyl_u[c1] + 1] - u_off[cyl_u[c1]];
+                const int f1 = st->cfil[c1];
+                int nvalid = 0;
+                for (int q = 0; q < nU; q++) nvalid += st->cfil[U[q]] != f1;
+                if (nvalid == 0) continue;
+                int r = rng.below(nvalid), c2 = -1;
+                for (int q = 0; q < nU; q++)
+                    if (st->cfil[U[q]] != f1 && r-- == 0) { c2 = U[q]; break; }
                 double q1 = kSites[rng.below(4)], q2 = kSites[rng.below(4)];
                 double v1[3], v2[3];
                 for (int k = 0; k < 3; k++) {

@@ -1189,6 +1189,88 @@ int64_t orc_nl_pairs(const mdy_cylinders *cyl, const double *x, float rmax, floa
     return cnt;
 }
 
+/* The same pair set and order with candidates pruned by a uniform grid of
+ * cells at least 1.01 rMax wide (midpoints of a listed pair differ by at most
+ * rMax per axis, so their cells are neighbours); the predicate is the one
+ * above, evaluated in the same order. For large networks, where the O(n^2)
+ * loop is too slow for tests. */
+static int cmp_i32(const void *a, const void *b) {
+    int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return (x > y) - (x < y);
+}
+
+int64_t orc_nl_pairs_grid(const mdy_cylinders *cyl, const double *x, float rmax, float rmin, int32_t *pairs,
+                          int64_t cap) {
+    float rmaxsq = rmax * rmax, rminsq = rmin * rmin;
+    int32_t n = cyl->n;
+    if (n <= 0) return 0;
+    double *mid = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, h[3];
+    int64_t nc[3];
+    for (int32_t i = 0; i < n; i++) {
+        cyl_mid(cyl, x, i, mid + 3 * i);
+        for (int k = 0; k < 3; k++) { lo[k] = smin(lo[k], mid[3 * i + k]); hi[k] = smax(hi[k], mid[3 * i + k]); }
+    }
+    for (int k = 0; k < 3; k++) {
+        h[k] = smax(1.01 * (double)rmax, (hi[k] - lo[k]) / 512.0);
+        if (!(h[k] > 0)) h[k] = 1.0;
+        nc[k] = (int64_t)((hi[k] - lo[k]) / h[k]) + 1;
+    }
+    const int64_t ncell = nc[0] * nc[1] * nc[2];
+    int64_t *cell = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int32_t *start = (int32_t *)calloc((size_t)ncell + 1, sizeof(int32_t));
+    int32_t *list = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t *cand = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    int64_t *ci3 = (int64_t *)malloc(sizeof(int64_t) * 3 * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            int64_t c = (int64_t)((mid[3 * i + k] - lo[k]) / h[k]);
+            if (c >= nc[k]) c = nc[k] - 1;
+            ci3[3 * i + k] = c;
+        }
+        cell[i] = (ci3[3 * i] * nc[1] + ci3[3 * i + 1]) * nc[2] + ci3[3 * i + 2];
+        start[cell[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+    {
+        int32_t *cur = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncell);
+        memcpy(cur, start, sizeof(int32_t) * (size_t)ncell);
+        for (int32_t i = 0; i < n; i++) list[cur[cell[i]]++] = i;
+        free(cur);
+    }
+    int64_t cnt = 0;
+    for (int32_t i = 0; i < n; i++) {
+        int32_t nk = 0;
+        for (int64_t dx = -1; dx <= 1; dx++)
+            for (int64_t dy = -1; dy <= 1; dy++)
+                for (int64_t dz = -1; dz <= 1; dz++) {
+                    int64_t a = ci3[3 * i] + dx, b = ci3[3 * i + 1] + dy, c = ci3[3 * i + 2] + dz;
+                    if (a < 0 || b < 0 || c < 0 || a >= nc[0] || b >= nc[1] || c >= nc[2]) continue;
+                    int64_t cc = (a * nc[1] + b) * nc[2] + c;
+                    for (int32_t q = start[cc]; q < start[cc + 1]; q++) {
+                        int32_t j = list[q];
+                        if (cyl->id[i] <= cyl->id[j]) continue;
+                        if (cyl->filament[i] == cyl->filament[j]) {
+                            int ds = abs(cyl->position[i] - cyl->position[j]);
+                            if (ds <= 2) continue;
+                        }
+                        const double *m1 = mid + 3 * i, *m2 = mid + 3 * j;
+                        double d2 = 0;
+                        for (int k = 0; k < 3; k++) d2 += (m2[k] - m1[k]) * (m2[k] - m1[k]);
+                        if (d2 < rminsq || d2 > rmaxsq) continue;
+                        cand[nk++] = j;
+                    }
+                }
+        qsort(cand, (size_t)nk, sizeof(int32_t), cmp_i32);
+        for (int32_t q = 0; q < nk; q++) {
+            if (cnt < cap) { pairs[2 * cnt] = i; pairs[2 * cnt + 1] = cand[q]; }
+            cnt++;
+        }
+    }
+    free(mid); free(cell); free(start); free(list); free(cand); free(ci3);
+    return cnt;
+}
+
 /* CylinderExclVolume::vectorize (Volume/CylinderExclVolume.cpp:34-179):
  * drop pairs whose neighbour is the mother of the cylinder
  * (cin->getBranchingCylinder() == ci); k is the max over types when they
@@ -1271,7 +1353,7 @@ int orc_system_setup(orc_system *o, const mdy_system *sys, const double *x) {
         int64_t np;
         for (;;) {
             pairs = (int32_t *)realloc(pairs, sizeof(int32_t) * 2 * (size_t)cap);
-            np = orc_nl_pairs(cyl, x, sys->exvol_cutoff, 0.0f, pairs, cap);
+            np = orc_nl_pairs_grid(cyl, x, sys->exvol_cutoff, 0.0f, pairs, cap);
             if (np <= cap) break;
             cap = np;
         }

@@ -103,6 +103,10 @@ void   orc_bd_forces(const double *x, double *f, int32_t nplanes, const double *
  * a negative capacity error. Mother-daughter pairs are NOT dropped here. */
 int64_t orc_nl_pairs(const mdy_cylinders *cyl, const double *x, float rmax, float rmin,
                      int32_t *pairs, int64_t cap);
+/* The same pairs in the same order, candidates pruned by a cell grid (fast at
+ * 200k cylinders); equal to orc_nl_pairs (tested). Used by orc_system_setup. */
+int64_t orc_nl_pairs_grid(const mdy_cylinders *cyl, const double *x, float rmax, float rmin,
+                          int32_t *pairs, int64_t cap);
 /* CylinderExclVolume::vectorize over the pair list; returns interactions. */
 int64_t orc_ev_vectorize(const mdy_cylinders *cyl, const int32_t *pairs, int64_t npairs,
                          int32_t *bs, double *krep, double *eql);

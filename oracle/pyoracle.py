@@ -64,6 +64,7 @@ class Oracle:
                    "orc_bd_energy", "orc_bdh_energy", "orc_bdhq_energy", "orc_bp_energy"):
             getattr(L, fn).restype = C.c_double
         L.orc_nl_pairs.restype = C.c_int64
+        L.orc_nl_pairs_grid.restype = C.c_int64
         L.orc_ev_vectorize.restype = C.c_int64
         L.orc_bd_vectorize.restype = C.c_int64
         L.orc_minimize.argtypes = [C.POINTER(_abi.CGParams), _f64p, C.c_int32, C.c_int32, ENERGY_FN, FORCE_FN,
@@ -212,15 +213,18 @@ class Oracle:
                                _p(krep), _p(slen))
         return f
 
-    def nl_pairs(self, net, coord=None, rmax=None, rmin=0.0):
+    def nl_pairs(self, net, coord=None, rmax=None, rmin=0.0, brute=False):
+        """Neighbor pairs; brute=True runs the O(n^2) loop, otherwise the
+        grid-pruned form (same pairs, same order)."""
         coord = net.coord if coord is None else np.ascontiguousarray(coord, np.float64)
         sysc = net.system()
         rmax = net.exvol_cutoff if rmax is None else rmax
         cap = 4 * net.n_cyl + 16
+        fn = self.lib.orc_nl_pairs if brute else self.lib.orc_nl_pairs_grid
         while True:
             out = np.zeros(2 * cap, dtype=np.int32)
-            n = self.lib.orc_nl_pairs(C.byref(sysc.cyl), _p(coord), C.c_float(rmax), C.c_float(rmin),
-                                      _p(out, C.c_int32), C.c_int64(cap))
+            n = fn(C.byref(sysc.cyl), _p(coord), C.c_float(rmax), C.c_float(rmin), _p(out, C.c_int32),
+                   C.c_int64(cap))
             if n <= cap:
                 return out[: 2 * n].reshape(-1, 2)
             cap = n

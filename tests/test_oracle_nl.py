@@ -32,10 +32,30 @@ def np_pairs(net, coord, rmax):
                                 dict(n_filaments=30, cyl_per_filament=3, box=500.0, n_linkers=20, n_motors=2, seed=3)])
 def test_nl_brute_force_matches_numpy(oracle, lib, kw):
     net = network.generate(lib, "net500", **kw)
-    got = oracle.nl_pairs(net)
     ref = np_pairs(net, net.coord, net.exvol_cutoff)
-    np.testing.assert_array_equal(got, ref)
-    assert len(got) > 0
+    np.testing.assert_array_equal(oracle.nl_pairs(net, brute=True), ref)
+    np.testing.assert_array_equal(oracle.nl_pairs(net), ref)          # the grid-pruned form
+    assert len(ref) > 0
+
+
+def test_nl_grid_equals_brute_force(oracle, lib):
+    """The grid-pruned pair list (used for large networks) equals the O(n^2)
+    one: at net40k, and on midpoints placed at exactly the cutoff distance
+    along each axis and across cell boundaries."""
+    net = network.generate(lib, "net40k")
+    np.testing.assert_array_equal(oracle.nl_pairs(net), oracle.nl_pairs(net, brute=True))
+    # cylinders whose midpoints sit on a lattice of spacing = cutoff (float):
+    # pairs at exactly rmax are kept (d2 <= rmax^2), diagonal ones dropped
+    r = float(np.float32(108.0))
+    fils = []
+    for i in range(4):
+        for j in range(3):
+            c = np.array([100.0 + i * r, 100.0 + j * r, 300.0])
+            fils.append(np.array([c - [0, 0, 20.0], c + [0, 0, 20.0]]))
+    lat = network.from_filaments(fils, 2000.0)
+    g, b = oracle.nl_pairs(lat), oracle.nl_pairs(lat, brute=True)
+    np.testing.assert_array_equal(g, b)
+    assert len(b) == 4 * 2 + 3 * 3                  # the axis-neighbour pairs of a 4 x 3 lattice
 
 
 def test_ev_table_drops_mother_daughter(oracle, lib):
