@@ -262,6 +262,19 @@ int mdy_forces(mdy_ctx *ctx, const double *coord, double *force_accum);
  * the context stream). d_out receives MDY_NFF energies then the total. */
 int mdy_energy_dev(mdy_ctx *ctx, const double *d_coord, double *d_out, void *stream);
 int mdy_forces_dev(mdy_ctx *ctx, const double *d_coord, double *d_force, int accumulate, void *stream);
+/* mdy_energy_dev without any allocation: no hipMalloc, hipFree or allocating
+ * library call on the call path. No reference counterpart; for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops). The
+ * block partials, the quadrature-fallback queue and its counters, the
+ * per-pair quadrature energies and the error word live in one static device
+ * array of about 580 KB per GPU (2048 block records of 256 B, 4096 queue
+ * entries), held until the process ends and shared by every call on that
+ * GPU: calls that overlap on one GPU race. The counters are cleared by a
+ * kernel at the start of each call. The context's network and neighbor list
+ * must already be built (those calls allocate). Returns MDY_ERR_CAPACITY if
+ * the launch would need more than 2048 block records. Results equal
+ * mdy_energy_dev's. */
+int mdy_energy_dev_noalloc(mdy_ctx *ctx, const double *d_coord, double *d_out, void *stream);
 /* Hessian-vector product H v = d/dt grad E(x + t v) at t = 0 of the
  * exact-gradient force field (no coplanar perturbation); writes H v. */
 int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *d_hv, void *stream);

@@ -134,6 +134,10 @@ int mdy_create(mdy_ctx **out, int device, const mdy_opts *opts) {
     }
     (void)hipMemsetAsync(c.err.p, 0, 4 * sizeof(int), c.stream);
     (void)hipMemsetAsync(c.qn.p, 0, 16 * sizeof(int), c.stream);
+    if (resolve_noalloc_scratch(device) != MDY_OK) {
+        mdy_destroy(ctx);
+        return MDY_ERR_HIP;
+    }
     *out = ctx;
     return MDY_OK;
 }
@@ -440,6 +444,20 @@ int mdy_energy_dev(mdy_ctx *ctx, const double *d_coord, double *d_out, void *str
     // d_out: MDY_NFF energies then the total (sentinel form)
     if (hipMemcpyAsync(d_out, c.out.p, sizeof(double) * (MDY_NFF + 1), hipMemcpyDeviceToDevice, st) != hipSuccess)
         return hipfail(ctx, "energy copy");
+    return MDY_OK;
+}
+
+int mdy_energy_dev_noalloc(mdy_ctx *ctx, const double *d_coord, double *d_out, void *stream) {
+    if (!ctx || !d_coord || !d_out) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.have_sys || !c.have_nl) return fail(ctx, MDY_ERR_STATE, "network / neighbor list missing");
+    (void)hipSetDevice(c.device);
+    hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+    const int rc = launch_energy_noalloc(c, d_coord, d_out, st);
+    if (rc == MDY_ERR_CAPACITY)
+        return fail(ctx, rc, "the grid needs more block records than the static scratch holds (2048)");
+    if (rc == MDY_ERR_STATE) return fail(ctx, rc, "static scratch not resolved for this device");
+    if (rc != MDY_OK) return hipfail(ctx, "energy (no allocation)");
     return MDY_OK;
 }
 

@@ -1,6 +1,8 @@
 // mdy_eval.hip -- one-shot energy / force evaluation kernels behind
 // mdy_energy(_dev) and mdy_forces(_dev) (ForceFieldManager::computeEnergy and
 // ::computeForces), including the excluded-volume quadrature fallbacks.
+#include <mutex>
+
 #include "mdy_eval.hpp"
 #include "mdy_launch.hpp"
 
@@ -48,13 +50,15 @@ __global__ void __launch_bounds__(kBlock) k_energy_simpson(Dev D, P pos, double 
     }
 }
 
+// out: the kNFF energies, the total (+inf for an invalid pair), and when
+// n_out = kNFF + 2 the total without the sentinel
 __global__ void __launch_bounds__(kBlock) k_energy_final(Dev D, const double *red, int nblk, const double *qe_u,
-                                                         double *out) {
+                                                         double *out, int n_out) {
     __shared__ double lds[17 * kNFF + 8];
     double e[kNFF + 2];
     energy_epilogue(D, red, nblk, *D.qe_n, D.qe, qe_u, lds, e, true);
     if (threadIdx.x == 0)
-        for (int k = 0; k < kNFF + 2; k++) out[k] = e[k];
+        for (int k = 0; k < n_out; k++) out[k] = e[k];
 }
 
 template <bool MIXED, class P>
@@ -166,7 +170,71 @@ int launch_energy(Ctx &c, const double *d_x, double *d_out, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c.qn.p, 4);
     hipLaunchKernelGGL(k_energy<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.red.p);
     hipLaunchKernelGGL(k_energy_simpson<PosX>, dim3(64), dim3(kBlock), 0, st, D, PosX{d_x}, c.qe_u.p);
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, D, c.red.p, nblk, c.qe_u.p, d_out);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, D, c.red.p, nblk, c.qe_u.p, d_out, kNFF + 2);
+    return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free energy evaluation (mdy_energy_dev_noalloc): the same
+// kernels as launch_energy, with every piece of scratch memory (block
+// partials, fallback queue and its counters, the per-pair quadrature
+// energies, the error word) in one file-scope device array. Each device holds
+// its own copy; host code reaches it only through the address that
+// hipGetSymbolAddress returns for that device.
+// ---------------------------------------------------------------------------
+constexpr int kScratchBlocks = 2048;   // block records: 8 blocks per CU on a 256-CU device
+constexpr size_t kScRed = 0;                                                   // kRed doubles per block
+constexpr size_t kScQn = kScRed + sizeof(double) * kRed * kScratchBlocks;     // 16 ints: queue counters
+constexpr size_t kScErr = kScQn + 256;                                         // 4 ints: error word
+constexpr size_t kScQe = kScErr + 256;                                         // energy fallback queue
+constexpr size_t kScQf = kScQe + sizeof(int) * kQueueCap;                      // force queue (unused here)
+constexpr size_t kScQeU = kScQf + sizeof(int) * kQueueCap;                     // quadrature energies
+constexpr size_t kScratchBytes = kScQeU + sizeof(double) * kQueueCap;
+static_assert(kScQn % 256 == 0 && kScQe % 256 == 0 && kScQeU % 256 == 0, "scratch alignment");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices];
+
+// Called by mdy_create with `device` current: resolves this device's copy once.
+int resolve_noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return MDY_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MDY_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MDY_ERR_HIP;
+    g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+    return MDY_OK;
+}
+
+size_t noalloc_scratch_bytes() { return kScratchBytes; }
+
+int launch_energy_noalloc(Ctx &c, const double *d_x, double *d_out, hipStream_t st) {
+    unsigned char *sc = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c.device >= 0 && c.device < kMaxDevices) sc = g_scratch_ptr[c.device];
+    }
+    if (!sc) return MDY_ERR_STATE;
+    Dev D = c.dev();
+    const int nblk = grid_for(c, D.n_items());
+    if (nblk > kScratchBlocks) return MDY_ERR_CAPACITY;
+    double *red = reinterpret_cast<double *>(sc + kScRed);
+    int *qn = reinterpret_cast<int *>(sc + kScQn);
+    D.qe = reinterpret_cast<int *>(sc + kScQe);
+    D.qe_n = qn;
+    D.qf = reinterpret_cast<int *>(sc + kScQf);
+    D.qf_n = qn + 1;
+    D.err = reinterpret_cast<int *>(sc + kScErr);
+    double *qe_u = reinterpret_cast<double *>(sc + kScQeU);
+    // the counters and the error word keep the previous call's values: cleared by a kernel
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, qn, 4);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, D.err, 4);
+    hipLaunchKernelGGL(k_energy<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, red);
+    hipLaunchKernelGGL(k_energy_simpson<PosX>, dim3(64), dim3(kBlock), 0, st, D, PosX{d_x}, qe_u);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, D, red, nblk, qe_u, d_out, kNFF + 1);
     return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
 }
 

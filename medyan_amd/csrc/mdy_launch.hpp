@@ -7,6 +7,10 @@ namespace mdy {
 // mdy_eval.hip
 int launch_energy(Ctx &c, const double *d_x, double *d_out, hipStream_t st);
 int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hipStream_t st);
+// allocation-free energy: scratch in a file-scope device array of mdy_eval.hip
+int resolve_noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
+int launch_energy_noalloc(Ctx &c, const double *d_x, double *d_out, hipStream_t st);
 // mdy_nl.hip
 int build_neighbors(Ctx &c, const double *d_x);
 int exclusive_scan(Ctx &c, const int *in, int *out, int n, hipStream_t st);  // out has n+1 entries

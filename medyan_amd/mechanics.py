@@ -63,6 +63,7 @@ def load_library(path: str = LIB_PATH):
         "mdy_forces": (C.c_int, [vp, f64p, f64p]),
         "mdy_energy_dev": (C.c_int, [vp, vp, vp, vp]),
         "mdy_forces_dev": (C.c_int, [vp, vp, vp, C.c_int, vp]),
+        "mdy_energy_dev_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "mdy_hvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "mdy_energy_jvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "mdy_minimize": (C.c_int, [vp, C.POINTER(_abi.CGParams), f64p, f64p, C.POINTER(_abi.MinResult)]),
@@ -283,6 +284,12 @@ class Context:
         """d_out: MDY_NFF per-FF energies then the computeEnergy total."""
         self._check(self.lib.mdy_energy_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_out),
                                             C.c_void_p(stream) if stream else None))
+
+    def energy_dev_noalloc(self, d_coord: int, d_out: int, stream: int = 0):
+        """energy_dev without allocation (scratch in a static device array
+        shared by every call on the GPU; calls that overlap on one GPU race)."""
+        self._check(self.lib.mdy_energy_dev_noalloc(self.h, C.c_void_p(d_coord), C.c_void_p(d_out),
+                                                    C.c_void_p(stream) if stream else None))
 
     def forces_dev(self, d_coord: int, d_force: int, accumulate=True, stream: int = 0):
         self._check(self.lib.mdy_forces_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_force), 1 if accumulate else 0,
