@@ -108,10 +108,12 @@ def _timed(fn, budget_s):
 
 def cpu_cost_model(net, budget_s):
     """The oracle minimizer (reference release flags, reference-order
-    arithmetic), one thread: seconds per energy evaluation and per force
-    evaluation timed at the start coordinates, and the rest of a bounded
-    sample minimization (line-search vector passes, CG updates) per CG
-    iteration. Returns (model, sample result, sample seconds)."""
+    arithmetic), one thread, on a bounded sample of the minimization: its
+    seconds per evaluation unit, where an energy evaluation is one unit and a
+    force evaluation c units (c = the ratio of the two, timed separately at
+    the start coordinates); the line-search vector passes and CG updates of
+    the sample are spread over its evaluations. Returns (model, sample
+    result, sample seconds)."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle
     from medyan_amd import _abi
@@ -129,12 +131,13 @@ def cpu_cost_model(net, budget_s):
     t0 = time.perf_counter()
     _, _, r, _ = s.minimize(p, iter_cap=n)
     dt = time.perf_counter() - t0
-    t_v = max(0.0, dt - r.num_energy_call * t_e - r.num_force_call * t_f) / max(r.num_iter, 1)
-    return dict(t_energy=t_e, t_force=t_f, t_iter_rest=t_v), r, dt
+    c = t_f / t_e
+    unit = dt / (r.num_energy_call + c * r.num_force_call)
+    return dict(force_over_energy=c, seconds_per_unit=unit), r, dt
 
 
-def cpu_model_seconds(m, n_energy_call, n_force_call, n_iter):
-    return n_energy_call * m["t_energy"] + n_force_call * m["t_force"] + n_iter * m["t_iter_rest"]
+def cpu_model_seconds(m, n_energy_call, n_force_call, n_iter=None):
+    return m["seconds_per_unit"] * (n_energy_call + m["force_over_energy"] * n_force_call)
 
 
 def cpu_baseline(net, gpu_res, budget_s, name=WORKLOAD):
@@ -145,10 +148,11 @@ def cpu_baseline(net, gpu_res, budget_s, name=WORKLOAD):
     m, r, dt = cpu_cost_model(net, budget_s)
     secs = cpu_model_seconds(m, gpu_res.num_energy_call, gpu_res.num_force_call, gpu_res.num_iter)
     return dict(value=1.0 / secs, unit="minimizations/s", cores=1, kind="port",
-                sample=(f"{name}: oracle minimizer (-O2 -ffast-math, 1 thread): energy evaluation "
-                        f"{m['t_energy'] * 1e3:.2f} ms, force evaluation {m['t_force'] * 1e3:.2f} ms, "
-                        f"{m['t_iter_rest'] * 1e3:.2f} ms other work per CG iteration (from the first "
-                        f"{r.num_iter} iterations, {dt:.1f} s); applied to the GPU run's "
+                sample=(f"{name}: oracle minimizer (-O2 -ffast-math, 1 thread): "
+                        f"{m['seconds_per_unit'] * 1e3:.2f} ms per energy evaluation and "
+                        f"{m['force_over_energy'] * m['seconds_per_unit'] * 1e3:.2f} ms per force evaluation, "
+                        f"other work included (first {r.num_iter} iterations, {r.num_energy_call} energy and "
+                        f"{r.num_force_call} force calls in {dt:.1f} s); applied to the GPU run's "
                         f"{gpu_res.num_energy_call} energy calls, {gpu_res.num_force_call} force calls, "
                         f"{gpu_res.num_iter} iterations = {secs:.0f} s per minimization; "
                         f"host {_cpu_model()}, 1 of {os.cpu_count()} cores"))
@@ -189,6 +193,7 @@ def cpu_validate(workload, budget_s):
         "num_iter": int(r.num_iter), "num_energy_call": int(r.num_energy_call),
         "num_force_call": int(r.num_force_call), "grad_inf_norm": float(r.grad_inf_norm),
         "status": int(r.status), "model": m, "sample_iterations": int(r_s.num_iter), "sample_seconds": dt_s,
+        "sample_energy_calls": int(r_s.num_energy_call), "sample_force_calls": int(r_s.num_force_call),
     }))
 
 
@@ -271,6 +276,8 @@ def main():
         return bench_ensemble(a, lib)
     if a.workload == "net200k":
         return bench_successive(a, lib)
+    if a.workload == "mixed_tight":
+        return bench_mixed_tight(a, lib)
     if a.workload == "net50k_grad":
         return bench_gradient(a, lib)
     net = network.generate(lib, a.workload)
@@ -614,6 +621,62 @@ def bench_successive(a, lib):
                          if a.cpu_seconds > 0 else None),
     }
     print(json.dumps(out))
+
+
+def bench_mixed_tight(a, lib):
+    """Mixed vs FP64 at equal final energy: from one FP64 minimum at 1 pN
+    (identical start for both forms), each form minimizes to tighter
+    gradient tolerances; the final energies, their relative difference and
+    the iterations are reported per tolerance, with an FP64 control started
+    from the same coordinates moved by N(0, 1e-9 nm). net500 and net40k."""
+    import numpy as np
+    import torch
+    from medyan_amd import _abi, mechanics, network
+
+    nff = _abi.MDY_NFF
+    rows = []
+    for wl in ("net500", "net40k"):
+        net = network.generate(lib, wl)
+        ctxs = {}
+        for name, prec in (("fp64", _abi.MDY_PREC_FP64), ("mixed", _abi.MDY_PREC_MIXED)):
+            c = mechanics.Context(precision=prec)
+            c.set_network(net)
+            ctxs[name] = c
+        x0 = torch.from_numpy(np.ascontiguousarray(net.coord)).cuda()
+        x1 = x0.clone()
+        ctxs["fp64"].build_neighbors_dev(x1.data_ptr())
+        ctxs["mixed"].build_neighbors_dev(x1.data_ptr())
+        p1 = _abi.default_cg_params(net.n_beads, grad_tol=1.0, energy_rel_tol=0.0)
+        r1 = ctxs["fp64"].minimize_dev(p1, x1.data_ptr())
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(5)
+        for g in (1.0, 1e-1, 1e-2):
+            p = _abi.default_cg_params(net.n_beads, grad_tol=g, energy_rel_tol=0.0, force_call_limit=400000)
+            out = {"workload": wl, "grad_tol_pN": g}
+            for name in ("fp64", "mixed", "fp64_control"):
+                c = ctxs["mixed" if name == "mixed" else "fp64"]
+                x = x1.clone()
+                if name == "fp64_control":
+                    x.add_(torch.randn(x.shape, generator=gen, device="cuda", dtype=torch.float64), alpha=1e-9)
+                t0 = time.perf_counter()
+                r = c.minimize_dev(p, x.data_ptr())
+                torch.cuda.synchronize()
+                out[name] = {"energy": float(r.energies_after[nff]), "iterations": int(r.num_iter),
+                             "max_force_pN": float(r.grad_inf_norm), "reached": bool(r.grad_inf_norm < g),
+                             "status": int(r.status), "seconds": time.perf_counter() - t0,
+                             "us_per_iteration": 1e3 * c.last_ms() / max(int(r.num_iter), 1)}
+            ef = out["fp64"]["energy"]
+            out["mixed_vs_fp64_rel"] = abs(out["mixed"]["energy"] - ef) / abs(ef)
+            out["control_vs_fp64_rel"] = abs(out["fp64_control"]["energy"] - ef) / abs(ef)
+            print(f"[mixed_tight] {wl} gradTol {g}: mixed {out['mixed_vs_fp64_rel']:.2e} "
+                  f"control {out['control_vs_fp64_rel']:.2e}", file=sys.stderr, flush=True)
+            rows.append(out)
+        rows.append({"workload": wl, "start": "FP64 minimum at 1 pN from the generated coordinates",
+                     "start_iterations": int(r1.num_iter), "start_energy": float(r1.energies_after[nff])})
+        for c in ctxs.values():
+            c.close()
+    print(json.dumps({"metric": "final-energy difference, mixed vs FP64, from identical starts by gradient tolerance",
+                      "rows": rows}))
 
 
 def bench_gradient(a, lib):

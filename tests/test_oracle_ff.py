@@ -11,6 +11,8 @@ quadrature of the excluded-volume integral.
 """
 import math
 
+import functools
+
 import numpy as np
 import pytest
 
@@ -169,11 +171,16 @@ EV_GENERAL = dict(bs=[0, 3, 6, 9, 0, 3, 12, 15], k=[100.0, 2e3], eql=[20.0, 20.0
                   x=[10, 0, 0, -10, 0, 0, 0, 10, 5, 3, -10, 5, 3, 70, 30, 2, -20, 0])
 
 
+@functools.lru_cache(maxsize=8)
+def _leggauss01(n):
+    t, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (t + 1), 0.5 * w
+
+
 def gauss_ev(c1, c2, c3, c4, kr, n=400):
     """kr * Int_0^1 Int_0^1 ds dt / |P(s) - Q(t)|^4 by tensor Gauss-Legendre:
     an independent quadrature of the integral the analytic form evaluates."""
-    t, w = np.polynomial.legendre.leggauss(n)
-    t, w = 0.5 * (t + 1), 0.5 * w
+    t, w = _leggauss01(n)
     P = c1[None, :] + t[:, None] * (c2 - c1)[None, :]
     Q = c3[None, :] + t[:, None] * (c4 - c3)[None, :]
     d2 = ((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1)
