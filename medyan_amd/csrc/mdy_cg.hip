@@ -271,7 +271,7 @@ struct Cg {
         mark(0);
         sweep<MIXED>(d, pos, gtid, gs, nullptr, xs, slots(), false, true);
         mark(1);
-        block_reduce_xsum2(xs, sd, ldsl());
+        block_reduce_xsum2_lds(xs, sd, ldsl());
         acc_push(accs(p0) + kAcc0, xs);
         acc_push(accs(p0) + kAcc1, sd);
         if (!sync(PK_EVAL)) return false;
@@ -420,10 +420,9 @@ struct Cg {
             }
         }
         mark(1);
-        block_reduce<1>(m, lds, 0x1u);
+        block_reduce_xsum2_lds(sff, sfp, ldsl(), m);   // F.F and F.Fprev together, and max|F|
         unsigned long long *a = accs(p1);
         if (tid == 0) atomicMax(a + kAccMax, (unsigned long long)__double_as_longlong(m[0]));
-        block_reduce_xsum2(sff, sfp, ldsl());   // F.F and F.Fprev together
         acc_push(a + kAcc0, sff);
         acc_push(a + kAcc1, sfp);
         if (!sync(PK_GATHER)) return false;
@@ -869,9 +868,13 @@ aborted:
 
 // The arguments come by value: pointers read from the kernel-argument segment
 // are known to be global, so every access is a global (not flat) access.
+// LDS: the shuffle-free exact reductions need block_combine_words(17) words
+constexpr int kCgLds = block_combine_words(17);
+static_assert(kCgLds >= 17 * 16, "LDS of the other block reductions");
+
 template <bool MIXED, bool QUAD>
 __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
-    __shared__ double lds[17 * 16];
+    __shared__ double lds[kCgLds];
     cg_run<MIXED, QUAD>(Av, lds, blockIdx.x, gridDim.x);
 }
 
@@ -885,7 +888,7 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
 // kernel.
 template <bool MIXED, bool QUAD>
 __global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ all, const int2 *__restrict__ map) {
-    __shared__ double lds[17 * 16];
+    __shared__ double lds[kCgLds];
     const int2 m = map[blockIdx.x];
 #if defined(__HIP_DEVICE_COMPILE__)
     typedef __attribute__((address_space(4))) const CgArgs ConstArgs;

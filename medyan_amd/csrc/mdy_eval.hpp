@@ -496,6 +496,75 @@ __device__ __forceinline__ void block_reduce_xsum2(mdy_xsum &s, mdy_xsum &t, lon
     mdy_xs_norm(&t);
     __syncthreads();
 }
+// Block-wide combination of NW 64-bit words per thread through LDS: word k
+// is summed, or OR-ed where bit k of or_mask is set, or unsigned-max-ed where
+// bit k of max_mask is set. Every thread writes its words; one thread per
+// 16-entry chunk combines it; one thread per word combines the chunks; every
+// thread reads the results. Exact for integers, so the result does not depend
+// on the order, and it takes three short LDS round trips instead of the
+// dependent shuffle chains of a butterfly. blockDim.x must be a multiple of 16.
+// lds: >= block_combine_words(NW) 64-bit words.
+__host__ __device__ constexpr int block_combine_words(int nw, int nthreads = kBlock) {
+    return nw * (nthreads + nthreads / 16 + 1);
+}
+template <int NW>
+__device__ __forceinline__ void block_combine(long long w[NW], long long *lds, unsigned or_mask, unsigned max_mask) {
+    const int n = blockDim.x, t = threadIdx.x, nch = n >> 4;
+    long long *part = lds + NW * n;
+    long long *res = part + NW * nch;
+#pragma unroll
+    for (int k = 0; k < NW; k++) lds[k * n + t] = w[k];
+    __syncthreads();
+    for (int job = t; job < NW * nch; job += n) {
+        const int k = job / nch, c = job - k * nch;
+        const bool isor = (or_mask >> k) & 1u, ismax = (max_mask >> k) & 1u;
+        const long long *src = lds + k * n + c * 16;
+        long long r = src[0];
+#pragma unroll
+        for (int j = 1; j < 16; j++) {
+            const long long v = src[j];
+            r = isor ? (r | v) : ismax ? ((unsigned long long)v > (unsigned long long)r ? v : r) : r + v;
+        }
+        part[k * nch + c] = r;
+    }
+    __syncthreads();
+    if (t < NW) {
+        const bool isor = (or_mask >> t) & 1u, ismax = (max_mask >> t) & 1u;
+        long long r = part[t * nch];
+        for (int c = 1; c < nch; c++) {
+            const long long v = part[t * nch + c];
+            r = isor ? (r | v) : ismax ? ((unsigned long long)v > (unsigned long long)r ? v : r) : r + v;
+        }
+        res[t] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NW; k++) w[k] = res[k];
+    __syncthreads();
+}
+// Exact block sums of two per-thread accumulators through block_combine;
+// with_max: also the maximum of a non-negative double m (all threads get it).
+__device__ __forceinline__ void block_reduce_xsum2_lds(mdy_xsum &s, mdy_xsum &t, long long *lds, double *m = nullptr) {
+    mdy_xs_norm(&s);   // digits below 2^48: a block's sum of them cannot overflow
+    mdy_xs_norm(&t);
+    long long w[17];
+#pragma unroll
+    for (int k = 0; k < MDY_XS_N; k++) { w[k] = s.d[k]; w[8 + k] = t.d[k]; }
+    w[7] = s.flags;
+    w[15] = t.flags;
+    w[16] = m ? __double_as_longlong(*m) : 0;
+    if (m) block_combine<17>(w, lds, (1u << 7) | (1u << 15), 1u << 16);
+    else block_combine<16>(w, lds, (1u << 7) | (1u << 15), 0u);
+#pragma unroll
+    for (int k = 0; k < MDY_XS_N; k++) { s.d[k] = w[k]; t.d[k] = w[8 + k]; }
+    s.flags = (int)w[7];
+    t.flags = (int)w[15];
+    s.cnt = 0;
+    t.cnt = 0;
+    mdy_xs_norm(&s);
+    mdy_xs_norm(&t);
+    if (m) *m = __longlong_as_double(w[16]);
+}
 // per-block partial records: [blk * kRed + off .. off+7] hold a normalized accumulator
 __device__ __forceinline__ void store_xsum(double *red, int blk, int off, const mdy_xsum &s) {
     long long *r = reinterpret_cast<long long *>(red) + (size_t)blk * kRed + off;
