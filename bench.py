@@ -803,4 +803,9 @@ def bench_gradient(a, lib):
 
 
 if __name__ == "__main__":
-    main()
+    try:
+        main()
+    finally:
+        # device memory released while the HIP runtime is still up
+        if "medyan_amd.mechanics" in sys.modules:
+            sys.modules["medyan_amd.mechanics"].close_all()

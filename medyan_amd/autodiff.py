@@ -25,6 +25,14 @@ with w from conjugate gradients on Hessian-vector products. H is positive
 semi-definite at a minimum; rigid motions of free clusters are its null
 space, so grad L must not move along them (L invariant under those motions,
 e.g. distances within a filament).
+
+Tethered minima: a structure stopped at a force tolerance need not be a
+minimum (its Hessian can have negative curvature), and then dx*/ds does not
+exist. Tethering every bead to reference coordinates x_ref by a spring of
+stiffness t, E_t(x; s) = E(x; s) + t/2 |x - x_ref|^2, gives a minimum with
+(H + t I) positive definite once t exceeds the most negative curvature, and
+    dL/ds_k = -w . grad E_k(x_t*),   (H + t I) w = grad L(x_t*),
+on the full space (the tether removes the rigid-motion symmetries).
 """
 import math
 
@@ -42,8 +50,20 @@ NFF = _abi.MDY_NFF
 STIFFNESS_FIELDS = ("fs_k", "fb_k", "ls_k", "ms_k", "bs_k", "bb_k", "bdh_k", "bp_k", "cyl_exvol_k", "plane_k")
 
 
-def _stream() -> int:
-    return torch.cuda.current_stream().cuda_stream
+def _ordered(call):
+    """Runs call(stream) ordered with torch's current stream. A stream with a
+    handle is passed through; torch's null stream cannot be named through the
+    C ABI (NULL there means the context's own non-blocking stream), so the
+    call then runs on the context stream between two device synchronizations:
+    its inputs are complete before it starts and its outputs before torch
+    reads them."""
+    s = torch.cuda.current_stream().cuda_stream
+    if s:
+        call(s)
+    else:
+        torch.cuda.synchronize()
+        call(0)
+        torch.cuda.synchronize()
 
 
 def _check(ctx: Context, x: torch.Tensor, name="x"):
@@ -58,7 +78,7 @@ class _Energy(torch.autograd.Function):
     @staticmethod
     def forward(fc, x, ctx):
         out = torch.empty(NFF + 2, dtype=torch.float64, device=x.device)
-        ctx.energy_dev(x.data_ptr(), out.data_ptr(), _stream())
+        _ordered(lambda s: ctx.energy_dev(x.data_ptr(), out.data_ptr(), s))
         fc.save_for_backward(x)
         fc.mech = ctx
         return out[NFF].clone()
@@ -73,7 +93,7 @@ class _Gradient(torch.autograd.Function):
     @staticmethod
     def forward(fc, x, ctx):
         f = torch.empty_like(x)
-        ctx.forces_dev(x.data_ptr(), f.data_ptr(), accumulate=False, stream=_stream())
+        _ordered(lambda s: ctx.forces_dev(x.data_ptr(), f.data_ptr(), accumulate=False, stream=s))
         fc.save_for_backward(x)
         fc.mech = ctx
         return f.neg_()
@@ -103,7 +123,7 @@ def hvp(ctx: Context, x: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
     v = v.detach().to(torch.float64).contiguous()
     _check(ctx, v, "v")
     hv = torch.empty_like(v)
-    ctx.hvp_dev(x.data_ptr(), v.data_ptr(), hv.data_ptr(), _stream())
+    _ordered(lambda s: ctx.hvp_dev(x.data_ptr(), v.data_ptr(), hv.data_ptr(), s))
     return hv
 
 
@@ -113,7 +133,7 @@ def energy_jvp(ctx: Context, x: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
     w = w.detach().to(torch.float64).contiguous()
     _check(ctx, w, "w")
     out = torch.empty(NFF + 1, dtype=torch.float64, device=x.device)
-    ctx.energy_jvp_dev(x.data_ptr(), w.data_ptr(), out.data_ptr(), _stream())
+    _ordered(lambda s: ctx.energy_jvp_dev(x.data_ptr(), w.data_ptr(), out.data_ptr(), s))
     return out
 
 
@@ -208,13 +228,24 @@ class SymmetryModes:
         return (V - (self._q * c[self._comp][:, None, :]).sum(2)).reshape(v.shape)
 
 
+class NotAMinimum(RuntimeError):
+    """Conjugate gradients on H met a direction of non-positive curvature:
+    `direction` (a device tensor, unit length) with `curvature` = p.Hp / p.p."""
+
+    def __init__(self, msg, direction, curvature):
+        super().__init__(msg)
+        self.direction = direction
+        self.curvature = curvature
+
+
 def solve_hessian(ctx: Context, x: torch.Tensor, b: torch.Tensor, rtol=1e-10, maxiter=20000,
                   modes: "SymmetryModes | None" = None, shift=0.0):
     """w with (H(x) + shift I) w = b by conjugate gradients on the
     complement of the symmetry directions `modes` (H is positive definite
     there at a well-determined minimum; b is projected first). shift > 0
     regularizes near-null mechanisms (Tikhonov). Returns (w, iterations,
-    relative residual)."""
+    relative residual); raises NotAMinimum where H + shift I is not positive
+    definite along a conjugate direction."""
     P = modes.project if modes is not None else (lambda v: v)
     b = P(b.detach().to(torch.float64).contiguous())
     w = torch.zeros_like(b)
@@ -232,8 +263,10 @@ def solve_hessian(ctx: Context, x: torch.Tensor, b: torch.Tensor, rtol=1e-10, ma
         hp = P(hp)
         php = float(torch.dot(p, hp))
         if not php > 0.0:
-            raise RuntimeError(f"Hessian not positive along a conjugate direction (p.Hp = {php:g}, "
-                               f"|p|^2 = {float(torch.dot(p, p)):g}, iteration {it}): not a minimum")
+            pp = float(torch.dot(p, p))
+            raise NotAMinimum(f"Hessian not positive along a conjugate direction (p.Hp = {php:g}, "
+                              f"|p|^2 = {pp:g}, iteration {it}): not a minimum",
+                              p / math.sqrt(pp), php / pp - shift)
         a = rr / php
         w.add_(p, alpha=a)
         r.add_(hp, alpha=-a)
@@ -245,17 +278,78 @@ def solve_hessian(ctx: Context, x: torch.Tensor, b: torch.Tensor, rtol=1e-10, ma
 
 
 def stiffness_sensitivity(ctx: Context, x_star: torch.Tensor, grad_l: torch.Tensor, rtol=1e-10, maxiter=20000,
-                          shift=0.0):
+                          shift=0.0, tether=0.0):
     """dL/ds_k (k over the force fields) of a quantity L of the minimized
     coordinates x_star, given grad L(x_star); s_k scales force field k's
     stiffness constants (STIFFNESS_FIELDS). L must be invariant under the
     symmetry directions (rigid motions of free clusters). shift > 0 solves
     with H + shift I (a regularized derivative, for networks with floppy
-    mechanisms). Returns (dL/ds [NFF], info)."""
+    mechanisms). tether > 0: x_star is the minimum of the tethered energy
+    (tethered_minimum with that stiffness) and the solve is (H + tether I)
+    w = grad L on the full space. Returns (dL/ds [NFF], info)."""
+    if tether > 0.0:
+        w, it, res = solve_hessian(ctx, x_star, grad_l, rtol, maxiter, None, tether)
+        d = -energy_jvp(ctx, x_star, w)[:NFF]
+        return d, dict(cg_iterations=it, relative_residual=res, tether=tether)
     modes = SymmetryModes(ctx, x_star)
     w, it, res = solve_hessian(ctx, x_star, grad_l, rtol, maxiter, modes, shift)
     d = -energy_jvp(ctx, x_star, w)[:NFF]
     return d, dict(cg_iterations=it, relative_residual=res, clusters=modes.n_clusters, symmetry_modes=modes.n_modes)
+
+
+def tethered_minimum(ctx: Context, x_ref: torch.Tensor, tether: float, x0: "torch.Tensor | None" = None,
+                     ftol=1e-9, maxsteps=50, rtol=1e-12, maxiter=50000, maxstep=1.0):
+    """Minimum of E(x) + tether/2 |x - x_ref|^2 by Newton steps
+    (H + tether I) dx = F - tether (x - x_ref), F from the force kernel and
+    the solves by conjugate gradients on Hessian-vector products. A step
+    longer than `maxstep` nm on some bead is scaled down to it, and a step
+    that raises the tethered energy is halved until it does not (at most 30
+    times). A step is kept when the tethered energy does not rise beyond
+    rounding or, where energy differences are below the energy's own rounding
+    (close to the minimum), when the 2-norm of the tethered force falls.
+    Starts from x0 (default x_ref); stops when the largest tethered force is
+    below ftol. Returns (x, max|F_t|, info). Raises NotAMinimum if the tether
+    does not make H + tether I positive definite."""
+    _check(ctx, x_ref, "x_ref")
+    if not tether > 0.0:
+        raise ValueError("tether stiffness must be positive")
+    x_ref = x_ref.detach()
+    x = (x0 if x0 is not None else x_ref).detach().clone()
+    _check(ctx, x)
+
+    def tethered(z):
+        d = z - x_ref
+        return float(energy(ctx, z)) + 0.5 * tether * float(torch.dot(d, d))
+
+    def force(z):
+        return -gradient(ctx, z) - tether * (z - x_ref)
+
+    e = tethered(x)
+    f = force(x)
+    fmax = float(f.abs().max())
+    fnorm = float(torch.linalg.vector_norm(f))
+    steps = cg_total = halvings = 0
+    while fmax >= ftol and steps < maxsteps:
+        dx, it, _ = solve_hessian(ctx, x, f, rtol=rtol, maxiter=maxiter, modes=None, shift=tether)
+        cg_total += it
+        big = float(dx.abs().max())
+        if big > maxstep:
+            dx.mul_(maxstep / big)
+        for _ in range(30):
+            xn = x + dx
+            en = tethered(xn)
+            fn = force(xn)
+            fnn = float(torch.linalg.vector_norm(fn))
+            if math.isfinite(en) and math.isfinite(fnn) and (en <= e + 1e-13 * abs(e) or fnn < fnorm):
+                break
+            dx.mul_(0.5)
+            halvings += 1
+        else:
+            break
+        x, e, f, fnorm = xn, en, fn, fnn
+        fmax = float(f.abs().max())
+        steps += 1
+    return x, fmax, dict(steps=steps, cg_iterations=cg_total, halvings=halvings, tethered_energy=e)
 
 
 def newton_polish(ctx: Context, x: torch.Tensor, ftol=1e-10, maxiter=30, modes: "SymmetryModes | None" = None):
@@ -276,6 +370,88 @@ def newton_polish(ctx: Context, x: torch.Tensor, ftol=1e-10, maxiter=30, modes: 
         fmax = float(f.abs().max())
         steps += 1
     return x, fmax, steps
+
+
+def lm_polish(ctx: Context, x: torch.Tensor, ftol=1e-9, mu=0.0, maxsteps=60, rtol=1e-10, maxiter=20000,
+              modes: "SymmetryModes | None" = None, maxstep=1.0, log=None):
+    """Converges a near-minimum to a true local minimum by damped Newton
+    (Levenberg-Marquardt) steps (H + mu I) dx = F, with F from the force
+    kernel and the solves by conjugate gradients on Hessian-vector products
+    outside the symmetry directions. A step is kept when no bead moves more
+    than `maxstep` nm, the forces stay finite, and it lowers the energy or,
+    at equal energy to rounding, the largest force; then mu shrinks (x 1/4),
+    otherwise it grows (x 8). mu starts at least at max|F| / maxstep. Where
+    the Hessian has negative curvature (a saddle of a stopped minimization)
+    mu stays above it, so every kept step descends until the Hessian is
+    positive semi-definite. Stops when max|F| < ftol. Returns (x, max|F|,
+    info)."""
+    _check(ctx, x)
+    x = x.detach().clone()
+    modes = modes if modes is not None else SymmetryModes(ctx, x)
+    e = float(energy(ctx, x))
+    f = modes.project(-gradient(ctx, x))
+    fraw = float((-gradient(ctx, x)).abs().max())
+    mu = max(mu, fraw / maxstep)
+    steps = solves = cg_total = 0
+    while fraw >= ftol and steps < maxsteps:
+        steps += 1
+        try:
+            dx, it, res = solve_hessian(ctx, x, f, rtol=rtol, maxiter=maxiter, modes=modes, shift=mu)
+            cg_total += it
+            solves += 1
+        except RuntimeError:
+            mu *= 8.0
+            continue
+        if float(dx.abs().max()) > maxstep:
+            mu *= 8.0
+            continue
+        xn = x + dx
+        en = float(energy(ctx, xn))
+        fn_raw = float((-gradient(ctx, xn)).abs().max())
+        if math.isfinite(en) and math.isfinite(fn_raw) and (en < e or (en <= e + 1e-13 * abs(e) and fn_raw < fraw)):
+            x, e, fraw = xn, en, fn_raw
+            f = modes.project(-gradient(ctx, x))
+            mu = max(mu * 0.25, 1e-12)
+        else:
+            mu *= 8.0
+        if log:
+            log(f"lm step {steps}: mu {mu:.2e}, E {e:.15e}, max|F| {fraw:.3e}, cg {it}")
+    return x, fraw, dict(steps=steps, solves=solves, cg_iterations=cg_total, final_mu=mu)
+
+
+def lowest_curvature(ctx: Context, x: torch.Tensor, steps=200, seed=0):
+    """Estimate of the smallest eigenvalue of H(x) and its eigenvector, by
+    `steps` Lanczos steps on Hessian-vector products (full
+    re-orthogonalization, start vector seeded). A negative value shows that
+    x is not a minimum; the returned unit vector is a direction of negative
+    curvature that the energy alone can confirm (second differences).
+    Returns (eigenvalue, vector)."""
+    from scipy.linalg import eigh_tridiagonal
+
+    _check(ctx, x)
+    n = x.numel()
+    g = torch.Generator(device=x.device).manual_seed(seed)
+    q = torch.randn(n, dtype=torch.float64, device=x.device, generator=g)
+    q /= torch.linalg.vector_norm(q)
+    V = torch.empty(steps, n, dtype=torch.float64, device=x.device)
+    alpha, beta = [], []
+    m = steps
+    for j in range(steps):
+        V[j] = q
+        w = hvp(ctx, x, q)
+        a = float(torch.dot(w, q))
+        alpha.append(a)
+        for _ in range(2):
+            w -= V[:j + 1].T @ (V[:j + 1] @ w)
+        b = float(torch.linalg.vector_norm(w))
+        if j == steps - 1 or b < 1e-12 * max(abs(a), 1.0):
+            m = j + 1
+            break
+        beta.append(b)
+        q = w / b
+    lam, Y = eigh_tridiagonal(np.array(alpha[:m]), np.array(beta[:m - 1]), select="i", select_range=(0, 0))
+    vec = V[:m].T @ torch.as_tensor(Y[:, 0], device=x.device)
+    return float(lam[0]), vec / torch.linalg.vector_norm(vec)
 
 
 def scale_stiffness(net: Network, s) -> Network:

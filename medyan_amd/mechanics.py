@@ -18,8 +18,10 @@ The library is required: there is no CPU fallback. Loading fails loudly if
 libmdy_hip.so is missing, and every GPU entry point fails if no HIP device is
 present.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -31,6 +33,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdy_hip.so")
 
 _lib = None
+# every open context, closed at interpreter exit before the HIP runtime and
+# torch tear down (a context freed from a finalizer after that would call
+# into a runtime that is gone)
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def close_all():
+    """Destroys every context still open (also run at interpreter exit)."""
+    for ctx in list(_live):
+        ctx.close()
 
 
 def load_library(path: str = LIB_PATH):
@@ -158,11 +171,14 @@ class Context:
         self.h = h
         self.net = None
         self._sys = None
+        _live.add(self)
 
     def close(self):
+        """mdy_destroy: frees the device memory; the context is unusable after."""
         if getattr(self, "h", None):
             self.lib.mdy_destroy(self.h)
             self.h = None
+        _live.discard(self)
 
     def __del__(self):
         try:

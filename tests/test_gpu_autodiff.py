@@ -195,3 +195,29 @@ def test_implicit_stiffness_derivative_matches_reminimization():
         print(f"[gpu] force field {k}: implicit {d[k]:.8g}  differences {fd:.8g}")
         assert abs(d[k] - fd) <= 1e-5 * np.abs(d).max()
     assert all(d[k] == 0 for k in range(4, NFF))   # no branch points, pairs or boundary terms
+
+
+@pytest.mark.parametrize("side_stream", [False, True])
+def test_device_calls_follow_torch_stream(lib, side_stream):
+    """The device-pointer calls read coordinates that torch has just written:
+    on torch's null stream (which the C ABI cannot name, so the calls run on
+    the context stream between synchronizations) and on a side stream (passed
+    through), each result equals the host-path evaluation at the same
+    coordinates bit for bit."""
+    net = network.generate(lib, "net500")
+    ctx = exact_ctx(net)
+    x0 = dev(net.coord)
+    rng = np.random.default_rng(5)
+    s = torch.cuda.Stream() if side_stream else torch.cuda.current_stream()
+    with torch.cuda.stream(s):
+        for k in range(6):
+            step = dev(rng.normal(0.0, 0.5, x0.numel()))
+            xn = x0 + step * (k + 1)       # written by torch immediately before each call
+            e = autodiff.energy(ctx, xn).item()
+            f = (-autodiff.gradient(ctx, xn)).cpu().numpy()
+            xh = xn.cpu().numpy()
+            eh, _ = ctx.energy(xh)
+            assert e == eh, (k, e, eh)
+            assert np.array_equal(f, ctx.forces(xh)), k
+    torch.cuda.synchronize()
+    ctx.close()
