@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MDY_ABI_VERSION 3
+#define MDY_ABI_VERSION 4
 
 enum {
     MDY_OK = 0,
@@ -192,7 +192,16 @@ typedef struct mdy_min_result {
 /* Context                                                                    */
 /* ------------------------------------------------------------------------- */
 enum { MDY_PREC_FP64 = 0, MDY_PREC_MIXED = 1 };
-enum { MDY_COPLANAR_REFERENCE = 0, MDY_COPLANAR_EXACT = 1 };
+/* MDY_COPLANAR_REFERENCE: the reference's force path (coplanar pairs perturbed);
+ * MDY_COPLANAR_EXACT: the exact gradient of the closed-form energy everywhere;
+ * MDY_COPLANAR_QUADRATURE: as EXACT, but excluded-volume pairs whose closed
+ *   form is ill-conditioned (squared normalized triple product below 1e-3:
+ *   nearly coplanar or nearly parallel cylinders) are evaluated by composite
+ *   Gauss-Legendre quadrature of the defining integral, so that their
+ *   derivatives are not rounding noise. No counterpart in the reference; for
+ *   the differentiable form. One-shot evaluations (energy, forces, H v,
+ *   energy directional derivatives) only: FP64 contexts, no minimization. */
+enum { MDY_COPLANAR_REFERENCE = 0, MDY_COPLANAR_EXACT = 1, MDY_COPLANAR_QUADRATURE = 2 };
 
 typedef struct mdy_opts {
     int32_t precision;       /* MDY_PREC_*                                                 */
@@ -284,6 +293,10 @@ int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *
  * ForceFieldManager::computeIndividualEnergies, ForceFieldManager.cpp:209-227);
  * used for parameter sensitivities of minimized structures. */
 int mdy_energy_jvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_w, double *d_out, void *stream);
+/* Number of excluded-volume pairs that the MDY_COPLANAR_QUADRATURE mode
+ * evaluates by quadrature at host coordinates coord (0 in the other modes).
+ * No reference counterpart; reports the mode's extent. */
+int mdy_quadrature_pair_count(mdy_ctx *ctx, const double *coord, int32_t *count);
 
 /* Post-minimization auxiliary outputs (MController::afterMinimization_,
  * Controller/MController.h:240-268), at coord:

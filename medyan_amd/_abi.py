@@ -6,7 +6,7 @@ medyan_amd.mechanics.
 """
 import ctypes as C
 
-MDY_ABI_VERSION = 3
+MDY_ABI_VERSION = 4
 
 MDY_OK = 0
 ERRORS = {
@@ -38,7 +38,7 @@ MDY_NFF = 10
 
 MDY_CG_STEEPEST, MDY_CG_FLETCHER_RIEVES, MDY_CG_POLAK_RIBIERE = 0, 1, 2
 MDY_PREC_FP64, MDY_PREC_MIXED = 0, 1
-MDY_COPLANAR_REFERENCE, MDY_COPLANAR_EXACT = 0, 1
+MDY_COPLANAR_REFERENCE, MDY_COPLANAR_EXACT, MDY_COPLANAR_QUADRATURE = 0, 1, 2
 MDY_ERROR_LINESEARCH = 1
 
 _i32p = C.POINTER(C.c_int32)

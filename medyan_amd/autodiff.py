@@ -11,7 +11,11 @@ src/Mechanics/ForceField/ForceFieldManager.cpp:96-290) to torch. Derivatives
 are those of the exact-gradient force field, so the context must use
 MDY_COPLANAR_EXACT: the reference's coplanar excluded-volume force
 (CylinderExclVolRepulsion.cpp:610-626) perturbs the geometry and is not the
-gradient of the energy there. Coordinates are float64 CUDA tensors of
+gradient of the energy there. MDY_COPLANAR_QUADRATURE additionally
+evaluates the excluded-volume pairs whose closed form is ill-conditioned
+(nearly coplanar or parallel cylinders) by quadrature of the defining
+integral, so their second derivatives are not rounding noise; that is the
+mode for Hessians of large random networks. Coordinates are float64 CUDA tensors of
 3 * n_beads entries, on the context's device; work is ordered on torch's
 current stream.
 
@@ -67,8 +71,9 @@ def _ordered(call):
 
 
 def _check(ctx: Context, x: torch.Tensor, name="x"):
-    if ctx.opts.coplanar_mode != _abi.MDY_COPLANAR_EXACT:
-        raise ValueError("derivatives need a context with coplanar_mode=MDY_COPLANAR_EXACT")
+    if ctx.opts.coplanar_mode not in (_abi.MDY_COPLANAR_EXACT, _abi.MDY_COPLANAR_QUADRATURE):
+        raise ValueError("derivatives need a context with coplanar_mode MDY_COPLANAR_EXACT or "
+                         "MDY_COPLANAR_QUADRATURE")
     n = 3 * ctx.net.n_beads
     if x.dtype != torch.float64 or not x.is_cuda or x.numel() != n or not x.is_contiguous():
         raise ValueError(f"{name}: expected a contiguous float64 CUDA tensor of {n} entries")

@@ -17,7 +17,8 @@ TRAJ_OUT = os.path.join(HERE, "libmdy_traj.so")
 HDF5_ROOT = os.environ.get("MDY_HDF5_ROOT", "/opt/conda")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["mdy_api.hip", "mdy_eval.hip", "mdy_nl.hip", "mdy_cg.hip", "mdy_hvp.hip", "mdy_aux.hip"]
+HIP_SOURCES = ["mdy_api.hip", "mdy_eval.hip", "mdy_nl.hip", "mdy_cg.hip", "mdy_hvp.hip", "mdy_aux.hip",
+               "mdy_quad.hip"]
 CXX_SOURCES = ["netgen.cpp"]
 ARCH = os.environ.get("MDY_ARCH", "gfx950")
 

@@ -42,6 +42,7 @@ Dev Ctx::dev() const {
     D.n_slots = D.slot_bd + D.n_bd;
     D.st_off = st_off.p; D.st_idx = st_idx.p; D.dy_off = dy_off.p; D.dy_idx = dy_idx.p;
     D.coplanar_mode = opts.coplanar_mode;
+    D.evq = nullptr;   // set by the one-shot launchers in quadrature mode
     D.cq = std::cos(0.5 * M_PI);   // BranchingPositionCosine.cpp:235, 372 (posheta = pi/2)
     D.sq = std::sin(0.5 * M_PI);
     D.qe = qe.p; D.qe_n = qn.p; D.qf = qf.p; D.qf_n = qn.p + 1; D.err = err.p;
@@ -108,6 +109,15 @@ void mdy_default_opts(mdy_opts *o) {
     o->coplanar_mode = MDY_COPLANAR_REFERENCE;
 }
 
+static const char *bad_opts(const mdy_opts *o) {
+    if (o->precision != MDY_PREC_FP64 && o->precision != MDY_PREC_MIXED) return "precision";
+    if (o->coplanar_mode < MDY_COPLANAR_REFERENCE || o->coplanar_mode > MDY_COPLANAR_QUADRATURE)
+        return "coplanar_mode";
+    if (o->coplanar_mode == MDY_COPLANAR_QUADRATURE && o->precision != MDY_PREC_FP64)
+        return "the quadrature mode needs an FP64 context";
+    return nullptr;
+}
+
 int mdy_create(mdy_ctx **out, int device, const mdy_opts *opts) {
     if (!out) return MDY_ERR_ARG;
     *out = nullptr;
@@ -120,6 +130,7 @@ int mdy_create(mdy_ctx **out, int device, const mdy_opts *opts) {
     c.device = device;
     if (opts) c.opts = *opts;
     else mdy_default_opts(&c.opts);
+    if (bad_opts(&c.opts)) { delete ctx; return MDY_ERR_ARG; }
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return MDY_ERR_HIP; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; return MDY_ERR_HIP; }
@@ -161,7 +172,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.pairs); release(c.ev_bs); release(c.ev_kr); release(c.bd_flag); release(c.bd_off); release(c.bd_bead);
     release(c.bd_plane); release(c.bd_k); release(c.bd_sl); release(c.dy_cnt); release(c.dy_off); release(c.dy_cur);
     release(c.dy_idx); release(c.scan_tmp); release(c.bbox); release(c.x); release(c.xt); release(c.f);
-    release(c.batch_args); release(c.batch_map);
+    release(c.batch_args); release(c.batch_map); release(c.evq);
     release(c.contrib); release(c.red); release(c.out); release(c.qe); release(c.qf); release(c.qn); release(c.err);
     release(c.qe_u); release(c.simpson_part); release(c.F0); release(c.F1); release(c.S0); release(c.S1);
     release(c.xmin); release(c.bar); release(c.cg_out); release(c.tr_lam); release(c.tr_e); release(c.tr_mf);
@@ -174,7 +185,7 @@ const char *mdy_last_error(const mdy_ctx *ctx) { return ctx ? ctx->c.msg.c_str()
 
 int mdy_set_opts(mdy_ctx *ctx, const mdy_opts *o) {
     if (!ctx || !o) return MDY_ERR_ARG;
-    if (o->precision != MDY_PREC_FP64 && o->precision != MDY_PREC_MIXED) return fail(ctx, MDY_ERR_ARG, "precision");
+    if (const char *m = bad_opts(o)) return fail(ctx, MDY_ERR_ARG, m);
     ctx->c.opts = *o;
     return MDY_OK;
 }
@@ -451,6 +462,8 @@ int mdy_energy_dev_noalloc(mdy_ctx *ctx, const double *d_coord, double *d_out, v
     if (!ctx || !d_coord || !d_out) return MDY_ERR_ARG;
     Ctx &c = ctx->c;
     if (!c.have_sys || !c.have_nl) return fail(ctx, MDY_ERR_STATE, "network / neighbor list missing");
+    if (c.opts.coplanar_mode == MDY_COPLANAR_QUADRATURE)
+        return fail(ctx, MDY_ERR_ARG, "the quadrature mode allocates its per-pair flags: use mdy_energy_dev");
     (void)hipSetDevice(c.device);
     hipStream_t st = stream ? (hipStream_t)stream : c.stream;
     const int rc = launch_energy_noalloc(c, d_coord, d_out, st);
@@ -512,6 +525,19 @@ int mdy_forces(mdy_ctx *ctx, const double *coord, double *force_accum) {
     return MDY_OK;
 }
 
+int mdy_quadrature_pair_count(mdy_ctx *ctx, const double *coord, int32_t *count) {
+    if (!ctx || !coord || !count) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.have_sys || !c.have_nl) return fail(ctx, MDY_ERR_STATE, "network / neighbor list missing");
+    (void)hipSetDevice(c.device);
+    if (hipMemcpyAsync(c.xt.p, coord, sizeof(double) * 3 * c.nb, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+        return hipfail(ctx, "coord upload");
+    int n = 0;
+    if (quad_pair_count(c, c.xt.p, c.stream, &n) != MDY_OK) return hipfail(ctx, "quadrature pair count");
+    *count = n;
+    return MDY_OK;
+}
+
 int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *d_hv, void *stream) {
     if (!ctx || !d_coord || !d_v || !d_hv) return MDY_ERR_ARG;
     Ctx &c = ctx->c;
@@ -542,6 +568,8 @@ int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *p, double *d_coord, doub
     if (p->method < MDY_CG_STEEPEST || p->method > MDY_CG_POLAK_RIBIERE) return fail(ctx, MDY_ERR_ARG, "method");
     if (p->line_search != MDY_LS_BACKTRACKING && p->line_search != MDY_LS_QUADRATIC)
         return fail(ctx, MDY_ERR_ARG, "line_search");
+    if (c.opts.coplanar_mode == MDY_COPLANAR_QUADRATURE)
+        return fail(ctx, MDY_ERR_ARG, "the quadrature mode serves one-shot evaluations; minimize in another mode");
     (void)hipSetDevice(c.device);
     int rc = run_minimize(c, p, d_coord, d_force_out, result);
     if (rc == MDY_ERR_LAUNCH) return fail(ctx, rc, "cooperative launch failed");
@@ -577,6 +605,8 @@ int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params,
         if (!c.have_sys || !c.have_nl) return fail(ctxs[0], MDY_ERR_STATE, "network / neighbor list missing");
         if (c.device != ctxs[0]->c.device || c.opts.precision != ctxs[0]->c.opts.precision)
             return fail(ctxs[0], MDY_ERR_ARG, "batched contexts must share device and precision");
+        if (c.opts.coplanar_mode == MDY_COPLANAR_QUADRATURE)
+            return fail(ctxs[0], MDY_ERR_ARG, "the quadrature mode serves one-shot evaluations; minimize in another mode");
         if (params[i].method < MDY_CG_STEEPEST || params[i].method > MDY_CG_POLAK_RIBIERE)
             return fail(ctxs[0], MDY_ERR_ARG, "method");
         if (params[i].line_search != MDY_LS_BACKTRACKING && params[i].line_search != MDY_LS_QUADRATIC)

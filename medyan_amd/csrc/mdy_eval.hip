@@ -9,7 +9,7 @@
 namespace mdy {
 
 // energies: per-FF double sums at [0, kNFF), exact total at kXsE
-template <class P>
+template <class P, bool QUAD = false>
 __global__ void __launch_bounds__(kBlock) k_energy(Dev D, P pos, double *red) {
     __shared__ double lds[17 * kNFF + 8];
     double acc[kNFF];
@@ -19,7 +19,7 @@ __global__ void __launch_bounds__(kBlock) k_energy(Dev D, P pos, double *red) {
     mdy_xs_zero(&xs);
     const int total = D.n_items(), gs = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs)
-        item(D, i, pos, acc, xs, nullptr, true, false);
+        item<false, QUAD>(D, i, pos, acc, xs, nullptr, true, false);
     block_reduce<kNFF>(acc, lds);
     block_reduce_xsum(xs, reinterpret_cast<long long *>(lds));
     if (threadIdx.x < kNFF) red[(size_t)blockIdx.x * kRed + threadIdx.x] = acc[threadIdx.x];
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(Dev D, const double *re
         for (int k = 0; k < n_out; k++) out[k] = e[k];
 }
 
-template <bool MIXED, class P>
+template <bool MIXED, class P, bool QUAD = false>
 __global__ void __launch_bounds__(kBlock) k_force(Dev D, P pos, double *contrib_) {
     auto *contrib = reinterpret_cast<typename SlotT<MIXED>::T *>(contrib_);
     const int total = D.n_items(), gs = gridDim.x * blockDim.x;
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, P pos, double *contrib_
     mdy_xsum xs;
     mdy_xs_zero(&xs);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs)
-        item<MIXED>(D, i, pos, acc, xs, contrib, false, true);
+        item<MIXED, QUAD>(D, i, pos, acc, xs, contrib, false, true);
 }
 
 // Quadrature force (forceN, 10^4 x 10^4 intervals) of one queued pair,
@@ -166,11 +166,20 @@ static int grid_for(const Ctx &c, long long n) {
 int launch_energy(Ctx &c, const double *d_x, double *d_out, hipStream_t st) {
     Dev D = c.dev();
     const int nblk = grid_for(c, D.n_items());
-    MDY_TRY(ensure(c.red, (size_t)kRed * nblk));
+    const bool quad = quad_on(c);
+    const int nqb = quad ? quad_blocks(c) : 0;   // the quadrature pairs' block records follow the others
+    MDY_TRY(ensure(c.red, (size_t)kRed * (nblk + nqb)));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c.qn.p, 4);
-    hipLaunchKernelGGL(k_energy<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.red.p);
+    if (quad) {
+        MDY_TRY(quad_classify(c, D, d_x, st));
+        hipLaunchKernelGGL((k_energy<PosX, true>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.red.p);
+        MDY_TRY(launch_quad(c, D, 0, d_x, nullptr, c.red.p, nblk, st));
+    } else {
+        hipLaunchKernelGGL(k_energy<PosX>, dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.red.p);
+    }
     hipLaunchKernelGGL(k_energy_simpson<PosX>, dim3(64), dim3(kBlock), 0, st, D, PosX{d_x}, c.qe_u.p);
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, D, c.red.p, nblk, c.qe_u.p, d_out, kNFF + 2);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, D, c.red.p, nblk + nqb, c.qe_u.p, d_out,
+                       kNFF + 2);
     return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
 }
 
@@ -243,8 +252,16 @@ int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hi
     const int nblk = grid_for(c, D.n_items());
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c.qn.p, 4);
     const bool mixed = c.opts.precision == MDY_PREC_MIXED;
-    if (mixed) hipLaunchKernelGGL((k_force<true, PosX>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
-    else hipLaunchKernelGGL((k_force<false, PosX>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
+    if (quad_on(c)) {   // FP64 only (the mode is refused for mixed-precision contexts)
+        MDY_TRY(quad_classify(c, D, d_x, st));
+        hipLaunchKernelGGL((k_force<false, PosX, true>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x},
+                           c.contrib.p);
+        MDY_TRY(launch_quad(c, D, 1, d_x, nullptr, c.contrib.p, 0, st));
+    } else if (mixed) {
+        hipLaunchKernelGGL((k_force<true, PosX>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
+    } else {
+        hipLaunchKernelGGL((k_force<false, PosX>), dim3(nblk), dim3(kBlock), 0, st, D, PosX{d_x}, c.contrib.p);
+    }
     if (hipGetLastError() != hipSuccess) return MDY_ERR_HIP;
     // quadrature fallbacks: rare; the count is read back to drive them
     int h_q[4];

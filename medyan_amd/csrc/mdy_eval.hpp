@@ -164,7 +164,8 @@ __device__ __forceinline__ d3 rec_pos(const PosAny &P, const ItemRec &r, int k) 
 // the pairs of a random network qualify).
 constexpr double kMixedMinQ = 0.03;
 
-template <bool MIXED = false>
+// QUAD: the pairs flagged in D.evq are skipped (the quadrature kernels evaluate them).
+template <bool MIXED = false, bool QUAD = false>
 __device__ __forceinline__ void item_compute(const Dev &D, const ItemRec &r, const d3 c[4], double acc[kNFF],
                                              mdy_xsum &xs, typename SlotT<MIXED>::T *__restrict__ contrib,
                                              const bool PER_FF, const bool FORCES) {
@@ -172,7 +173,9 @@ __device__ __forceinline__ void item_compute(const Dev &D, const ItemRec &r, con
     int ff = 0;
     bool add = true;
     const int j = r.j;
-    if (r.type == IT_EV) {
+    if (QUAD && r.type == IT_EV && D.evq[j]) {
+        add = false;
+    } else if (r.type == IT_EV) {
         const double kr = r.p0;
         const d3 c1 = c[0], c2 = c[1], c3 = c[2], c4 = c[3];
         ff = MDY_FF_EV;
@@ -334,7 +337,7 @@ __device__ __forceinline__ void sweep(const Dev &D, const PosAny &P, int start, 
 }
 
 // single item (one-shot kernels)
-template <bool MIXED = false, class P>
+template <bool MIXED = false, bool QUAD = false, class P>
 __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double acc[kNFF], mdy_xsum &xs,
                                      typename SlotT<MIXED>::T *__restrict__ contrib, const bool PER_FF,
                                      const bool FORCES) {
@@ -345,7 +348,7 @@ __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double a
     c[1] = nb > 1 ? pos(r.b.y) : c[0];
     c[2] = nb > 2 ? pos(r.b.z) : c[0];
     c[3] = nb > 3 ? pos(r.b.w) : c[0];
-    item_compute<MIXED>(D, r, c, acc, xs, contrib, PER_FF, FORCES);
+    item_compute<MIXED, QUAD>(D, r, c, acc, xs, contrib, PER_FF, FORCES);
 }
 
 // Per-bead force: the slots of bead b in reference order, starting from 0.0

@@ -1,6 +1,7 @@
 // mdy_hvp.hip -- derivatives of the exact-gradient force field (no coplanar
-// perturbation, no quadrature fallback) for implicit differentiation of
-// minimized structures:
+// perturbation, no Simpson fallback; in the quadrature mode the flagged
+// excluded-volume pairs come from mdy_quad.hip) for implicit differentiation
+// of minimized structures:
 //   * Hessian-vector products H v, by forward-mode differentiation of the
 //     force code: bead positions are the dual numbers x + t v, every
 //     interaction's forces are evaluated in dual arithmetic and the tangent
@@ -18,6 +19,7 @@ namespace mdy {
 template <class T>
 __device__ __forceinline__ int item_forces(const Dev &D, const ItemRec &r, const V3<T> c[4], V3<T> f[4]) {
     if (r.type == IT_EV) {
+        if (D.evq && D.evq[r.j]) return -1;   // quadrature mode: evaluated by the quadrature kernels
         EvInvT<T> inv;
         ev_invariants(c[0], c[1], c[2], c[3], inv);
         ev_force_inv_t<T>(inv, c[0], c[1], c[2], c[3], r.p0, f);
@@ -184,7 +186,10 @@ static int finish(Ctx &c, hipStream_t st) {
 
 int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st) {
     Dev D = c.dev();
+    const bool quad = quad_on(c);
+    if (quad) MDY_TRY(quad_classify(c, D, d_x, st));
     hipLaunchKernelGGL(k_hvp, dim3(grid_of(c, D.n_items())), dim3(kBlock), 0, st, D, d_x, d_v, c.contrib.p);
+    if (quad) MDY_TRY(launch_quad(c, D, 2, d_x, d_v, c.contrib.p, 0, st));
     hipLaunchKernelGGL(k_hvp_gather, dim3(grid_of(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_hv);
     return finish(c, st);
 }
@@ -192,9 +197,13 @@ int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipSt
 int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_out, hipStream_t st) {
     Dev D = c.dev();
     const int nblk = grid_of(c, D.n_items());
-    MDY_TRY(ensure(c.red, (size_t)kRed * nblk));
+    const bool quad = quad_on(c);
+    const int nqb = quad ? quad_blocks(c) : 0;   // the quadrature pairs' block records follow the others
+    MDY_TRY(ensure(c.red, (size_t)kRed * (nblk + nqb)));
+    if (quad) MDY_TRY(quad_classify(c, D, d_x, st));
     hipLaunchKernelGGL(k_energy_jvp, dim3(nblk), dim3(kBlock), 0, st, D, d_x, d_w, c.red.p);
-    hipLaunchKernelGGL(k_energy_jvp_final, dim3(1), dim3(kBlock), 0, st, c.red.p, nblk, d_out);
+    if (quad) MDY_TRY(launch_quad(c, D, 3, d_x, d_w, c.red.p, nblk, st));
+    hipLaunchKernelGGL(k_energy_jvp_final, dim3(1), dim3(kBlock), 0, st, c.red.p, nblk + nqb, d_out);
     return finish(c, st);
 }
 

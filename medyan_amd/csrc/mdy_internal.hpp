@@ -41,6 +41,7 @@ struct Dev {
     const int *st_off, *st_idx;   // static CSR: bead -> bonded/cross-link slots
     const int *dy_off, *dy_idx;   // dynamic CSR: bead -> excluded-volume/boundary slots
     int coplanar_mode;
+    const unsigned char *evq;     // quadrature mode: per excluded-volume pair, 1 = evaluated by quadrature
     double cq, sq;                // cos and sin of pi/2 (C library), branching position
     // excluded-volume quadrature fallback queues (pair indices)
     int *qe, *qe_n;               // energy
@@ -99,6 +100,7 @@ struct Ctx {
     DBuf<int> bd_flag, bd_off, bd_bead, bd_plane; DBuf<double> bd_k, bd_sl; int n_bd = 0;
     std::vector<int> h_bd_nneigh;
     DBuf<int> dy_cnt, dy_off, dy_cur, dy_idx;
+    DBuf<unsigned char> evq;      // quadrature mode: per-pair flags of the current evaluation
     DBuf<int> scan_tmp;
     DBuf<double> bbox;            // 6
     // evaluation scratch

@@ -24,6 +24,16 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
 // mdy_hvp.hip
 int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st);
 int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_out, hipStream_t st);
+// mdy_quad.hip: quadrature mode (MDY_COPLANAR_QUADRATURE)
+bool quad_on(const Ctx &c);
+int quad_blocks(const Ctx &c);   // block records the quadrature energy / JVP kernels append
+// flags the pairs evaluated by quadrature at d_x and points D.evq at the flags
+int quad_classify(Ctx &c, Dev &D, const double *d_x, hipStream_t st);
+// what: 0 energy (block records from rec0), 1 forces (slots), 2 H v along d_v (slots),
+// 3 grad E_EV . d_v (block records from rec0)
+int launch_quad(Ctx &c, const Dev &D, int what, const double *d_x, const double *d_v, double *out, int rec0,
+                hipStream_t st);
+int quad_pair_count(Ctx &c, const double *d_x, hipStream_t st, int *count);
 // mdy_aux.hip
 int launch_stretch_forces(Ctx &c, const double *d_x, double *d_linker, double *d_motor, double *d_branch,
                           hipStream_t st);

@@ -1,0 +1,281 @@
+// mdy_quad.hip -- excluded-volume pairs evaluated by quadrature of their
+// defining integral (MDY_COPLANAR_QUADRATURE).
+//
+// The cylinder-cylinder excluded-volume energy is
+//     E = kr * Int_0^1 Int_0^1 ds dt / |P(s) - Q(t)|^4,
+//     P(s) = c1 + s (c2 - c1),  Q(t) = c3 + t (c4 - c3),
+// which CylinderExclVolRepulsion::energy (CylinderExclVolRepulsion.cpp:
+// 28-120) evaluates in closed form. For nearly coplanar or nearly parallel
+// cylinders that closed form is a difference of nearly equal atan terms
+// divided by a vanishing triple product: the rounding of its FP64 terms is
+// amplified as ~5e-14 / q, where q is the squared normalized triple product
+// (v1 x v2 . v3)^2 / (|v1|^2 |v2|^2 |v3|^2), so its second derivatives are
+// dominated by rounding noise there. In this mode the pairs with q < 1e-3
+// (about 2-5% of the pairs of a random network; closed-form force error
+// below ~5e-11 elsewhere) are evaluated instead by a composite 16-point
+// Gauss-Legendre rule in s and t: m equal panels per cylinder, m =
+// ceil(max length / segment distance) (at most 64), so every panel is no
+// longer than the distance to the other segment and the rule's error is
+// below ~1e-19 of the integral. Energy, forces, Hessian-vector products and
+// directional derivatives of a pair all differentiate the same discrete
+// rule, so they are mutually consistent to rounding.
+//
+// One block per flagged pair; each thread accumulates a fixed subset of the
+// nodes in a fixed order and the block sums in a fixed tree, so results are
+// reproducible bit for bit. A per-pair flag array, computed once per call by
+// k_ev_classify, tells the closed-form kernels which pairs to skip.
+#include <cmath>
+
+#include "mdy_eval.hpp"
+#include "mdy_launch.hpp"
+
+namespace mdy {
+
+constexpr double kQuadMaxQ = 1e-3;   // pairs with q below this are integrated numerically
+constexpr int kQuadPanels = 64;      // at most this many panels per cylinder
+
+// 16-point Gauss-Legendre nodes and weights on [0, 1], ascending
+struct GL16 {
+    double t[16], w[16];
+};
+
+// Newton iteration on the Legendre polynomial P_16 in long double from the
+// standard cosine initial guesses; weights 2 / ((1 - x^2) P_16'(x)^2).
+static GL16 make_gl16() {
+    GL16 g;
+    const int n = 16;
+    for (int i = 0; i < n; i++) {
+        long double x = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+        long double p0 = 1, p1 = x, dp = 1;
+        for (int it = 0; it < 100; it++) {
+            p0 = 1;
+            p1 = x;
+            for (int k = 2; k <= n; k++) {
+                const long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (x * p1 - p0) / (x * x - 1);
+            const long double dx = p1 / dp;
+            x -= dx;
+            if (fabsl(dx) < 1e-30L) break;
+        }
+        p0 = 1;
+        p1 = x;
+        for (int k = 2; k <= n; k++) {
+            const long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (x * p1 - p0) / (x * x - 1);
+        g.t[i] = (double)(0.5L * (1 - x));
+        g.w[i] = (double)(1 / ((1 - x * x) * dp * dp));   // (2 / ...) * 1/2 for [0, 1]
+    }
+    return g;
+}
+
+static const GL16 &gl16() {
+    static const GL16 g = make_gl16();
+    return g;
+}
+
+__device__ __forceinline__ d3 ld3(const double *x, int o) { return d3{x[o], x[o + 1], x[o + 2]}; }
+__device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// q = (v1 x v2 . v3)^2 / (|v1|^2 |v2|^2 |v3|^2); NaN (degenerate pair) counts as flagged
+__device__ __forceinline__ bool needs_quadrature(d3 c1, d3 c2, d3 c3, d3 c4) {
+    const d3 v1 = sub(c2, c1), v2 = sub(c4, c3), v3 = sub(c3, c1);
+    const d3 n{v1.y * v2.z - v1.z * v2.y, v1.z * v2.x - v1.x * v2.z, v1.x * v2.y - v1.y * v2.x};
+    const double tp = dot3(n, v3);
+    const double q = tp * tp / (dot3(v1, v1) * dot3(v2, v2) * dot3(v3, v3));
+    return !(q >= kQuadMaxQ);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ev_classify(Dev D, const double *__restrict__ x,
+                                                         unsigned char *__restrict__ flag) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < D.n_ev; j += gridDim.x * blockDim.x) {
+        const int4 b = D.ev_bs[j];
+        flag[j] = needs_quadrature(ld3(x, b.x), ld3(x, b.y), ld3(x, b.z), ld3(x, b.w)) ? 1 : 0;
+    }
+}
+
+__device__ __forceinline__ double clamp01(double v) { return v < 0.0 ? 0.0 : v > 1.0 ? 1.0 : v; }
+
+// Distance between the segments c1-c2 and c3-c4: the minimum over the unit
+// square of |c1 + s d1 - c3 - t d2|, from the stationary point of that
+// quadratic, clamped to the square edge by edge (for parallel segments s
+// starts at 0).
+__device__ __forceinline__ double segment_distance(d3 c1, d3 c2, d3 c3, d3 c4) {
+    const d3 d1 = sub(c2, c1), d2 = sub(c4, c3), r = sub(c1, c3);
+    const double a = dot3(d1, d1), e = dot3(d2, d2), f = dot3(d2, r), c = dot3(d1, r), b = dot3(d1, d2);
+    const double den = a * e - b * b;
+    double s = den > 1e-14 * a * e ? clamp01((b * f - c * e) / den) : 0.0;
+    double t = (b * s + f) / e;
+    if (t < 0.0) {
+        t = 0.0;
+        s = clamp01(-c / a);
+    } else if (t > 1.0) {
+        t = 1.0;
+        s = clamp01((b - c) / a);
+    }
+    const d3 p{r.x + s * d1.x - t * d2.x, r.y + s * d1.y - t * d2.y, r.z + s * d1.z - t * d2.z};
+    return sqrt(dot3(p, p));
+}
+
+enum { Q_ENERGY = 0, Q_FORCE = 1, Q_HVP = 2, Q_JVP = 3 };
+
+// W = Q_ENERGY: block records [rec0 + block] (EV energy sum, exact total);
+// Q_FORCE / Q_HVP: the pair's four force (H v) slots; Q_JVP: block records of
+// grad E_EV . w. `v` is the HVP direction or the JVP weights.
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_quad(Dev D, const double *__restrict__ x, const double *__restrict__ v,
+                                                 GL16 g, double *__restrict__ out, int rec0) {
+    constexpr int NA = W == Q_ENERGY ? 1 : 9;
+    __shared__ double lds[17 * NA + 8];
+    __shared__ double gt[16], gw[16];   // the rule, indexed per node (no dynamic indexing of kernel arguments)
+    if (threadIdx.x < 16) {
+        gt[threadIdx.x] = g.t[threadIdx.x];
+        gw[threadIdx.x] = g.w[threadIdx.x];
+    }
+    __syncthreads();
+    mdy_xsum X;
+    mdy_xs_zero(&X);
+    double blk = 0.0;
+    bool ok = true;
+    for (int j = blockIdx.x; j < D.n_ev; j += gridDim.x) {
+        if (!D.evq[j]) continue;   // uniform over the block
+        const int4 b = D.ev_bs[j];
+        const d3 c1 = ld3(x, b.x), c2 = ld3(x, b.y), c3 = ld3(x, b.z), c4 = ld3(x, b.w);
+        const d3 v1 = sub(c2, c1), v2 = sub(c4, c3), c13 = sub(c1, c3);
+        const double L = sqrt(fmax(dot3(v1, v1), dot3(v2, v2)));
+        const double dm = segment_distance(c1, c2, c3, c4);
+        const double mr = dm > 0.0 ? ceil(L / dm) : (double)kQuadPanels;
+        const int m = mr >= (double)kQuadPanels ? kQuadPanels : mr <= 1.0 ? 1 : (int)mr;
+        const int n = 16 * m, nn = n * n;
+        const double im = 1.0 / (double)m;
+        d3 e1{0, 0, 0}, e2{0, 0, 0}, e13{0, 0, 0};   // HVP: the direction at (c2 - c1), (c4 - c3), (c1 - c3)
+        if (W == Q_HVP) {
+            const d3 u1 = ld3(v, b.x), u2 = ld3(v, b.y), u3 = ld3(v, b.z), u4 = ld3(v, b.w);
+            e1 = sub(u2, u1);
+            e2 = sub(u4, u3);
+            e13 = sub(u1, u3);
+        }
+        double acc[NA];
+#pragma unroll
+        for (int k = 0; k < NA; k++) acc[k] = 0.0;
+        for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+            const int i = k / n, l = k - i * n;
+            const double s = ((double)(i >> 4) + gt[i & 15]) * im, ws = gw[i & 15] * im;
+            const double t = ((double)(l >> 4) + gt[l & 15]) * im, wt = gw[l & 15] * im;
+            const d3 r{c13.x + s * v1.x - t * v2.x, c13.y + s * v1.y - t * v2.y, c13.z + s * v1.z - t * v2.z};
+            const double d2 = dot3(r, r), inv = 1.0 / d2, inv2 = inv * inv, wst = ws * wt;
+            if (W == Q_ENERGY) {
+                acc[0] += wst * inv2;
+            } else {
+                d3 h;
+                if (W == Q_HVP) {
+                    // d/dt of the gradient of |r|^-4 along dr: -4 dr/|r|^6 + 24 (r.dr) r/|r|^8
+                    const d3 dr{e13.x + s * e1.x - t * e2.x, e13.y + s * e1.y - t * e2.y, e13.z + s * e1.z - t * e2.z};
+                    const double a = -4.0 * wst * inv2 * inv, c = 24.0 * wst * inv2 * inv2 * dot3(r, dr);
+                    h = d3{a * dr.x + c * r.x, a * dr.y + c * r.y, a * dr.z + c * r.z};
+                } else {
+                    // -grad of |r|^-4 with respect to r: 4 r / |r|^6
+                    const double a = 4.0 * wst * inv2 * inv;
+                    h = d3{a * r.x, a * r.y, a * r.z};
+                }
+                acc[0] += h.x; acc[1] += h.y; acc[2] += h.z;
+                acc[3] += s * h.x; acc[4] += s * h.y; acc[5] += s * h.z;
+                acc[6] += t * h.x; acc[7] += t * h.y; acc[8] += t * h.z;
+            }
+        }
+        block_reduce<NA>(acc, lds);
+        if (threadIdx.x == 0) {
+            const double kr = D.ev_kr[j];
+            if (W == Q_ENERGY) {
+                const double e = kr * acc[0];
+                mdy_xs_add(&X, e);
+                blk += e;
+            } else {
+                // bead weights of r = P(s) - Q(t): c1 (1 - s), c2 s, c3 -(1 - t), c4 -t. For
+                // Q_FORCE the terms were -W grad |r|^-4 (f = forces); for Q_HVP
+                // W grad^2 |r|^-4 dr (f = H v)
+                const d3 G{acc[0], acc[1], acc[2]}, Gs{acc[3], acc[4], acc[5]}, Gt{acc[6], acc[7], acc[8]};
+                d3 f[4];
+                f[0] = d3{kr * (G.x - Gs.x), kr * (G.y - Gs.y), kr * (G.z - Gs.z)};
+                f[1] = d3{kr * Gs.x, kr * Gs.y, kr * Gs.z};
+                f[2] = d3{-kr * (G.x - Gt.x), -kr * (G.y - Gt.y), -kr * (G.z - Gt.z)};
+                f[3] = d3{-kr * Gt.x, -kr * Gt.y, -kr * Gt.z};
+                ok = ok && finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]);
+                if (W == Q_JVP) {
+                    const int o[4] = {b.x, b.y, b.z, b.w};
+                    double sdot = 0.0;
+                    for (int q = 0; q < 4; q++) sdot -= f[q].x * v[o[q]] + f[q].y * v[o[q] + 1] + f[q].z * v[o[q] + 2];
+                    blk += sdot;
+                } else {
+                    // Q_FORCE: the forces; Q_HVP: H v (the terms summed were W grad^2 |r|^-4 dr)
+                    const int s0 = D.slot_ev + 4 * j;
+                    for (int q = 0; q < 4; q++) put3(out, s0 + q, f[q]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (!ok) set_err(D.err, MDY_ERR_FORCE);
+        if (W == Q_ENERGY || W == Q_JVP) {
+            double *r = out + (size_t)(rec0 + blockIdx.x) * kRed;
+            for (int k = 0; k < kNFF; k++) r[k] = k == MDY_FF_EV ? blk : 0.0;
+            if (W == Q_ENERGY) store_xsum(out, rec0 + blockIdx.x, kXsE, X);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+bool quad_on(const Ctx &c) {
+    return c.opts.coplanar_mode == MDY_COPLANAR_QUADRATURE && !c.ev_mon && c.n_ev > 0;
+}
+
+int quad_blocks(const Ctx &c) {
+    const int g = c.num_cu * 4;
+    return c.n_ev < g ? (c.n_ev < 1 ? 1 : c.n_ev) : g;
+}
+
+int quad_classify(Ctx &c, Dev &D, const double *d_x, hipStream_t st) {
+    MDY_TRY(ensure(c.evq, (size_t)c.n_ev));
+    long long g = ((long long)c.n_ev + kBlock - 1) / kBlock, cap = (long long)c.num_cu * 8;
+    hipLaunchKernelGGL(k_ev_classify, dim3((int)(g > cap ? cap : g < 1 ? 1 : g)), dim3(kBlock), 0, st, D, d_x,
+                       c.evq.p);
+    D.evq = c.evq.p;
+    return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
+}
+
+int launch_quad(Ctx &c, const Dev &D, int what, const double *d_x, const double *d_v, double *out, int rec0,
+                hipStream_t st) {
+    const dim3 grid(quad_blocks(c)), blk(kBlock);
+    const GL16 &g = gl16();
+    switch (what) {
+        case 0: hipLaunchKernelGGL(k_quad<Q_ENERGY>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
+        case 1: hipLaunchKernelGGL(k_quad<Q_FORCE>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
+        case 2: hipLaunchKernelGGL(k_quad<Q_HVP>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
+        default: hipLaunchKernelGGL(k_quad<Q_JVP>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
+    }
+    return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
+}
+
+int quad_pair_count(Ctx &c, const double *d_x, hipStream_t st, int *count) {
+    *count = 0;
+    if (!quad_on(c)) return MDY_OK;
+    Dev D = c.dev();
+    MDY_TRY(quad_classify(c, D, d_x, st));
+    std::vector<unsigned char> h((size_t)c.n_ev);
+    MDY_HIP(hipMemcpyAsync(h.data(), c.evq.p, h.size(), hipMemcpyDeviceToHost, st));
+    MDY_HIP(hipStreamSynchronize(st));
+    int n = 0;
+    for (unsigned char f : h) n += f;
+    *count = n;
+    return MDY_OK;
+}
+
+}  // namespace mdy

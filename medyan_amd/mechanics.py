@@ -79,6 +79,7 @@ def load_library(path: str = LIB_PATH):
         "mdy_energy_dev_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "mdy_hvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "mdy_energy_jvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mdy_quadrature_pair_count": (C.c_int, [vp, f64p, i32p]),
         "mdy_minimize": (C.c_int, [vp, C.POINTER(_abi.CGParams), f64p, f64p, C.POINTER(_abi.MinResult)]),
         "mdy_minimize_dev": (C.c_int, [vp, C.POINTER(_abi.CGParams), vp, vp, C.POINTER(_abi.MinResult)]),
         "mdy_minimize_batch": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(_abi.CGParams), C.POINTER(vp),
@@ -191,9 +192,15 @@ class Context:
             raise MdyError(rc, self.lib.mdy_last_error(self.h).decode())
 
     def set_opts(self, **kw):
+        """Changes context options (mdy_set_opts); on refusal the options stay as they were."""
+        old = {k: getattr(self.opts, k) for k in kw}
         for k, v in kw.items():
             setattr(self.opts, k, v)
-        self._check(self.lib.mdy_set_opts(self.h, C.byref(self.opts)))
+        rc = self.lib.mdy_set_opts(self.h, C.byref(self.opts))
+        if rc != 0:
+            for k, v in old.items():
+                setattr(self.opts, k, v)
+        self._check(rc)
 
     # -- system ----------------------------------------------------------------
     def set_network(self, net: Network):
@@ -342,6 +349,14 @@ class Context:
         """d_out[k] = grad E_k . w per force field, then the sum (MDY_NFF + 1 doubles)."""
         self._check(self.lib.mdy_energy_jvp_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_w), C.c_void_p(d_out),
                                                 C.c_void_p(stream) if stream else None))
+
+    def quadrature_pair_count(self, coord=None) -> int:
+        """Excluded-volume pairs that MDY_COPLANAR_QUADRATURE evaluates by
+        quadrature at coord (default: the network's coordinates)."""
+        x = self._coord(coord)
+        n = C.c_int32(0)
+        self._check(self.lib.mdy_quadrature_pair_count(self.h, _f64(x), C.byref(n)))
+        return int(n.value)
 
     def search_dir(self):
         """CGMethod::searchDir after the last minimization (see mdy_get_search_dir)."""
