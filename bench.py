@@ -333,7 +333,11 @@ def main():
             "final_energy": float(emx),
             "final_energy_rel_diff_vs_fp64": abs(emx - e64) / abs(e64),
             "note": ("both forms stop below 1 pN; they follow different trajectories and settle in "
-                     "different local minima, so final energies differ by percent, not by rounding"),
+                     "different local minima, so final energies differ by percent, not by rounding, and the "
+                     "minimizations/s of the two forms are not a precision speed-up: compare "
+                     "us_per_cg_iteration (the iteration is latency-bound in both). At tighter tolerances the "
+                     "final-energy difference stays at the size of FP64's own sensitivity to a 1e-9 nm "
+                     "change of the start (--workload mixed_tight)"),
         }
         del mctx
     out = {
