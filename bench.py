@@ -391,9 +391,9 @@ def bench_ensemble(a, lib):
     to 1 pN together by one batched launch in the mixed form. One step = one
     batched minimization of the whole ensemble from its initial coordinates
     (tables and neighbor lists resident), in passes whose force-call caps
-    move the blocks of finished networks to the slow ones. Threads per block
-    and the first pass's cap come from an autotuning pass over the
-    candidates before the warmup."""
+    move the blocks of finished networks to the slow ones. Threads per block,
+    the first pass's cap and the cluster shares come from the library's
+    autotuner (medyan_amd.ensemble.autotune) before the warmup."""
     import numpy as np
     import torch
     from medyan_amd import _abi, ensemble, mechanics
@@ -414,24 +414,24 @@ def bench_ensemble(a, lib):
     ncyl = np.array([n.n_cyl for n in nets], dtype=np.float64)
     print(f"[ensemble256] {len(nets)} networks ready", file=sys.stderr, flush=True)
 
-    def run(tpb, first, prm=params):
+    def restore():
         for x, x_0 in zip(xs, x0):
             x.copy_(x_0)
         torch.cuda.synchronize()
-        return mechanics.minimize_batch(ctxs, prm, ptrs, threads_per_block=tpb, first_pass_calls=first)
 
-    tune = {}
-    for tpb in (64, 128):
-        for first in (250, 500, 1000, 2000):
-            tune[(tpb, first)] = run(tpb, first)[2]
-            print(f"[ensemble256] autotune {tpb} threads, first pass {first} force calls: "
-                  f"{tune[(tpb, first)]:.1f} ms", file=sys.stderr, flush=True)
-    tpb, first = min(tune, key=tune.get)
+    def run(prm=params):
+        restore()
+        return mechanics.minimize_batch(ctxs, prm, ptrs, **best)
+
+    best, tune = ensemble.autotune(ctxs, params, ptrs, restore,
+                                   log=lambda m: print(f"[ensemble256] {m}", file=sys.stderr, flush=True))
+    tpb, first = best["threads_per_block"], best["first_pass_calls"]
+    share_exp = min((k for k in tune if k[0] == tpb and k[1] == first), key=tune.get)[2]
     for _ in range(a.warmup):
-        run(tpb, first)
+        run()
     ms_steps, per_net = [], []
     for _ in range(a.steps):
-        res, ms, tot = run(tpb, first)
+        res, ms, tot = run()
         ms_steps.append(tot)
         per_net.append(ms)
         print(f"[ensemble256] step: {tot:.1f} ms", file=sys.stderr, flush=True)
@@ -449,7 +449,7 @@ def bench_ensemble(a, lib):
     traffic = traffic_from_profiles("ensemble256")
     # the reference's default stopping rule (force OR relative energy change < 1e-9)
     dparams = [_abi.default_cg_params(n.n_beads, grad_tol=1.0) for n in nets]
-    _, _, default_ms = run(tpb, first, dparams)
+    _, _, default_ms = run(dparams)
     # the same networks one after another, each on the whole GPU (single-network kernel)
     seq_ms = 0.0
     for c, x, x_0, p in zip(ctxs, xs, x0, params):
@@ -477,7 +477,9 @@ def bench_ensemble(a, lib):
                          "energy criterion off; tables and neighbor lists resident; " + generator_rule(lib)),
             "threads_per_block": tpb,
             "first_pass_force_calls": first,
-            "autotune_ms": {f"{k[0]} threads, first pass {k[1]}": round(v, 1) for k, v in tune.items()},
+            "cluster_share_exponent": share_exp,
+            "autotune_ms": {f"{k[0]} threads, first pass {k[1]}, shares work^{k[2]}": round(v, 1)
+                            for k, v in tune.items()},
             "per_network_ms_p50": float(np.percentile(per_net, 50)),
             "per_network_ms_p99": float(np.percentile(per_net, 99)),
             "per_network_ms_max": float(per_net.max()),

@@ -36,15 +36,50 @@ def make_ensemble(lib, n=256, seed=SEED, sizes=None):
     return [ensemble_network(lib, int(c), i, seed) for i, c in enumerate(sizes)]
 
 
-def autotune(ctxs, d_coords, coords0, params, candidates=(64, 128, 256), copy=None):
-    """Threads per block of the batched launch: each candidate minimizes the
-    ensemble once from the same coordinates; the fastest is returned with
-    all timings. copy(i, src) restores network i's coordinates."""
+BONDED_TABLES = ("fs_k", "fb_k", "ls_k", "ms_k", "bs_k", "bb_k", "bdh_k", "bp_k")
+
+
+def work(ctx):
+    """A network's per-evaluation work as the batched launch sizes its block
+    cluster by default: its interactions (bonded, excluded volume, boundary)
+    plus 2 per bead."""
+    n = ctx.net
+    bonded = sum(len(getattr(n, f)) for f in BONDED_TABLES)
+    return bonded + len(ctx.exvol_table()[1]) + len(ctx.boundary_table()[1]) + 2 * n.n_beads
+
+
+def autotune(ctxs, params, d_coords, restore, threads=(64, 128, 256), first_pass=(250, 500, 1000, 2000),
+             exponents=(1.0, 1.25), log=None):
+    """Block and tile shapes of the batched launch (mdy_minimize_batch) for
+    one ensemble, by coordinate search over measured runs:
+      1. threads per block, the unit every network's cluster is built from
+         (64: more, smaller blocks per network; 256: fewer, wider ones);
+      2. the force-call cap of the first pass (how long each network keeps
+         its first share of blocks before the blocks of finished networks
+         are moved to the ones still running);
+      3. the exponent a of the cluster shares w_i = work_i ** a (a > 1 gives
+         the large networks more than their share of the blocks).
+    Each stage keeps the best value of the previous ones. Each candidate
+    minimizes the whole ensemble once from the same coordinates (restore()
+    puts them back); every candidate gives the same results bit for bit,
+    only the time differs. Returns (settings for minimize_batch: dict with
+    threads_per_block, first_pass_calls, weights; {(threads, cap, a): ms})."""
+    wk = np.array([work(c) for c in ctxs], dtype=np.float64)
     times = {}
-    for tpb in candidates:
-        for i in range(len(ctxs)):
-            copy(i, coords0[i])
-        _, _, tot = mechanics.minimize_batch(ctxs, params, d_coords, threads_per_block=tpb)
-        times[tpb] = tot
-    best = min(times, key=times.get)
-    return best, times
+
+    def run(tpb, first, a):
+        key = (tpb, first, a)
+        if key not in times:
+            restore()
+            _, _, tot = mechanics.minimize_batch(ctxs, params, d_coords, threads_per_block=tpb,
+                                                 weights=wk ** a, first_pass_calls=first)
+            times[key] = tot
+            if log:
+                log(f"autotune: {tpb} threads, first pass {first} force calls, shares work^{a}: {tot:.1f} ms")
+        return times[key]
+
+    first0 = first_pass[min(1, len(first_pass) - 1)]
+    tpb = min(threads, key=lambda t: run(t, first0, 1.0))
+    first = min(first_pass, key=lambda f: run(tpb, f, 1.0))
+    a = min(exponents, key=lambda e: run(tpb, first, e))
+    return dict(threads_per_block=tpb, first_pass_calls=first, weights=wk ** a), times

@@ -122,3 +122,34 @@ def test_batch_argument_errors(lib):
     with pytest.raises(mechanics.MdyError) as e:
         mechanics.minimize_batch([c, m], [p, p], [x.data_ptr(), x.data_ptr()])
     assert e.value.code == _abi.MDY_ERR_ARG
+
+
+def test_autotuner_choice_leaves_results_unchanged(lib):
+    """medyan_amd.ensemble.autotune times each candidate (threads per block,
+    first-pass cap, cluster-share exponent) on the ensemble and returns the
+    fastest; whatever it picks, the minimizations equal the default launch's
+    bit for bit."""
+    nets = nets_small(lib)
+    ctxs = [ctx_for(net, _abi.MDY_PREC_MIXED) for net in nets]
+    x0 = [torch.tensor(net.coord, device="cuda") for net in nets]
+    xs = [x.clone() for x in x0]
+    ptrs = [x.data_ptr() for x in xs]
+    params = [_abi.default_cg_params(net.n_beads, energy_rel_tol=0.0) for net in nets]
+
+    def restore():
+        for x, x_0 in zip(xs, x0):
+            x.copy_(x_0)
+        torch.cuda.synchronize()
+
+    best, times = ensemble.autotune(ctxs, params, ptrs, restore, threads=(64, 128), first_pass=(40, 400),
+                                    exponents=(1.0, 1.5), log=lambda m: print(f"[gpu] {m}", flush=True))
+    assert len(times) == 2 + 1 + 1 and best["threads_per_block"] in (64, 128)
+    assert best["first_pass_calls"] in (40, 400) and len(best["weights"]) == len(nets)
+    restore()
+    res_b, _, _ = mechanics.minimize_batch(ctxs, params, ptrs, **best)
+    got = [x.cpu().numpy() for x in xs]
+    restore()
+    res_d, _, _ = mechanics.minimize_batch(ctxs, params, ptrs)
+    for i, x in enumerate(xs):
+        assert same(got[i], x.cpu().numpy()), i
+        assert res_b[i].num_iter == res_d[i].num_iter and res_b[i].grad_inf_norm == res_d[i].grad_inf_norm
