@@ -172,7 +172,8 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.pairs); release(c.ev_bs); release(c.ev_kr); release(c.bd_flag); release(c.bd_off); release(c.bd_bead);
     release(c.bd_plane); release(c.bd_k); release(c.bd_sl); release(c.dy_cnt); release(c.dy_off); release(c.dy_cur);
     release(c.dy_idx); release(c.scan_tmp); release(c.bbox); release(c.x); release(c.xt); release(c.f);
-    release(c.batch_args); release(c.batch_map); release(c.evq);
+    release(c.batch_args); release(c.batch_map); release(c.evq); release(c.qpan); release(c.qcnt); release(c.qoff);
+    release(c.qmap); release(c.qpart); release(c.qpe); release(c.qtot);
     release(c.contrib); release(c.red); release(c.out); release(c.qe); release(c.qf); release(c.qn); release(c.err);
     release(c.qe_u); release(c.simpson_part); release(c.F0); release(c.F1); release(c.S0); release(c.S1);
     release(c.xmin); release(c.bar); release(c.cg_out); release(c.tr_lam); release(c.tr_e); release(c.tr_mf);

@@ -101,6 +101,9 @@ struct Ctx {
     std::vector<int> h_bd_nneigh;
     DBuf<int> dy_cnt, dy_off, dy_cur, dy_idx;
     DBuf<unsigned char> evq;      // quadrature mode: per-pair flags of the current evaluation
+    DBuf<int> qpan, qcnt, qoff, qmap; // quadrature mode: panels, chunk counts, first chunks, chunk -> pair
+    DBuf<int> qtot;               // quadrature mode: running chunk total (zero between calls)
+    DBuf<double> qpart, qpe;      // quadrature mode: chunk records, per-pair values
     DBuf<int> scan_tmp;
     DBuf<double> bbox;            // 6
     // evaluation scratch

@@ -20,10 +20,13 @@
 // directional derivatives of a pair all differentiate the same discrete
 // rule, so they are mutually consistent to rounding.
 //
-// One block per flagged pair; each thread accumulates a fixed subset of the
-// nodes in a fixed order and the block sums in a fixed tree, so results are
-// reproducible bit for bit. A per-pair flag array, computed once per call by
-// k_ev_classify, tells the closed-form kernels which pairs to skip.
+// Work is split into chunks of at most kQuadChunk nodes (a pair with many
+// panels spans several chunks, so the closest pair does not set the kernel's
+// time): k_ev_classify flags the pairs and places each one's chunks, one
+// block per chunk sums its nodes (each thread a fixed subset in a fixed
+// order, then a fixed tree) into a chunk record, and one thread per pair adds
+// its chunk records in chunk order. Results are reproducible bit for bit.
+// The closed-form kernels skip the flagged pairs.
 #include <cmath>
 
 #include "mdy_eval.hpp"
@@ -33,6 +36,7 @@ namespace mdy {
 
 constexpr double kQuadMaxQ = 1e-3;   // pairs with q below this are integrated numerically
 constexpr int kQuadPanels = 64;      // at most this many panels per cylinder
+constexpr int kQuadChunk = 65536;    // nodes per chunk (256 per thread)
 
 // 16-point Gauss-Legendre nodes and weights on [0, 1], ascending
 struct GL16 {
@@ -91,13 +95,50 @@ __device__ __forceinline__ bool needs_quadrature(d3 c1, d3 c2, d3 c3, d3 c4) {
     return !(q >= kQuadMaxQ);
 }
 
+// the distance helper is defined below; declared here for the classifier
+__device__ __forceinline__ double segment_distance(d3 c1, d3 c2, d3 c3, d3 c4);
+
+// panels per cylinder: each no longer than the segments' distance (at most kQuadPanels)
+__device__ __forceinline__ int quad_panels(d3 c1, d3 c2, d3 c3, d3 c4) {
+    const d3 v1 = sub(c2, c1), v2 = sub(c4, c3);
+    const double L = sqrt(fmax(dot3(v1, v1), dot3(v2, v2)));
+    const double dm = segment_distance(c1, c2, c3, c4);
+    const double mr = dm > 0.0 ? ceil(L / dm) : (double)kQuadPanels;
+    return mr >= (double)kQuadPanels ? kQuadPanels : mr <= 1.0 ? 1 : (int)mr;
+}
+
+// flag[j]; for a flagged pair the panel count pan[j], its chunk count
+// cnt[j] and the index off[j] of its first chunk, taken from the running
+// total *tot (atomically: the order of pairs among the chunks changes no
+// result, since each chunk's nodes and each pair's sum over its chunks in
+// chunk order are fixed), and the chunk -> pair map entries
 __global__ void __launch_bounds__(kBlock) k_ev_classify(Dev D, const double *__restrict__ x,
-                                                         unsigned char *__restrict__ flag) {
+                                                         unsigned char *__restrict__ flag, int *__restrict__ pan,
+                                                         int *__restrict__ cnt, int *__restrict__ off,
+                                                         int *__restrict__ cmap, int *__restrict__ tot, int cap) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < D.n_ev; j += gridDim.x * blockDim.x) {
         const int4 b = D.ev_bs[j];
-        flag[j] = needs_quadrature(ld3(x, b.x), ld3(x, b.y), ld3(x, b.z), ld3(x, b.w)) ? 1 : 0;
+        const d3 c1 = ld3(x, b.x), c2 = ld3(x, b.y), c3 = ld3(x, b.z), c4 = ld3(x, b.w);
+        const bool f = needs_quadrature(c1, c2, c3, c4);
+        flag[j] = f ? 1 : 0;
+        int nc = 0;
+        if (f) {
+            const int m = quad_panels(c1, c2, c3, c4);
+            pan[j] = m;
+            nc = (256 * m * m + kQuadChunk - 1) / kQuadChunk;   // (16 m)^2 nodes
+            const int base = atomicAdd(tot, nc);
+            off[j] = base;
+            if (base + nc <= cap) {
+                for (int q = 0; q < nc; q++) cmap[base + q] = j;
+            } else {
+                set_err(D.err, MDY_ERR_CAPACITY);   // cannot happen with a cleared total (16 chunks per pair at most)
+            }
+        }
+        cnt[j] = nc;
     }
 }
+
+__global__ void k_quad_zero(int *tot) { *tot = 0; }
 
 __device__ __forceinline__ double clamp01(double v) { return v < 0.0 ? 0.0 : v > 1.0 ? 1.0 : v; }
 
@@ -123,14 +164,19 @@ __device__ __forceinline__ double segment_distance(d3 c1, d3 c2, d3 c3, d3 c4) {
 }
 
 enum { Q_ENERGY = 0, Q_FORCE = 1, Q_HVP = 2, Q_JVP = 3 };
+template <int W> struct QuadAcc { static constexpr int N = W == Q_ENERGY ? 1 : 9; };
 
-// W = Q_ENERGY: block records [rec0 + block] (EV energy sum, exact total);
-// Q_FORCE / Q_HVP: the pair's four force (H v) slots; Q_JVP: block records of
-// grad E_EV . w. `v` is the HVP direction or the JVP weights.
+// One block per chunk: the chunk's nodes of its pair into part[chunk]. For
+// Q_ENERGY the sum of W |r|^-4; otherwise G, sum of s h, sum of t h with
+// h = W * 4 r/|r|^6 (Q_FORCE, Q_JVP: minus the gradient of |r|^-4) or
+// h = W grad^2 |r|^-4 dr (Q_HVP, dr the direction v along r).
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_quad(Dev D, const double *__restrict__ x, const double *__restrict__ v,
-                                                 GL16 g, double *__restrict__ out, int rec0) {
-    constexpr int NA = W == Q_ENERGY ? 1 : 9;
+__global__ void __launch_bounds__(kBlock) k_quad_chunks(Dev D, const double *__restrict__ x,
+                                                        const double *__restrict__ v, GL16 g,
+                                                        const int *__restrict__ off, const int *__restrict__ cmap,
+                                                        const int *__restrict__ pan, const int *__restrict__ tot, int cap,
+                                                        double *__restrict__ part) {
+    constexpr int NA = QuadAcc<W>::N;
     __shared__ double lds[17 * NA + 8];
     __shared__ double gt[16], gw[16];   // the rule, indexed per node (no dynamic indexing of kernel arguments)
     if (threadIdx.x < 16) {
@@ -138,20 +184,15 @@ __global__ void __launch_bounds__(kBlock) k_quad(Dev D, const double *__restrict
         gw[threadIdx.x] = g.w[threadIdx.x];
     }
     __syncthreads();
-    mdy_xsum X;
-    mdy_xs_zero(&X);
-    double blk = 0.0;
-    bool ok = true;
-    for (int j = blockIdx.x; j < D.n_ev; j += gridDim.x) {
-        if (!D.evq[j]) continue;   // uniform over the block
+    const int total = min(*tot, cap);
+    for (int ch = blockIdx.x; ch < total; ch += gridDim.x) {
+        const int j = cmap[ch];
         const int4 b = D.ev_bs[j];
         const d3 c1 = ld3(x, b.x), c2 = ld3(x, b.y), c3 = ld3(x, b.z), c4 = ld3(x, b.w);
         const d3 v1 = sub(c2, c1), v2 = sub(c4, c3), c13 = sub(c1, c3);
-        const double L = sqrt(fmax(dot3(v1, v1), dot3(v2, v2)));
-        const double dm = segment_distance(c1, c2, c3, c4);
-        const double mr = dm > 0.0 ? ceil(L / dm) : (double)kQuadPanels;
-        const int m = mr >= (double)kQuadPanels ? kQuadPanels : mr <= 1.0 ? 1 : (int)mr;
+        const int m = pan[j];
         const int n = 16 * m, nn = n * n;
+        const int k0 = (ch - off[j]) * kQuadChunk, k1 = min(nn, k0 + kQuadChunk);
         const double im = 1.0 / (double)m;
         d3 e1{0, 0, 0}, e2{0, 0, 0}, e13{0, 0, 0};   // HVP: the direction at (c2 - c1), (c4 - c3), (c1 - c3)
         if (W == Q_HVP) {
@@ -163,7 +204,7 @@ __global__ void __launch_bounds__(kBlock) k_quad(Dev D, const double *__restrict
         double acc[NA];
 #pragma unroll
         for (int k = 0; k < NA; k++) acc[k] = 0.0;
-        for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
             const int i = k / n, l = k - i * n;
             const double s = ((double)(i >> 4) + gt[i & 15]) * im, ws = gw[i & 15] * im;
             const double t = ((double)(l >> 4) + gt[l & 15]) * im, wt = gw[l & 15] * im;
@@ -189,44 +230,87 @@ __global__ void __launch_bounds__(kBlock) k_quad(Dev D, const double *__restrict
             }
         }
         block_reduce<NA>(acc, lds);
-        if (threadIdx.x == 0) {
-            const double kr = D.ev_kr[j];
-            if (W == Q_ENERGY) {
-                const double e = kr * acc[0];
-                mdy_xs_add(&X, e);
-                blk += e;
+        if (threadIdx.x < NA) part[(size_t)ch * NA + threadIdx.x] = acc[threadIdx.x];
+    }
+}
+
+// One thread per pair: its chunk records added in chunk order. Q_FORCE /
+// Q_HVP: the pair's four slots; Q_ENERGY: pe[j] = its energy; Q_JVP:
+// pe[j] = grad E_pair . w (pe[j] = 0 for pairs left to the closed form).
+// The running chunk total is cleared for the next call (this kernel follows
+// the chunk kernel on the stream, so every chunk has run).
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_quad_finish(Dev D, const double *__restrict__ v,
+                                                        const int *__restrict__ off, const int *__restrict__ cnt,
+                                                        const double *__restrict__ part, double *__restrict__ slots,
+                                                        double *__restrict__ pe, int *__restrict__ tot, int cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *tot = 0;
+    constexpr int NA = QuadAcc<W>::N;
+    bool ok = true;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < D.n_ev; j += gridDim.x * blockDim.x) {
+        if (!D.evq[j]) {
+            if (W == Q_ENERGY || W == Q_JVP) pe[j] = 0.0;
+            continue;
+        }
+        double acc[NA];
+#pragma unroll
+        for (int k = 0; k < NA; k++) acc[k] = 0.0;
+        for (int q = off[j], qe = min(off[j] + cnt[j], cap); q < qe; q++)
+#pragma unroll
+            for (int k = 0; k < NA; k++) acc[k] += part[(size_t)q * NA + k];
+        const double kr = D.ev_kr[j];
+        if constexpr (W == Q_ENERGY) {
+            pe[j] = kr * acc[0];
+        } else {
+            // bead weights of r = P(s) - Q(t): c1 (1 - s), c2 s, c3 -(1 - t), c4 -t. For
+            // Q_FORCE / Q_JVP the terms were -W grad |r|^-4 (f = forces); for Q_HVP
+            // W grad^2 |r|^-4 dr (f = H v)
+            const d3 G{acc[0], acc[1], acc[2]}, Gs{acc[3], acc[4], acc[5]}, Gt{acc[6], acc[7], acc[8]};
+            d3 f[4];
+            f[0] = d3{kr * (G.x - Gs.x), kr * (G.y - Gs.y), kr * (G.z - Gs.z)};
+            f[1] = d3{kr * Gs.x, kr * Gs.y, kr * Gs.z};
+            f[2] = d3{-kr * (G.x - Gt.x), -kr * (G.y - Gt.y), -kr * (G.z - Gt.z)};
+            f[3] = d3{-kr * Gt.x, -kr * Gt.y, -kr * Gt.z};
+            ok = ok && finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]);
+            if constexpr (W == Q_JVP) {
+                const int4 b = D.ev_bs[j];
+                const int o[4] = {b.x, b.y, b.z, b.w};
+                double sdot = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) sdot -= f[q].x * v[o[q]] + f[q].y * v[o[q] + 1] + f[q].z * v[o[q] + 2];
+                pe[j] = sdot;
             } else {
-                // bead weights of r = P(s) - Q(t): c1 (1 - s), c2 s, c3 -(1 - t), c4 -t. For
-                // Q_FORCE the terms were -W grad |r|^-4 (f = forces); for Q_HVP
-                // W grad^2 |r|^-4 dr (f = H v)
-                const d3 G{acc[0], acc[1], acc[2]}, Gs{acc[3], acc[4], acc[5]}, Gt{acc[6], acc[7], acc[8]};
-                d3 f[4];
-                f[0] = d3{kr * (G.x - Gs.x), kr * (G.y - Gs.y), kr * (G.z - Gs.z)};
-                f[1] = d3{kr * Gs.x, kr * Gs.y, kr * Gs.z};
-                f[2] = d3{-kr * (G.x - Gt.x), -kr * (G.y - Gt.y), -kr * (G.z - Gt.z)};
-                f[3] = d3{-kr * Gt.x, -kr * Gt.y, -kr * Gt.z};
-                ok = ok && finite_d3(f[0]) && finite_d3(f[1]) && finite_d3(f[2]) && finite_d3(f[3]);
-                if (W == Q_JVP) {
-                    const int o[4] = {b.x, b.y, b.z, b.w};
-                    double sdot = 0.0;
-                    for (int q = 0; q < 4; q++) sdot -= f[q].x * v[o[q]] + f[q].y * v[o[q] + 1] + f[q].z * v[o[q] + 2];
-                    blk += sdot;
-                } else {
-                    // Q_FORCE: the forces; Q_HVP: H v (the terms summed were W grad^2 |r|^-4 dr)
-                    const int s0 = D.slot_ev + 4 * j;
-                    for (int q = 0; q < 4; q++) put3(out, s0 + q, f[q]);
-                }
+                const int s0 = D.slot_ev + 4 * j;
+#pragma unroll
+                for (int q = 0; q < 4; q++) put3(slots, s0 + q, f[q]);
             }
         }
-        __syncthreads();
     }
+    if (!ok) set_err(D.err, MDY_ERR_FORCE);
+}
+
+// Block records [rec0 + block] of the per-pair values: pairs j = block * 256
+// + thread + k * (grid * 256), each thread's share summed in order, then the
+// block's fixed tree; EV entry = that double sum, and for Q_ENERGY the exact
+// total in the record's accumulator.
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_quad_records(Dev D, const double *__restrict__ pe, double *__restrict__ out,
+                                                         int rec0) {
+    __shared__ double lds[17 * 8 + 8];
+    double s[1] = {0.0};
+    mdy_xsum X;
+    mdy_xs_zero(&X);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < D.n_ev; j += gridDim.x * blockDim.x) {
+        if (!D.evq[j]) continue;
+        s[0] += pe[j];
+        if (W == Q_ENERGY) mdy_xs_add(&X, pe[j]);
+    }
+    block_reduce<1>(s, lds);
+    if (W == Q_ENERGY) block_reduce_xsum(X, reinterpret_cast<long long *>(lds));
     if (threadIdx.x == 0) {
-        if (!ok) set_err(D.err, MDY_ERR_FORCE);
-        if (W == Q_ENERGY || W == Q_JVP) {
-            double *r = out + (size_t)(rec0 + blockIdx.x) * kRed;
-            for (int k = 0; k < kNFF; k++) r[k] = k == MDY_FF_EV ? blk : 0.0;
-            if (W == Q_ENERGY) store_xsum(out, rec0 + blockIdx.x, kXsE, X);
-        }
+        double *r = out + (size_t)(rec0 + blockIdx.x) * kRed;
+        for (int k = 0; k < kNFF; k++) r[k] = k == MDY_FF_EV ? s[0] : 0.0;
+        if (W == Q_ENERGY) store_xsum(out, rec0 + blockIdx.x, kXsE, X);
     }
 }
 
@@ -243,23 +327,51 @@ int quad_blocks(const Ctx &c) {
 }
 
 int quad_classify(Ctx &c, Dev &D, const double *d_x, hipStream_t st) {
-    MDY_TRY(ensure(c.evq, (size_t)c.n_ev));
-    long long g = ((long long)c.n_ev + kBlock - 1) / kBlock, cap = (long long)c.num_cu * 8;
-    hipLaunchKernelGGL(k_ev_classify, dim3((int)(g > cap ? cap : g < 1 ? 1 : g)), dim3(kBlock), 0, st, D, d_x,
-                       c.evq.p);
+    const size_t n = (size_t)c.n_ev;
+    MDY_TRY(ensure(c.evq, n));
+    MDY_TRY(ensure(c.qpan, n));
+    MDY_TRY(ensure(c.qcnt, n));
+    MDY_TRY(ensure(c.qoff, n));
+    MDY_TRY(ensure(c.qmap, 16 * n));   // at most (16 * kQuadPanels)^2 / kQuadChunk = 16 chunks per pair
+    // everything the quadrature kernels use is allocated before the chunk total
+    // is taken, so nothing fails between this call and the kernel that clears it
+    MDY_TRY(ensure(c.qpart, 16 * n * 9));
+    MDY_TRY(ensure(c.qpe, n));
+    if (!c.qtot.p) {
+        MDY_TRY(ensure(c.qtot, 1));
+        hipLaunchKernelGGL(k_quad_zero, dim3(1), dim3(1), 0, st, c.qtot.p);
+    }
+    long long g = ((long long)n + kBlock - 1) / kBlock, cap = (long long)c.num_cu * 8;
+    const dim3 grid((int)(g > cap ? cap : g < 1 ? 1 : g));
+    hipLaunchKernelGGL(k_ev_classify, grid, dim3(kBlock), 0, st, D, d_x, c.evq.p, c.qpan.p, c.qcnt.p, c.qoff.p,
+                       c.qmap.p, c.qtot.p, (int)(16 * n));
     D.evq = c.evq.p;
     return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
 }
 
+template <int W>
+static void quad_launch(Ctx &c, const Dev &D, const double *d_x, const double *d_v, double *out, int rec0,
+                        hipStream_t st) {
+    const long long n = c.n_ev;
+    const dim3 blk(kBlock), chunks(c.num_cu * 8);
+    long long g = (n + kBlock - 1) / kBlock, cap = (long long)c.num_cu * 8;
+    const dim3 pairs((int)(g > cap ? cap : g < 1 ? 1 : g));
+    const int ccap = (int)(16 * n);
+    hipLaunchKernelGGL(k_quad_chunks<W>, chunks, blk, 0, st, D, d_x, d_v, gl16(), c.qoff.p, c.qmap.p, c.qpan.p,
+                       c.qtot.p, ccap, c.qpart.p);
+    hipLaunchKernelGGL(k_quad_finish<W>, pairs, blk, 0, st, D, d_v, c.qoff.p, c.qcnt.p, c.qpart.p, out, c.qpe.p,
+                       c.qtot.p, ccap);
+    if (W == Q_ENERGY || W == Q_JVP)
+        hipLaunchKernelGGL(k_quad_records<W>, dim3(quad_blocks(c)), blk, 0, st, D, c.qpe.p, out, rec0);
+}
+
 int launch_quad(Ctx &c, const Dev &D, int what, const double *d_x, const double *d_v, double *out, int rec0,
                 hipStream_t st) {
-    const dim3 grid(quad_blocks(c)), blk(kBlock);
-    const GL16 &g = gl16();
     switch (what) {
-        case 0: hipLaunchKernelGGL(k_quad<Q_ENERGY>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
-        case 1: hipLaunchKernelGGL(k_quad<Q_FORCE>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
-        case 2: hipLaunchKernelGGL(k_quad<Q_HVP>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
-        default: hipLaunchKernelGGL(k_quad<Q_JVP>, grid, blk, 0, st, D, d_x, d_v, g, out, rec0); break;
+        case 0: quad_launch<Q_ENERGY>(c, D, d_x, d_v, out, rec0, st); break;
+        case 1: quad_launch<Q_FORCE>(c, D, d_x, d_v, out, rec0, st); break;
+        case 2: quad_launch<Q_HVP>(c, D, d_x, d_v, out, rec0, st); break;
+        default: quad_launch<Q_JVP>(c, D, d_x, d_v, out, rec0, st); break;
     }
     return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
 }
@@ -269,6 +381,7 @@ int quad_pair_count(Ctx &c, const double *d_x, hipStream_t st, int *count) {
     if (!quad_on(c)) return MDY_OK;
     Dev D = c.dev();
     MDY_TRY(quad_classify(c, D, d_x, st));
+    hipLaunchKernelGGL(k_quad_zero, dim3(1), dim3(1), 0, st, c.qtot.p);   // no quadrature follows: clear the total
     std::vector<unsigned char> h((size_t)c.n_ev);
     MDY_HIP(hipMemcpyAsync(h.data(), c.evq.p, h.size(), hipMemcpyDeviceToHost, st));
     MDY_HIP(hipStreamSynchronize(st));
