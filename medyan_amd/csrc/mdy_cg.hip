@@ -139,7 +139,7 @@ struct Cg {
         for (int b = tid; b < G; b += blockDim.x) {
             long long spins = 0;
             while (__hip_atomic_load(flags + (size_t)b * 16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-                __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_s_sleep(64);   // ~4k cycles between polls: less L2 traffic while others work
                 if (__builtin_expect((++spins & 255) == 0, 0)) {
                     if (__hip_atomic_load(A.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ab = 1; break; }
                     if (spins > A.spin_limit) {
@@ -394,6 +394,7 @@ struct Cg {
 
     // ---- per-bead forces from the slots into Fout; x += lam*d when `move`.
     // Returns max|F_i|, F.F and F.Fprev (Fprev may be null), sums exact.
+    template <bool WIDE>
     __attribute__((always_inline)) __device__ bool gather(double *Fout, const double *Fprev, bool move, const double *dvec, double lam, double &maxF,
                            double &FF, double &FFp) {
         const int p1 = ph;
@@ -403,7 +404,7 @@ struct Cg {
         mdy_xs_zero(&sfp);
         mark(0);
         for (int b = gtid; b < A.D.nb; b += gs) {
-            d3 f = gather_bead(A.D, slots(), b);
+            d3 f = WIDE ? gather_bead_w<12, 24>(A.D, slots(), b) : gather_bead_w<8, 8>(A.D, slots(), b);
             const int o = 3 * b;
             Fout[o] = f.x; Fout[o + 1] = f.y; Fout[o + 2] = f.z;
             m[0] = fmax(m[0], fmax(fabs(f.x), fmax(fabs(f.y), fabs(f.z))));
@@ -445,7 +446,9 @@ struct Cg {
 template <bool MIXED>
 __device__ __attribute__((always_inline)) void spec_direction(Cg<MIXED> &K, const double *Fn, const double *dprev,
                                                               double beta, double *Sn, mdy_xsum &sd) {
-    for (int b = K.gtid; b < K.A.D.nb; b += K.gs) {
+    // thread t takes beads gs-1-t, 2gs-1-t, ...: the first threads carry the
+    // most excluded-volume items in the sweep that follows in the same phase
+    for (int b = K.gs - 1 - K.gtid; b < K.A.D.nb; b += K.gs) {
         const int o = 3 * b;
         for (int k = 0; k < 3; k++) {
             double s = Fn[o + k] + beta * dprev[o + k];
@@ -471,7 +474,8 @@ enum Stage {
 // One minimization run by a cluster of G blocks (blk = index within it).
 // QUAD: compiled with the quadratic line-search stages (a separate kernel, so
 // the backtracking kernel carries none of their state in registers).
-template <bool MIXED, bool QUAD>
+// WIDE: the wide per-bead gather (one bead per thread).
+template <bool MIXED, bool QUAD, bool WIDE>
 __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *lds, int blk, int G) {
     __shared__ unsigned long long accl[24];
     Cg<MIXED> K(A, lds, accl, blk, G);
@@ -573,7 +577,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         }
         case OP_GATHER:
             if (__builtin_expect(ev.nqf != 0, 0)) ok = K.force_fallback(P, ev);
-            if (ok && !K.status) ok = K.gather(gOut, gPrev, gMove, P.d, P.lam, maxF, FF, FFp);
+            if (ok && !K.status) ok = K.template gather<WIDE>(gOut, gPrev, gMove, P.d, P.lam, maxF, FF, FFp);
             break;
         case OP_MINSTEP: {
             // MController::moveAlongDirectionLimitingBeadMove_: step = min maxDist/|d_i|
@@ -872,10 +876,10 @@ aborted:
 constexpr int kCgLds = block_combine_words(17);
 static_assert(kCgLds >= 17 * 16, "LDS of the other block reductions");
 
-template <bool MIXED, bool QUAD>
+template <bool MIXED, bool QUAD, bool WIDE>
 __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
     __shared__ double lds[kCgLds];
-    cg_run<MIXED, QUAD>(Av, lds, blockIdx.x, gridDim.x);
+    cg_run<MIXED, QUAD, WIDE>(Av, lds, blockIdx.x, gridDim.x);
 }
 
 // Batched form: independent minimizations in one cooperative launch. Block b
@@ -896,13 +900,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ 
 #else
     CgArgs A = all[m.x];
 #endif
-    cg_run<MIXED, QUAD>(A, lds, m.y, A.grid);
+    cg_run<MIXED, QUAD, false>(A, lds, m.y, A.grid);
 }
 
-static const void *cg_kernel(bool mixed, bool quad) {
-    if (mixed) return quad ? reinterpret_cast<const void *>(k_cg<true, true>)
-                           : reinterpret_cast<const void *>(k_cg<true, false>);
-    return quad ? reinterpret_cast<const void *>(k_cg<false, true>) : reinterpret_cast<const void *>(k_cg<false, false>);
+template <bool WIDE>
+static const void *cg_kernel_w(bool mixed, bool quad) {
+    if (mixed) return quad ? reinterpret_cast<const void *>(k_cg<true, true, WIDE>)
+                           : reinterpret_cast<const void *>(k_cg<true, false, WIDE>);
+    return quad ? reinterpret_cast<const void *>(k_cg<false, true, WIDE>)
+                : reinterpret_cast<const void *>(k_cg<false, false, WIDE>);
+}
+static const void *cg_kernel(bool mixed, bool quad, bool wide) {
+    return wide ? cg_kernel_w<true>(mixed, quad) : cg_kernel_w<false>(mixed, quad);
 }
 static const void *cg_batch_kernel(bool mixed, bool quad) {
     if (mixed) return quad ? reinterpret_cast<const void *>(k_cg_batch<true, true>)
@@ -1046,11 +1055,18 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
     // persistent grid: all blocks resident
     int per_cu = 0;
     const bool mixed = c.opts.precision == MDY_PREC_MIXED;
-    const void *kern = cg_kernel(mixed, p->line_search == MDY_LS_QUADRATIC);
+    const bool quad = p->line_search == MDY_LS_QUADRATIC;
+    const void *kern = cg_kernel(mixed, quad, false);
     MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
     if (per_cu < 1) return MDY_ERR_LAUNCH;
     if (c.opts.blocks_per_cu > 0 && c.opts.blocks_per_cu < per_cu) per_cu = c.opts.blocks_per_cu;
     const int G = per_cu * c.num_cu;
+    if ((long long)c.nb <= (long long)G * kBlock) {   // one bead per thread in the gather: the wide form
+        int per_cu_w = 0;
+        const void *kw = cg_kernel(mixed, quad, true);
+        MDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_w, kw, kBlock, 0));
+        if (per_cu_w >= per_cu) kern = kw;
+    }
     c.coop_blocks = G;
     CgArgs A;
     MDY_TRY(prepare_cg(c, p, d_x, G, st, A));

@@ -355,10 +355,14 @@ __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double a
 // (ForceFieldManager::computeForces zero-fills, then each force field adds).
 // The first kGS static and kGD dynamic slot indices and values are loaded
 // together before any addition, so a bead costs three dependent memory round
-// trips (offsets, indices, values) instead of two per slot.
-constexpr int kGS = 8, kGD = 8;
-template <class S>
-__device__ __forceinline__ d3 gather_bead(const Dev &D, const S *__restrict__ contrib, int b) {
+// trips (offsets, indices, values) instead of two per slot; slots beyond
+// those follow one by one. Every phase waits for its slowest bead, so when a
+// thread gathers one bead (the minimizer on networks with fewer beads than
+// threads) the wide form (12 static, 24 dynamic) covers the beads with the
+// most slots; with several beads per thread the narrow form's fewer
+// instructions per bead win.
+template <int kGS, int kGD, class S>
+__device__ __forceinline__ d3 gather_bead_w(const Dev &D, const S *__restrict__ contrib, int b) {
     const int s0 = D.st_off[b], s1 = D.st_off[b + 1], d0 = D.dy_off[b], d1 = D.dy_off[b + 1];
     int is[kGS], id[kGD];
 #pragma unroll
@@ -390,6 +394,10 @@ __device__ __forceinline__ d3 gather_bead(const Dev &D, const S *__restrict__ co
         fx += contrib[3 * s]; fy += contrib[3 * s + 1]; fz += contrib[3 * s + 2];
     }
     return d3{fx, fy, fz};
+}
+template <class S>
+__device__ __forceinline__ d3 gather_bead(const Dev &D, const S *__restrict__ contrib, int b) {
+    return gather_bead_w<8, 8>(D, contrib, b);
 }
 
 // ---- block reductions (fixed butterfly + fixed wave order: deterministic) ----
