@@ -390,10 +390,11 @@ def bench_ensemble(a, lib):
     60 cylinders per um^3, 0.5 linkers and 0.05 motors per cylinder) minimized
     to 1 pN together by one batched launch in the mixed form. One step = one
     batched minimization of the whole ensemble from its initial coordinates
-    (tables and neighbor lists resident), in passes whose force-call caps
-    move the blocks of finished networks to the slow ones. Threads per block,
-    the first pass's cap and the cluster shares come from the library's
-    autotuner (medyan_amd.ensemble.autotune) before the warmup."""
+    (tables and neighbor lists resident), in passes that move the blocks of
+    finished networks to the slow ones (passes ended by force-call caps or
+    by completions). Threads per block, the pass rule and the cluster shares
+    come from the library's autotuner (medyan_amd.ensemble.autotune) before
+    the warmup."""
     import numpy as np
     import torch
     from medyan_amd import _abi, ensemble, mechanics
@@ -476,9 +477,10 @@ def bench_ensemble(a, lib):
                          "um^3, 0.5 linkers and 0.05 motors per cylinder; PR-CG + backtracking to 1 pN, "
                          "energy criterion off; tables and neighbor lists resident; " + generator_rule(lib)),
             "threads_per_block": tpb,
-            "first_pass_force_calls": first,
+            "first_pass_calls": first,
+            "pass_rule": ensemble.pass_rule(first),
             "cluster_share_exponent": share_exp,
-            "autotune_ms": {f"{k[0]} threads, first pass {k[1]}, shares work^{k[2]}": round(v, 1)
+            "autotune_ms": {f"{k[0]} threads, {ensemble.pass_rule(k[1])}, shares work^{k[2]}": round(v, 1)
                             for k, v in tune.items()},
             "per_network_ms_p50": float(np.percentile(per_net, 50)),
             "per_network_ms_p99": float(np.percentile(per_net, 99)),

@@ -341,8 +341,11 @@ int mdy_minimize_dev(mdy_ctx *ctx, const mdy_cg_params *params, double *d_coord,
  * minimization at the top of an iteration after first_pass_calls * 4^(k-1)
  * force calls, and the next pass resumes the suspended ones where they
  * stopped, on the blocks the finished networks released (uncapped once each
- * gets 64 blocks or more).
- * Results equal those of one uncapped pass (a
+ * gets 64 blocks or more). first_pass_calls = -k (k = 1..1024) ends the
+ * passes by completions instead: once max(1, m / k) of a launch's m networks
+ * have finished, the others suspend at the top of their next iteration and
+ * the next pass re-splits the blocks among them, down to one network on all
+ * blocks. Results equal those of one uncapped pass (a
  * trajectory does not depend on its cluster size); 0 = one pass.
  * params, d_coord (device, updated in place), results: n entries each;
  * d_force_out (nullable array; entries nullable) receives final forces;

@@ -172,7 +172,7 @@ void mdy_destroy(mdy_ctx *ctx) {
     release(c.pairs); release(c.ev_bs); release(c.ev_kr); release(c.bd_flag); release(c.bd_off); release(c.bd_bead);
     release(c.bd_plane); release(c.bd_k); release(c.bd_sl); release(c.dy_cnt); release(c.dy_off); release(c.dy_cur);
     release(c.dy_idx); release(c.scan_tmp); release(c.bbox); release(c.x); release(c.xt); release(c.f);
-    release(c.batch_args); release(c.batch_map); release(c.evq); release(c.qpan); release(c.qcnt); release(c.qoff);
+    release(c.batch_args); release(c.batch_map); release(c.batch_done); release(c.evq); release(c.qpan); release(c.qcnt); release(c.qoff);
     release(c.qmap); release(c.qpart); release(c.qpe); release(c.qtot);
     release(c.contrib); release(c.red); release(c.out); release(c.qe); release(c.qf); release(c.qn); release(c.err);
     release(c.qe_u); release(c.simpson_part); release(c.F0); release(c.F1); release(c.S0); release(c.S1);
@@ -619,7 +619,7 @@ int mdy_minimize_batch(mdy_ctx *const *ctxs, int n, const mdy_cg_params *params,
     Ctx &c0 = ctxs[0]->c;
     (void)hipSetDevice(c0.device);
     hipStream_t st = stream ? (hipStream_t)stream : c0.stream;
-    if (first_pass_calls < 0) return fail(ctxs[0], MDY_ERR_ARG, "first_pass_calls must be >= 0");
+    if (first_pass_calls < -1024) return fail(ctxs[0], MDY_ERR_ARG, "first_pass_calls must be >= -1024");
     int rc = run_minimize_batch(cs.data(), n, params, d_coord, d_force_out, results, net_ms, threads_per_block,
                                 weights, first_pass_calls, st, total_ms);
     if (rc == MDY_ERR_LAUNCH) return fail(ctxs[0], rc, "cooperative launch failed");

@@ -71,7 +71,7 @@ __device__ __forceinline__ int ld_i(const int *p) { return __hip_atomic_load(p, 
 
 // phase accumulator sets: 3 rotating sets of kAccWords 64-bit words
 constexpr int kAccWords = 32;
-constexpr int kAcc0 = 0, kAcc1 = 8, kAccMax = 16;
+constexpr int kAcc0 = 0, kAcc1 = 8, kAccMax = 16, kAccDone = 19;
 
 // phase kinds recorded by the profile
 enum { PK_ENERGY = 1, PK_EFALLBACK, PK_EVAL, PK_FFALLBACK, PK_GATHER, PK_REDUCE, PK_RESTORE };
@@ -93,6 +93,10 @@ struct Cg {
     unsigned long long *accl;    // LDS copy of the words a phase's decisions read
     int G, blk, tid, gtid, gs;
     int ph = 0;                  // phase counter (uniform)
+    // batched passes by completions: the launch's count of finished
+    // minimizations (nullable), and its value as of the last gather (uniform)
+    int *done_ctr = nullptr;
+    int done_seen = 0;
     bool aborted = false;
     int status = MDY_OK;
 
@@ -394,7 +398,7 @@ struct Cg {
 
     // ---- per-bead forces from the slots into Fout; x += lam*d when `move`.
     // Returns max|F_i|, F.F and F.Fprev (Fprev may be null), sums exact.
-    template <bool WIDE>
+    template <bool WIDE, bool BATCH>
     __attribute__((always_inline)) __device__ bool gather(double *Fout, const double *Fprev, bool move, const double *dvec, double lam, double &maxF,
                            double &FF, double &FFp) {
         const int p1 = ph;
@@ -424,11 +428,17 @@ struct Cg {
         block_reduce_xsum2_lds(sff, sfp, ldsl(), m);   // F.F and F.Fprev together, and max|F|
         unsigned long long *a = accs(p1);
         if (tid == 0) atomicMax(a + kAccMax, (unsigned long long)__double_as_longlong(m[0]));
+        // the cluster's blocks all read the launch's completion count as one
+        // block saw it, through this phase's accumulators
+        if (BATCH && done_ctr && blk == 0 && tid == 0)
+            atomicMax(a + kAccDone, (unsigned long long)(unsigned)ld_i(done_ctr));
         acc_push(a + kAcc0, sff);
         acc_push(a + kAcc1, sfp);
         if (!sync(PK_GATHER)) return false;
         acc_fetch(a, nullptr, nullptr);
         maxF = __longlong_as_double((long long)accl[kAccMax]);
+        if (BATCH && done_ctr)
+            done_seen = (int)__hip_atomic_load(a + kAccDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double v2[2];
 #pragma clang loop unroll(disable)
         for (int k = 0; k < 2; k++) {
@@ -474,11 +484,14 @@ enum Stage {
 // One minimization run by a cluster of G blocks (blk = index within it).
 // QUAD: compiled with the quadratic line-search stages (a separate kernel, so
 // the backtracking kernel carries none of their state in registers).
-// WIDE: the wide per-bead gather (one bead per thread).
-template <bool MIXED, bool QUAD, bool WIDE>
-__device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *lds, int blk, int G) {
+// WIDE: the wide per-bead gather (one bead per thread). BATCH: the batched
+// launch's passes ended by completions.
+template <bool MIXED, bool QUAD, bool WIDE, bool BATCH>
+__device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *lds, int blk, int G,
+                                                      int *done = nullptr, int suspend_done = 0) {
     __shared__ unsigned long long accl[24];
     Cg<MIXED> K(A, lds, accl, blk, G);
+    if (BATCH) K.done_ctr = done;
     double *R = A.out;
     const bool leader = (K.blk == 0 && K.tid == 0);
     if (leader) R[R_T0] = (double)wall_clock64();
@@ -489,6 +502,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
     long long trn = 0;
     double gradInf = INFINITY, relE = INFINITY;
     bool after = true;   // compute energiesAfter / output force
+    bool suspended = false;
     int cur = 0, sidx = 0;
     const double *dvec = nullptr;
     double maxF = 0, FF = 0, FFp = 0, FFcur = 0;
@@ -577,7 +591,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         }
         case OP_GATHER:
             if (__builtin_expect(ev.nqf != 0, 0)) ok = K.force_fallback(P, ev);
-            if (ok && !K.status) ok = K.template gather<WIDE>(gOut, gPrev, gMove, P.d, P.lam, maxF, FF, FFp);
+            if (ok && !K.status) ok = K.template gather<WIDE, BATCH>(gOut, gPrev, gMove, P.d, P.lam, maxF, FF, FFp);
             break;
         case OP_MINSTEP: {
             // MController::moveAlongDirectionLimitingBeadMove_: step = min maxDist/|d_i|
@@ -629,7 +643,8 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
                 stage = ST_FINAL;
                 break;
             }
-            if (A.suspend_calls > 0 && nFC >= A.suspend_calls) {   // this pass ends; the next resumes here
+            if ((A.suspend_calls > 0 && nFC >= A.suspend_calls) ||
+                (BATCH && suspend_done > 0 && K.done_seen >= suspend_done)) {   // this pass ends; the next resumes here
                 if (leader) {
                     double *S = R + R_RS;
                     S[0] = flags; S[1] = nEC; S[2] = nFC; S[3] = nIter; S[4] = nEE; S[5] = nSM; S[6] = nRec;
@@ -639,6 +654,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
                     S[23] = pendingSpec ? 1.0 : 0.0; S[24] = bufid(dprev); S[25] = bufid(Snew);
                     R[R_SUSP] = 1.0;
                 }
+                suspended = true;
                 after = false;
                 stage = ST_FINISH;
                 break;
@@ -862,6 +878,7 @@ __device__ __attribute__((always_inline)) void cg_run(const CgArgs &A, double *l
         // that shift is pending until the next trial
         R[R_DIR] = bufid(dvec); R[R_PEND] = pendingSpec ? 1.0 : 0.0; R[R_BETA] = beta; R[R_DPREV] = bufid(dprev);
         R[R_T1] = (double)wall_clock64();
+        if (BATCH && done && !suspended) atomicAdd(done, 1);
     }
     return;
 aborted:
@@ -879,7 +896,7 @@ static_assert(kCgLds >= 17 * 16, "LDS of the other block reductions");
 template <bool MIXED, bool QUAD, bool WIDE>
 __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
     __shared__ double lds[kCgLds];
-    cg_run<MIXED, QUAD, WIDE>(Av, lds, blockIdx.x, gridDim.x);
+    cg_run<MIXED, QUAD, WIDE, false>(Av, lds, blockIdx.x, gridDim.x);
 }
 
 // Batched form: independent minimizations in one cooperative launch. Block b
@@ -890,8 +907,11 @@ __global__ void __launch_bounds__(kBlock) k_cg(const CgArgs Av) {
 // global, so the arguments are read through a constant-space view and every
 // access through them uses global instructions, as in the single-network
 // kernel.
+// done (nullable): the launch's count of finished minimizations; each
+// suspends at the top of an iteration once the count reaches suspend_done (>0).
 template <bool MIXED, bool QUAD>
-__global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ all, const int2 *__restrict__ map) {
+__global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ all, const int2 *__restrict__ map,
+                                                     int *done, int suspend_done) {
     __shared__ double lds[kCgLds];
     const int2 m = map[blockIdx.x];
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -900,7 +920,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_batch(const CgArgs *__restrict__ 
 #else
     CgArgs A = all[m.x];
 #endif
-    cg_run<MIXED, QUAD, false>(A, lds, m.y, A.grid);
+    cg_run<MIXED, QUAD, false, true>(A, lds, m.y, A.grid, done, suspend_done);
 }
 
 template <bool WIDE>
@@ -1101,10 +1121,16 @@ int run_minimize(Ctx &c, const mdy_cg_params *p, double *d_x, double *d_force_ou
 // every network gets at least kUncappedShare blocks runs uncapped. A
 // minimization's trajectory does not depend on its cluster size, so the
 // results equal those of one uncapped pass; the passes only move the blocks
-// of finished networks to the slow ones. cs[i] owns network i's tables, neighbor lists and CG buffers; cs[0]
-// the launch arrays.
+// of finished networks to the slow ones. With first_pass_calls = -k < 0 the
+// passes end by completions instead: a launch of m > 1 networks counts its
+// finished minimizations on the device, and once max(1, m / k) have finished
+// the others suspend at the top of their next iteration (each reads the
+// count once per iteration); the next pass re-splits all the blocks among
+// them, down to one network on the whole grid. cs[i] owns network i's
+// tables, neighbor lists and CG buffers; cs[0] the launch arrays.
 constexpr int kUncappedShare = 64;   // blocks per network at which a pass runs uncapped
 constexpr int kCapGrowth = 4;        // force-call cap growth from one pass to the next (x2 measured slower)
+constexpr int kMaxClusterThreads = 8192;    // threads of one network's cluster at most
 int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d_x, double *const *d_force_out,
                        mdy_min_result *res, double *net_ms, int threads_per_block, const double *weights,
                        int first_pass_calls, hipStream_t st, double *total_ms) {
@@ -1130,7 +1156,9 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
     std::vector<int> gsz, todo(n), next;
     for (int i = 0; i < n; i++) todo[i] = i;
     long long cap = first_pass_calls > 0 ? first_pass_calls : 0;
+    const int by_done = first_pass_calls < 0 ? -first_pass_calls : 0;
     int rc = MDY_OK;
+    if (by_done) rc = ensure(c0.batch_done, 1);
     while (!todo.empty() && rc == MDY_OK) {
         next.clear();
         const int nt = (int)todo.size();
@@ -1165,6 +1193,10 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
                           return a.first > b.first || (a.first == b.first && a.second < b.second);
                       });
             for (int k = 0; k < spare - used && k < m; k++) gsz[frac[k].second]++;
+            // a cluster's barrier slows down with its block count: beyond
+            // kMaxClusterThreads threads a network gets no faster
+            const int gcap = std::max(1, kMaxClusterThreads / tpb);
+            for (int i = 0; i < m; i++) gsz[i] = std::min(gsz[i], gcap);
             args.assign(m, CgArgs{});
             map.clear();
             for (int i = 0; i < m && rc == MDY_OK; i++) {
@@ -1186,9 +1218,15 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
                 rc = MDY_ERR_HIP;
                 break;
             }
+            if (by_done && hipMemsetAsync(c0.batch_done.p, 0, sizeof(int), st) != hipSuccess) {
+                rc = MDY_ERR_HIP;
+                break;
+            }
             const CgArgs *pa = reinterpret_cast<const CgArgs *>(c0.batch_args.p);
             const int2 *pm = c0.batch_map.p;
-            void *kargs[] = {&pa, &pm};
+            int *done = by_done && m > 1 ? c0.batch_done.p : nullptr;
+            int suspend_done = done ? std::max(1, m / by_done) : 0;
+            void *kargs[] = {&pa, &pm, &done, &suspend_done};
             (void)hipEventRecord(e0, st);
             if (hipLaunchCooperativeKernel(kern, dim3(G), dim3(tpb), kargs, 0, st) != hipSuccess) {
                 rc = MDY_ERR_LAUNCH;

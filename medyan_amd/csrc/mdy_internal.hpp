@@ -121,6 +121,7 @@ struct Ctx {
     DBuf<long long> prof; DBuf<int> prof_kind; long long prof_cap = 0, prof_n = 0;
     DBuf<unsigned char> batch_args;  // batched launch: per-network kernel arguments
     DBuf<int2> batch_map;            // batched launch: (network, block within its cluster)
+    DBuf<int> batch_done;            // batched launch: finished minimizations (passes by completions)
     double last_ms = 0;
     // the last minimization's search direction (CGMethod::searchDir): the CG
     // buffer that holds it (0 F0, 1 F1, 2 S0, 3 S1) or, when its final update

@@ -48,15 +48,28 @@ def work(ctx):
     return bonded + len(ctx.exvol_table()[1]) + len(ctx.boundary_table()[1]) + 2 * n.n_beads
 
 
-def autotune(ctxs, params, d_coords, restore, threads=(64, 128, 256), first_pass=(250, 500, 1000, 2000),
+def pass_rule(first_pass_calls):
+    """The pass rule a first_pass_calls value of mdy_minimize_batch names."""
+    f = int(first_pass_calls)
+    if f > 0:
+        return f"passes capped at {f} x 4^k force calls"
+    if f < 0:
+        return f"passes end when 1/{-f} of their networks have finished" if f < -1 else "one pass"
+    return "one pass"
+
+
+def autotune(ctxs, params, d_coords, restore, threads=(64, 128, 256), first_pass=(-8, -4, -2, 250, 500, 1000, 2000),
              exponents=(1.0, 1.25), log=None):
     """Block and tile shapes of the batched launch (mdy_minimize_batch) for
     one ensemble, by coordinate search over measured runs:
       1. threads per block, the unit every network's cluster is built from
-         (64: more, smaller blocks per network; 256: fewer, wider ones);
-      2. the force-call cap of the first pass (how long each network keeps
-         its first share of blocks before the blocks of finished networks
-         are moved to the ones still running);
+         (64: more, smaller blocks per network; 256: fewer, wider ones), with
+         the first pass rule;
+      2. the pass rule: a force-call cap for the first pass (how long each
+         network keeps its first share of blocks before the blocks of
+         finished networks are moved to the ones still running), or, for a
+         negative value -k, passes that end once 1/k of their networks have
+         finished;
       3. the exponent a of the cluster shares w_i = work_i ** a (a > 1 gives
          the large networks more than their share of the blocks).
     Each stage keeps the best value of the previous ones. Each candidate
@@ -75,10 +88,10 @@ def autotune(ctxs, params, d_coords, restore, threads=(64, 128, 256), first_pass
                                                  weights=wk ** a, first_pass_calls=first)
             times[key] = tot
             if log:
-                log(f"autotune: {tpb} threads, first pass {first} force calls, shares work^{a}: {tot:.1f} ms")
+                log(f"autotune: {tpb} threads, {pass_rule(first)}, shares work^{a}: {tot:.1f} ms")
         return times[key]
 
-    first0 = first_pass[min(1, len(first_pass) - 1)]
+    first0 = first_pass[0]
     tpb = min(threads, key=lambda t: run(t, first0, 1.0))
     first = min(first_pass, key=lambda f: run(tpb, f, 1.0))
     a = min(exponents, key=lambda e: run(tpb, first, e))

@@ -414,7 +414,10 @@ def minimize_batch(ctxs, params, d_coords, d_forces=None, threads_per_block=64, 
     params: one CGParams / ConjugateGradientParams for all, or one per
     network. weights (optional): block shares. first_pass_calls > 0: passes
     with force-call caps growing 4x; a capped network is suspended and resumed
-    on more blocks. Returns (results, per-network completion ms, total ms)."""
+    on more blocks. first_pass_calls = -k: passes end by completions (once
+    max(1, m / k) of a pass's m networks have finished, the others are
+    suspended and resumed on all the blocks). Returns (results, per-network
+    completion ms, total ms)."""
     n = len(ctxs)
     lib = load_library()
     if not isinstance(params, (list, tuple)):

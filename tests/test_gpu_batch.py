@@ -44,10 +44,14 @@ def same(a, b):
 @pytest.mark.parametrize("precision,tpb,passes,ls", [(_abi.MDY_PREC_FP64, 64, 0, 0), (_abi.MDY_PREC_FP64, 256, 0, 0),
                                                      (_abi.MDY_PREC_MIXED, 128, 0, 0), (_abi.MDY_PREC_FP64, 64, 40, 0),
                                                      (_abi.MDY_PREC_MIXED, 64, 40, 0),
+                                                     (_abi.MDY_PREC_FP64, 64, -2, 0),
+                                                     (_abi.MDY_PREC_MIXED, 128, -4, 0),
                                                      (_abi.MDY_PREC_FP64, 128, 0, _abi.MDY_LS_QUADRATIC)])
 def test_batch_equals_single_minimizations(lib, precision, tpb, passes, ls):
     """passes > 0: a first pass capped at 40 force calls (six networks get
-    64 blocks or more each, so the pass runs uncapped). ls: line search."""
+    64 blocks or more each, so the pass runs uncapped). passes = -k: each
+    pass ends once max(1, m / k) of its m networks have finished (-2: after
+    3, then 1, ...; -4: after every completion, six passes). ls: line search."""
     nets = nets_small(lib)
     print(f"[gpu] batch of {len(nets)}, precision {precision}, {tpb} threads per block, first pass {passes}, "
           f"line search {ls}", flush=True)
@@ -80,10 +84,12 @@ def test_batch_equals_single_minimizations(lib, precision, tpb, passes, ls):
     print(f"[gpu] batch {tot:.1f} ms; per network {np.round(ms, 2)}")
 
 
-@pytest.mark.parametrize("passes", [0, 3])
+@pytest.mark.parametrize("passes", [0, 3, -2])
 def test_batch_larger_than_resident_blocks(lib, passes):
     """More networks than resident blocks: the call runs several launches;
-    with passes, capped networks are suspended and resumed (caps 3, 12, 48, ... force calls)."""
+    with passes, capped networks are suspended and resumed (3: caps 3, 12,
+    48, ... force calls; -2: each launch's pass ends when half of its
+    networks have finished)."""
     base = [network.generate(lib, None, seed=100 + i, n_filaments=3, cyl_per_filament=3, box=700.0, n_linkers=2,
                              n_motors=0, n_branches=0) for i in range(4)]
     n = 256 + 8   # above one 256-thread block per CU on 256 CUs
@@ -126,7 +132,7 @@ def test_batch_argument_errors(lib):
 
 def test_autotuner_choice_leaves_results_unchanged(lib):
     """medyan_amd.ensemble.autotune times each candidate (threads per block,
-    first-pass cap, cluster-share exponent) on the ensemble and returns the
+    pass rule, cluster-share exponent) on the ensemble and returns the
     fastest; whatever it picks, the minimizations equal the default launch's
     bit for bit."""
     nets = nets_small(lib)
@@ -141,10 +147,10 @@ def test_autotuner_choice_leaves_results_unchanged(lib):
             x.copy_(x_0)
         torch.cuda.synchronize()
 
-    best, times = ensemble.autotune(ctxs, params, ptrs, restore, threads=(64, 128), first_pass=(40, 400),
+    best, times = ensemble.autotune(ctxs, params, ptrs, restore, threads=(64, 128), first_pass=(-2, 40),
                                     exponents=(1.0, 1.5), log=lambda m: print(f"[gpu] {m}", flush=True))
     assert len(times) == 2 + 1 + 1 and best["threads_per_block"] in (64, 128)
-    assert best["first_pass_calls"] in (40, 400) and len(best["weights"]) == len(nets)
+    assert best["first_pass_calls"] in (-2, 40) and len(best["weights"]) == len(nets)
     restore()
     res_b, _, _ = mechanics.minimize_batch(ctxs, params, ptrs, **best)
     got = [x.cpu().numpy() for x in xs]
