@@ -351,12 +351,44 @@ __device__ __forceinline__ void item(const Dev &D, int i, const P &pos, double a
     item_compute<MIXED, QUAD>(D, r, c, acc, xs, contrib, PER_FF, FORCES);
 }
 
+// Slots k0..k1-1 of a bead's row (after the ones loaded with the row's
+// first), added in order. CHUNKED: kTail indices, then their kTail values, at
+// a time, so a bead with many slots (up to ~60 in a 200k-cylinder network)
+// costs two dependent round trips per kTail slots rather than per slot;
+// otherwise one slot at a time (fewer instructions where rows rarely go past
+// their first loads).
+constexpr int kTail = 8;
+template <bool CHUNKED, class S>
+__device__ __forceinline__ void gather_tail(const int *__restrict__ idx, int k0, int k1, const S *__restrict__ contrib,
+                                            double &fx, double &fy, double &fz) {
+    if (!CHUNKED) {
+        for (int k = k0; k < k1; k++) {
+            const int s = idx[k];
+            fx += contrib[3 * s]; fy += contrib[3 * s + 1]; fz += contrib[3 * s + 2];
+        }
+        return;
+    }
+#pragma clang loop unroll(disable)
+    for (int k = k0; k < k1; k += kTail) {
+        int ix[kTail];
+#pragma unroll
+        for (int j = 0; j < kTail; j++) ix[j] = (k + j < k1) ? idx[k + j] : 0;
+        S v[kTail][3];
+#pragma unroll
+        for (int j = 0; j < kTail; j++)
+            if (k + j < k1) { v[j][0] = contrib[3 * ix[j]]; v[j][1] = contrib[3 * ix[j] + 1]; v[j][2] = contrib[3 * ix[j] + 2]; }
+#pragma unroll
+        for (int j = 0; j < kTail; j++)
+            if (k + j < k1) { fx += v[j][0]; fy += v[j][1]; fz += v[j][2]; }
+    }
+}
 // Per-bead force: the slots of bead b in reference order, starting from 0.0
 // (ForceFieldManager::computeForces zero-fills, then each force field adds).
 // The first kGS static and kGD dynamic slot indices and values are loaded
 // together before any addition, so a bead costs three dependent memory round
 // trips (offsets, indices, values) instead of two per slot; slots beyond
-// those follow one by one. Every phase waits for its slowest bead, so when a
+// those follow in chunks in the narrow form (gather_tail). Every phase waits
+// for its slowest bead, so when a
 // thread gathers one bead (the minimizer on networks with fewer beads than
 // threads) the wide form (12 static, 24 dynamic) covers the beads with the
 // most slots; with several beads per thread the narrow form's fewer
@@ -382,17 +414,11 @@ __device__ __forceinline__ d3 gather_bead_w(const Dev &D, const S *__restrict__ 
 #pragma unroll
     for (int j = 0; j < kGS; j++)
         if (s0 + j < s1) { fx += vs[j][0]; fy += vs[j][1]; fz += vs[j][2]; }
-    for (int k = s0 + kGS; k < s1; k++) {
-        int s = D.st_idx[k];
-        fx += contrib[3 * s]; fy += contrib[3 * s + 1]; fz += contrib[3 * s + 2];
-    }
+    gather_tail<(kGS <= 8)>(D.st_idx, s0 + kGS, s1, contrib, fx, fy, fz);
 #pragma unroll
     for (int j = 0; j < kGD; j++)
         if (d0 + j < d1) { fx += vd[j][0]; fy += vd[j][1]; fz += vd[j][2]; }
-    for (int k = d0 + kGD; k < d1; k++) {
-        int s = D.dy_idx[k];
-        fx += contrib[3 * s]; fy += contrib[3 * s + 1]; fz += contrib[3 * s + 2];
-    }
+    gather_tail<(kGS <= 8)>(D.dy_idx, d0 + kGD, d1, contrib, fx, fy, fz);
     return d3{fx, fy, fz};
 }
 template <class S>
