@@ -116,25 +116,52 @@ __global__ void __launch_bounds__(kBlock) k_ev_classify(Dev D, const double *__r
                                                          unsigned char *__restrict__ flag, int *__restrict__ pan,
                                                          int *__restrict__ cnt, int *__restrict__ off,
                                                          int *__restrict__ cmap, int *__restrict__ tot, int cap) {
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < D.n_ev; j += gridDim.x * blockDim.x) {
-        const int4 b = D.ev_bs[j];
-        const d3 c1 = ld3(x, b.x), c2 = ld3(x, b.y), c3 = ld3(x, b.z), c4 = ld3(x, b.w);
-        const bool f = needs_quadrature(c1, c2, c3, c4);
-        flag[j] = f ? 1 : 0;
-        int nc = 0;
-        if (f) {
-            const int m = quad_panels(c1, c2, c3, c4);
-            pan[j] = m;
-            nc = (256 * m * m + kQuadChunk - 1) / kQuadChunk;   // (16 m)^2 nodes
-            const int base = atomicAdd(tot, nc);
-            off[j] = base;
-            if (base + nc <= cap) {
-                for (int q = 0; q < nc; q++) cmap[base + q] = j;
-            } else {
-                set_err(D.err, MDY_ERR_CAPACITY);   // cannot happen with a cleared total (16 chunks per pair at most)
+    // one atomic per block and sweep turn: the block's chunk counts are
+    // scanned in LDS (wave prefix sums, then the wave totals)
+    __shared__ int wsum[kBlock / 64];
+    __shared__ int bbase;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j0 = blockIdx.x * blockDim.x; j0 < D.n_ev; j0 += gridDim.x * blockDim.x) {
+        const int j = j0 + threadIdx.x;
+        int nc = 0, m = 0;
+        if (j < D.n_ev) {
+            const int4 b = D.ev_bs[j];
+            const d3 c1 = ld3(x, b.x), c2 = ld3(x, b.y), c3 = ld3(x, b.z), c4 = ld3(x, b.w);
+            const bool f = needs_quadrature(c1, c2, c3, c4);
+            flag[j] = f ? 1 : 0;
+            if (f) {
+                m = quad_panels(c1, c2, c3, c4);
+                nc = (256 * m * m + kQuadChunk - 1) / kQuadChunk;   // (16 m)^2 nodes
             }
         }
-        cnt[j] = nc;
+        int incl = nc;   // inclusive prefix within the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = 0;
+            for (int w = 0; w < kBlock / 64; w++) { const int v = wsum[w]; wsum[w] = s; s += v; }
+            bbase = s ? atomicAdd(tot, s) : 0;
+        }
+        __syncthreads();
+        if (j < D.n_ev) {
+            if (nc) {
+                const int base = bbase + wsum[wave] + incl - nc;
+                pan[j] = m;
+                off[j] = base;
+                if (base + nc <= cap) {
+                    for (int q = 0; q < nc; q++) cmap[base + q] = j;
+                } else {
+                    set_err(D.err, MDY_ERR_CAPACITY);   // cannot happen with a cleared total (16 chunks per pair at most)
+                }
+            }
+            cnt[j] = nc;
+        }
+        __syncthreads();   // wsum and bbase are rewritten by the next turn
     }
 }
 
