@@ -692,7 +692,8 @@ def bench_gradient(a, lib):
     cylinders, 5 um box, 20k linkers, 2k motors), FP64, in the quadrature
     mode (ill-conditioned excluded-volume pairs by quadrature, so second
     derivatives are not rounding noise). Minimization to 1 pN; the timed step
-    is one Hessian-vector product (the exact mode's time beside it); the
+    is one Hessian-vector product enqueued back to back (mdy_hvp_dev_async;
+    the exact mode's time and both modes waiting for each product beside it); the
     curvature that conjugate gradients on H meet at the 1 pN structure, with
     the energy's second difference along it (negative: not a minimum); and the
     implicit derivative of the extent (sum of squared filament end-to-end
@@ -719,34 +720,40 @@ def bench_gradient(a, lib):
     x1 = x.clone()
     x1h = x1.cpu().numpy()
     n_quad = ctx.quadrature_pair_count(x1h)   # 0 in this mode; counted again below
-    # H v throughput on a side stream (the device calls take its handle: no host synchronization per call)
+    # H v throughput on a side stream: mdy_hvp_dev_async enqueues without a host
+    # synchronization per call (failures checked once after the timed calls);
+    # mdy_hvp_dev, which waits for each product, is timed beside it
     v = torch.randn(x1.shape, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
     hv = torch.empty_like(v)
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
 
-    def time_hvp():
+    def time_hvp(call):
         with torch.cuda.stream(side):
             for _ in range(a.warmup):
-                ctx.hvp_dev(x1.data_ptr(), v.data_ptr(), hv.data_ptr(), side.cuda_stream)
+                call(x1.data_ptr(), v.data_ptr(), hv.data_ptr(), side.cuda_stream)
             t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             t0.record(side)
             for _ in range(a.steps):
-                ctx.hvp_dev(x1.data_ptr(), v.data_ptr(), hv.data_ptr(), side.cuda_stream)
+                call(x1.data_ptr(), v.data_ptr(), hv.data_ptr(), side.cuda_stream)
             t1.record(side)
+        ctx.sync_errors(side.cuda_stream)
         side.synchronize()
         return t0.elapsed_time(t1) / a.steps
 
-    exact_ms = time_hvp()
+    exact_ms = time_hvp(ctx.hvp_dev_async)
+    exact_sync_ms = time_hvp(ctx.hvp_dev)
     ctx.set_opts(coplanar_mode=_abi.MDY_COPLANAR_QUADRATURE)
     n_quad = ctx.quadrature_pair_count(x1h)
-    hvp_ms = time_hvp()
+    hvp_ms = time_hvp(ctx.hvp_dev_async)
+    hvp_sync_ms = time_hvp(ctx.hvp_dev)
     n_ev = len(ctx.exvol_table()[1])
     n_bd = len(ctx.boundary_table()[1])
     tables = (B_FS * len(net.fs_k) + B_FB * len(net.fb_k) + B_LM * (len(net.ls_k) + len(net.ms_k))
               + B_BS * len(net.bs_k) + B_BB * len(net.bb_k) + B_EV * n_ev + B_BD * n_bd)
     hvp_bytes = tables + 48 * net.n_beads + 24 * net.n_beads
-    log(f"H v: {hvp_ms:.3f} ms in the quadrature mode ({n_quad} of {n_ev} pairs), {exact_ms:.3f} ms in the exact mode")
+    log(f"H v: {hvp_ms:.3f} ms in the quadrature mode ({n_quad} of {n_ev} pairs), {exact_ms:.3f} ms in the exact mode; "
+        f"waiting for each product: {hvp_sync_ms:.3f} / {exact_sync_ms:.3f} ms")
 
     fil = net.cyl_filament
     first = np.searchsorted(fil, np.arange(fil.max() + 1))
@@ -828,6 +835,8 @@ def bench_gradient(a, lib):
             "minimization_ms": min_ms,
             "minimization_iterations": int(r.num_iter),
             "hvp_ms_exact_mode": exact_ms,
+            "hvp_ms_waiting_each_call": hvp_sync_ms,
+            "hvp_ms_exact_mode_waiting_each_call": exact_sync_ms,
             "extent_nm2": float(extent(x1)),
             "curvature_at_1pN": curvature,
             "tethered_implicit_derivative": tethered,

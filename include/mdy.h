@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MDY_ABI_VERSION 4
+#define MDY_ABI_VERSION 5
 
 enum {
     MDY_OK = 0,
@@ -287,6 +287,16 @@ int mdy_energy_dev_noalloc(mdy_ctx *ctx, const double *d_coord, double *d_out, v
 /* Hessian-vector product H v = d/dt grad E(x + t v) at t = 0 of the
  * exact-gradient force field (no coplanar perturbation); writes H v. */
 int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *d_hv, void *stream);
+/* Same product, enqueued on the stream without waiting for it (no host
+ * synchronization per call, so successive products run back to back). A
+ * device-side failure (a degenerate excluded-volume pair: MDY_ERR_FORCE)
+ * stays in the context until mdy_sync_errors, or the next call that waits,
+ * reports and clears it. No reference counterpart (the reference has no
+ * H v). */
+int mdy_hvp_dev_async(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *d_hv, void *stream);
+/* Waits for the stream (NULL = the context's) and returns the failure an
+ * asynchronous call left, clearing it; MDY_OK if none. */
+int mdy_sync_errors(mdy_ctx *ctx, void *stream);
 /* Directional derivatives of the individual energies of the exact-gradient
  * force field: d_out[k] = grad E_k(x) . w for the MDY_NFF force fields, then
  * their sum. No reference counterpart (derivative of

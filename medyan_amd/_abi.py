@@ -6,7 +6,7 @@ medyan_amd.mechanics.
 """
 import ctypes as C
 
-MDY_ABI_VERSION = 4
+MDY_ABI_VERSION = 5
 
 MDY_OK = 0
 ERRORS = {

@@ -550,6 +550,28 @@ int mdy_hvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *
     return MDY_OK;
 }
 
+int mdy_hvp_dev_async(mdy_ctx *ctx, const double *d_coord, const double *d_v, double *d_hv, void *stream) {
+    if (!ctx || !d_coord || !d_v || !d_hv) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.have_sys || !c.have_nl) return fail(ctx, MDY_ERR_STATE, "network / neighbor list missing");
+    (void)hipSetDevice(c.device);
+    hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+    int rc = launch_hvp(c, d_coord, d_v, d_hv, st, false);
+    if (rc != MDY_OK) return fail(ctx, rc, "hvp launch failed");
+    return MDY_OK;
+}
+
+int mdy_sync_errors(mdy_ctx *ctx, void *stream) {
+    if (!ctx) return MDY_ERR_ARG;
+    Ctx &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+    if (!c.err.p) return MDY_OK;
+    int rc = sync_errors(c, st);
+    if (rc != MDY_OK) return fail(ctx, rc, "a device call enqueued earlier failed");
+    return MDY_OK;
+}
+
 int mdy_energy_jvp_dev(mdy_ctx *ctx, const double *d_coord, const double *d_w, double *d_out, void *stream) {
     if (!ctx || !d_coord || !d_w || !d_out) return MDY_ERR_ARG;
     Ctx &c = ctx->c;

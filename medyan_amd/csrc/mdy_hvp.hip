@@ -171,7 +171,7 @@ static int grid_of(const Ctx &c, long long n) {
 }
 
 // reads and clears the error word after the stream's work
-static int finish(Ctx &c, hipStream_t st) {
+int sync_errors(Ctx &c, hipStream_t st) {
     if (hipGetLastError() != hipSuccess) return MDY_ERR_HIP;
     int e = 0;
     if (hipMemcpyAsync(&e, c.err.p, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return MDY_ERR_HIP;
@@ -184,14 +184,15 @@ static int finish(Ctx &c, hipStream_t st) {
     return MDY_OK;
 }
 
-int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st) {
+int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st, bool sync) {
     Dev D = c.dev();
     const bool quad = quad_on(c);
     if (quad) MDY_TRY(quad_classify(c, D, d_x, st));
     hipLaunchKernelGGL(k_hvp, dim3(grid_of(c, D.n_items())), dim3(kBlock), 0, st, D, d_x, d_v, c.contrib.p);
     if (quad) MDY_TRY(launch_quad(c, D, 2, d_x, d_v, c.contrib.p, 0, st));
     hipLaunchKernelGGL(k_hvp_gather, dim3(grid_of(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_hv);
-    return finish(c, st);
+    if (!sync) return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
+    return sync_errors(c, st);
 }
 
 int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_out, hipStream_t st) {
@@ -204,7 +205,7 @@ int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_ou
     hipLaunchKernelGGL(k_energy_jvp, dim3(nblk), dim3(kBlock), 0, st, D, d_x, d_w, c.red.p);
     if (quad) MDY_TRY(launch_quad(c, D, 3, d_x, d_w, c.red.p, nblk, st));
     hipLaunchKernelGGL(k_energy_jvp_final, dim3(1), dim3(kBlock), 0, st, c.red.p, nblk + nqb, d_out);
-    return finish(c, st);
+    return sync_errors(c, st);
 }
 
 }  // namespace mdy

@@ -22,7 +22,10 @@ int run_minimize_batch(Ctx **cs, int n, const mdy_cg_params *p, double *const *d
                        mdy_min_result *res, double *net_ms, int threads_per_block, const double *weights,
                        int first_pass_calls, hipStream_t st, double *total_ms);
 // mdy_hvp.hip
-int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st);
+// sync = false: enqueue only; a device error stays in the context's error
+// word for sync_errors (or the next call that waits)
+int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipStream_t st, bool sync = true);
+int sync_errors(Ctx &c, hipStream_t st);
 int launch_energy_jvp(Ctx &c, const double *d_x, const double *d_w, double *d_out, hipStream_t st);
 // mdy_quad.hip: quadrature mode (MDY_COPLANAR_QUADRATURE)
 bool quad_on(const Ctx &c);

@@ -78,6 +78,8 @@ def load_library(path: str = LIB_PATH):
         "mdy_forces_dev": (C.c_int, [vp, vp, vp, C.c_int, vp]),
         "mdy_energy_dev_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "mdy_hvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mdy_hvp_dev_async": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mdy_sync_errors": (C.c_int, [vp, vp]),
         "mdy_energy_jvp_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "mdy_quadrature_pair_count": (C.c_int, [vp, f64p, i32p]),
         "mdy_minimize": (C.c_int, [vp, C.POINTER(_abi.CGParams), f64p, f64p, C.POINTER(_abi.MinResult)]),
@@ -344,6 +346,16 @@ class Context:
         """H v of the exact-gradient force field at the device coordinates."""
         self._check(self.lib.mdy_hvp_dev(self.h, C.c_void_p(d_coord), C.c_void_p(d_v), C.c_void_p(d_hv),
                                          C.c_void_p(stream) if stream else None))
+
+    def hvp_dev_async(self, d_coord: int, d_v: int, d_hv: int, stream: int = 0):
+        """H v enqueued without waiting; a device failure is raised by
+        sync_errors (or the next call that waits)."""
+        self._check(self.lib.mdy_hvp_dev_async(self.h, C.c_void_p(d_coord), C.c_void_p(d_v), C.c_void_p(d_hv),
+                                               C.c_void_p(stream) if stream else None))
+
+    def sync_errors(self, stream: int = 0):
+        """Waits for the stream and raises the failure an asynchronous call left."""
+        self._check(self.lib.mdy_sync_errors(self.h, C.c_void_p(stream) if stream else None))
 
     def energy_jvp_dev(self, d_coord: int, d_w: int, d_out: int, stream: int = 0):
         """d_out[k] = grad E_k . w per force field, then the sum (MDY_NFF + 1 doubles)."""

@@ -221,3 +221,52 @@ def test_device_calls_follow_torch_stream(lib, side_stream):
             assert np.array_equal(f, ctx.forces(xh)), k
     torch.cuda.synchronize()
     ctx.close()
+
+
+def test_async_hvp_equals_waiting_hvp(lib):
+    """mdy_hvp_dev_async enqueues the same kernels as mdy_hvp_dev without
+    waiting: products bit for bit equal, no failure left to report."""
+    net = network.generate(lib, "net500")
+    ctx = mechanics.Context(0, coplanar_mode=_abi.MDY_COPLANAR_EXACT)
+    ctx.set_network(net)
+    ctx.build_neighbors()
+    x = torch.tensor(net.coord, device="cuda")
+    g = torch.Generator("cuda").manual_seed(5)
+    vs = [torch.randn(x.shape, dtype=torch.float64, device="cuda", generator=g) for _ in range(4)]
+    want = []
+    for v in vs:
+        hv = torch.empty_like(v)
+        ctx.hvp_dev(x.data_ptr(), v.data_ptr(), hv.data_ptr())
+        want.append(hv.cpu().numpy())
+    outs = [torch.empty_like(v) for v in vs]
+    for v, hv in zip(vs, outs):
+        ctx.hvp_dev_async(x.data_ptr(), v.data_ptr(), hv.data_ptr())
+    ctx.sync_errors()
+    for w, hv in zip(want, outs):
+        assert np.array_equal(hv.cpu().numpy().view(np.int64), w.view(np.int64))
+    ctx.close()
+
+
+def test_async_hvp_failure_reported_by_sync_errors(lib):
+    """Exactly parallel cylinders (the reference's 2filaments example) have
+    no analytic second derivative: the waiting call raises MDY_ERR_FORCE at
+    once, the asynchronous one leaves it for sync_errors, which clears it."""
+    import os
+    from medyan_amd import io
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    net = io.network_from_filament_file(os.path.join(golden, "filament_input_2filaments.txt"), 2000.0)
+    ctx = mechanics.Context(0, coplanar_mode=_abi.MDY_COPLANAR_EXACT)
+    ctx.set_network(net)
+    ctx.build_neighbors()
+    x = torch.tensor(net.coord, device="cuda")
+    v = torch.ones_like(x)
+    hv = torch.empty_like(x)
+    with pytest.raises(mechanics.MdyError) as e:
+        ctx.hvp_dev(x.data_ptr(), v.data_ptr(), hv.data_ptr())
+    assert e.value.code == _abi.MDY_ERR_FORCE
+    ctx.hvp_dev_async(x.data_ptr(), v.data_ptr(), hv.data_ptr())
+    with pytest.raises(mechanics.MdyError) as e:
+        ctx.sync_errors()
+    assert e.value.code == _abi.MDY_ERR_FORCE
+    ctx.sync_errors()   # cleared
+    ctx.close()
