@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_hvp(Dev D, const double *__restri
 __global__ void __launch_bounds__(kBlock) k_hvp_gather(Dev D, const double *__restrict__ contrib,
                                                         double *__restrict__ hv) {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < D.nb; b += gridDim.x * blockDim.x) {
-        const d3 h = gather_bead(D, contrib, b);
+        const d3 h = gather_bead_w<12, 24>(D, contrib, b);   // one bead per thread: the wide form
         hv[3 * b] = h.x; hv[3 * b + 1] = h.y; hv[3 * b + 2] = h.z;
     }
 }
