@@ -135,11 +135,12 @@ __global__ void __launch_bounds__(kBlock) k_force_simpson_final(Dev D, const dou
     }
 }
 
-template <bool MIXED>
+// WIDE: one bead per thread (the wide per-bead form); same sums either way
+template <bool MIXED, bool WIDE>
 __global__ void __launch_bounds__(kBlock) k_gather(Dev D, const double *contrib_, double *force, int accumulate) {
     const auto *contrib = reinterpret_cast<const typename SlotT<MIXED>::T *>(contrib_);
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < D.nb; b += gridDim.x * blockDim.x) {
-        d3 f = gather_bead(D, contrib, b);
+        d3 f = WIDE ? gather_bead_w<12, 24>(D, contrib, b) : gather_bead(D, contrib, b);
         if (accumulate) {
             force[3 * b] += f.x; force[3 * b + 1] += f.y; force[3 * b + 2] += f.z;
         } else {
@@ -289,12 +290,11 @@ int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hi
             if (moved || !fl[1]) return MDY_ERR_FORCE;  // the reference exits here
         }
     }
-    if (mixed)
-        hipLaunchKernelGGL(k_gather<true>, dim3(grid_for(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_force,
-                           accumulate);
-    else
-        hipLaunchKernelGGL(k_gather<false>, dim3(grid_for(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_force,
-                           accumulate);
+    const int gg = grid_for(c, D.nb);
+    const bool wide = (long long)gg * kBlock >= D.nb;
+    auto kern = mixed ? (wide ? k_gather<true, true> : k_gather<true, false>)
+                      : (wide ? k_gather<false, true> : k_gather<false, false>);
+    hipLaunchKernelGGL(kern, dim3(gg), dim3(kBlock), 0, st, D, c.contrib.p, d_force, accumulate);
     return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
 }
 
