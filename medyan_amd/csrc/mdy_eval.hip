@@ -8,9 +8,11 @@
 
 namespace mdy {
 
+// Three waves per SIMD (168 registers, ~5 words per lane spilled in one
+// instantiation): 7-8 % faster than two at 40k and 200k cylinders.
 // energies: per-FF double sums at [0, kNFF), exact total at kXsE
 template <class P, bool QUAD = false>
-__global__ void __launch_bounds__(kBlock) k_energy(Dev D, P pos, double *red) {
+__global__ void __launch_bounds__(kBlock, 3) k_energy(Dev D, P pos, double *red) {
     __shared__ double lds[17 * kNFF + 8];
     double acc[kNFF];
 #pragma unroll
