@@ -137,7 +137,9 @@ __global__ void __launch_bounds__(kBlock) k_force_simpson_final(Dev D, const dou
     }
 }
 
-// WIDE: one bead per thread (the wide per-bead form); same sums either way
+// WIDE: the wide per-bead form (fewer beads than one block per CU has
+// threads: at 200k cylinders the narrow form's occupancy wins); same sums
+// either way
 template <bool MIXED, bool WIDE>
 __global__ void __launch_bounds__(kBlock) k_gather(Dev D, const double *contrib_, double *force, int accumulate) {
     const auto *contrib = reinterpret_cast<const typename SlotT<MIXED>::T *>(contrib_);
@@ -293,7 +295,7 @@ int launch_forces(Ctx &c, const double *d_x, double *d_force, int accumulate, hi
         }
     }
     const int gg = grid_for(c, D.nb);
-    const bool wide = (long long)gg * kBlock >= D.nb;
+    const bool wide = (long long)D.nb <= (long long)c.num_cu * kBlock;
     auto kern = mixed ? (wide ? k_gather<true, true> : k_gather<true, false>)
                       : (wide ? k_gather<false, true> : k_gather<false, false>);
     hipLaunchKernelGGL(kern, dim3(gg), dim3(kBlock), 0, st, D, c.contrib.p, d_force, accumulate);

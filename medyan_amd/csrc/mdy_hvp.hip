@@ -106,10 +106,13 @@ __global__ void __launch_bounds__(kBlock, 2) k_hvp(Dev D, const double *__restri
     }
 }
 
+// WIDE: the wide per-bead form (fewer beads than one block per CU has
+// threads); same sums either way
+template <bool WIDE>
 __global__ void __launch_bounds__(kBlock) k_hvp_gather(Dev D, const double *__restrict__ contrib,
                                                         double *__restrict__ hv) {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < D.nb; b += gridDim.x * blockDim.x) {
-        const d3 h = gather_bead_w<12, 24>(D, contrib, b);   // one bead per thread: the wide form
+        const d3 h = WIDE ? gather_bead_w<12, 24>(D, contrib, b) : gather_bead(D, contrib, b);
         hv[3 * b] = h.x; hv[3 * b + 1] = h.y; hv[3 * b + 2] = h.z;
     }
 }
@@ -192,7 +195,8 @@ int launch_hvp(Ctx &c, const double *d_x, const double *d_v, double *d_hv, hipSt
     if (quad) MDY_TRY(quad_classify(c, D, d_x, st));
     hipLaunchKernelGGL(k_hvp, dim3(grid_of(c, D.n_items())), dim3(kBlock), 0, st, D, d_x, d_v, c.contrib.p);
     if (quad) MDY_TRY(launch_quad(c, D, 2, d_x, d_v, c.contrib.p, 0, st));
-    hipLaunchKernelGGL(k_hvp_gather, dim3(grid_of(c, D.nb)), dim3(kBlock), 0, st, D, c.contrib.p, d_hv);
+    hipLaunchKernelGGL(D.nb <= c.num_cu * kBlock ? k_hvp_gather<true> : k_hvp_gather<false>, dim3(grid_of(c, D.nb)),
+                       dim3(kBlock), 0, st, D, c.contrib.p, d_hv);
     if (!sync) return hipGetLastError() == hipSuccess ? MDY_OK : MDY_ERR_HIP;
     return sync_errors(c, st);
 }
